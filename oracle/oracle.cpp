@@ -1,0 +1,1544 @@
+/*
+ * oracle.cpp -- TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * A C++ restatement of BG_Flood's CPU step FlowCPU<T> for engine 2
+ * (Kurganov-Petrova), with the helpers it needs.  It follows the CPU
+ * semantics of the reference everywhere (the reference's own CPU and CUDA
+ * paths disagree in a few places; the CPU column is the oracle).  Arithmetic
+ * order, the precision of each intermediate (some reference expressions mix T
+ * and double) and the sequential block order are kept, so that the FP64 and
+ * FP32 instantiations reproduce the reference's CPU results operation by
+ * operation.  Build with -ffp-contract=off (the reference CPU build contracts
+ * nothing: its host compiler targets baseline x86-64).
+ *
+ * Reference paths are relative to the BG_Flood source tree.
+ *
+ * Parity pins (tests/test_oracle_*.py):
+ *   memloc, calcres, ftoi ............ closed forms, brute-force tables
+ *   minmod2 .......................... exhaustive sign/order cases vs textbook limiter
+ *   KurgSolver ....................... closed forms (rest state, supercritical upwind,
+ *                                      dry state, mirror symmetry)
+ *   updateKurgX/Y + gradients + halo . lake at rest over rough beds (fluxes balance),
+ *                                      Gaussian hump transect vs the reference's
+ *                                      stored Kurganov values (src/Testing.cu:434),
+ *                                      D4 symmetry of the hump solution
+ *   updateEV / Advkernel ............. closed forms on planted fluxes; mass balance with
+ *                                      ForceMassConserve and wall boundaries
+ *   dt reduction ..................... reductiontest semantics (src/Testing.cu:1204-1292)
+ *   bottom friction .................. closed form u/(1+tb) for the three friction laws
+ *   rain ............................. volume added = R*dt*area over active cells
+ *   AMR halo fill / restriction /
+ *   prolongation / flux restriction .. constant fields stay constant, linear fields are
+ *                                      reproduced by refine_linear, coarse-face flux
+ *                                      equals the fine-face mean, lake at rest across
+ *                                      levels
+ */
+#include "oracle.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+constexpr double kPi = 3.14159265;  // src/General.h:6
+constexpr int BW = 16;              // blkwidth (src/Param.h:52)
+constexpr int HW = 1;               // halowidth (src/Param.h:55)
+constexpr int MW = BW + 2 * HW;     // blkmemwidth
+constexpr int BS = MW * MW;         // blksize
+
+// src/MemManagement.cu:37-40
+inline int memloc(int i, int j, int ib) { return (i + HW) + (j + HW) * MW + ib * BS; }
+
+// src/Util_CPU.cu:194-198
+template <class T> T calcres(T dx, int level) {
+    return level < 0 ? dx * (1 << std::abs(level)) : dx / (1 << level);
+}
+
+// src/Util_CPU.cu:308-311
+template <class T> int ftoi(T value) {
+    return (value >= 0 ? static_cast<int>(value + 0.5) : static_cast<int>(value - 0.5));
+}
+
+// src/Util_CPU.cu:29-46 (utils::sq, utils::max, utils::min)
+template <class T> inline T sq(T a) { return a * a; }
+template <class T> inline T umax(T a, T b) { return (a < b) ? b : a; }
+template <class T> inline T umin(T a, T b) { return !(b < a) ? a : b; }
+
+// src/Util_CPU.cu:243-263
+template <class T> T minmod2(T theta, T s0, T s1, T s2) {
+    if (s0 < s1 && s1 < s2) {
+        T d1 = theta * (s1 - s0);
+        T d2 = (s2 - s0) / T(2.0);
+        T d3 = theta * (s2 - s1);
+        if (d2 < d1) d1 = d2;
+        return std::min(d1, d3);
+    }
+    if (s0 > s1 && s1 > s2) {
+        T d1 = theta * (s1 - s0), d2 = (s2 - s0) / T(2.0), d3 = theta * (s2 - s1);
+        if (d2 > d1) d1 = d2;
+        return std::max(d1, d3);
+    }
+    return T(0.0);
+}
+
+// src/Util_CPU.cu:129-144
+template <class T> T BilinearInterpolation(T q11, T q12, T q21, T q22, T x1, T x2, T y1, T y2, T x, T y) {
+    T x2x1 = x2 - x1, y2y1 = y2 - y1, x2x = x2 - x, y2y = y2 - y, yy1 = y - y1, xx1 = x - x1;
+    return (T)1.0 / (x2x1 * y2y1) * (q11 * x2x * y2y + q21 * xx1 * y2y + q12 * x2x * yy1 + q22 * xx1 * yy1);
+}
+
+// src/Kurganov.cu:1865-1911
+template <class T> T KurgSolver(T g, T delta, T epsi, T CFL, T cm, T fm, T hp, T hm, T up, T um, T& fh, T& fu) {
+    T dt;
+    T cp, cmo, ap, am, qm, qp, a, dlt, ad, hm2, hp2, ga, apm;
+    cp = std::sqrt(g * hp);
+    cmo = std::sqrt(g * hm);
+    ap = std::max(std::max(up + cp, um + cmo), T(0.0));
+    am = std::min(std::min(up - cp, um - cmo), T(0.0));
+    ad = T(1.0) / (ap - am);
+    qm = hm * um;
+    qp = hp * up;
+    hm2 = hm * hm;
+    hp2 = hp * hp;
+    a = std::max(ap, -am);
+    ga = g * T(0.5);
+    apm = ap * am;
+    dlt = delta * cm / fm;
+    if (a > epsi) {
+        fh = (ap * qm - am * qp + apm * (hp - hm)) * ad;
+        fu = (ap * (qm * um + ga * hm2) - am * (qp * up + ga * hp2) + apm * (qp - qm)) * ad;
+        dt = CFL * dlt / a;
+    } else {
+        fh = T(0.0);
+        fu = T(0.0);
+        dt = T(1.0) / epsi;
+    }
+    return dt;
+}
+
+struct Prm {
+    double g, eps, CFL, theta, cf, dx, delta, xo, yo, lat, zsinit, zsoffset, dtinit, mask;
+    int frictionmodel, aoibnd, ForceMassConserve, wetdryfix;
+    int nblk, nblkmem, initlevel, minlevel, maxlevel, nx, ny;
+    int rain_on;
+    float rain;
+    int nthreads;
+};
+
+template <class T> struct Blk {
+    std::vector<int> active, level, LeftBot, LeftTop, RightBot, RightTop, TopLeft, TopRight, BotLeft, BotRight;
+    std::vector<T> xo, yo;
+};
+
+template <class T> struct Ev { std::vector<T> zs, h, u, v; };
+
+struct Side { std::vector<int> blk; int isright, istop; };
+
+template <class T> struct Mdl {
+    Prm p;
+    Blk<T> b;
+    Ev<T> ev, evo;
+    std::vector<T> dzsdx, dhdx, dudx, dvdx, dzsdy, dhdy, dudy, dvdy, dzbdx, dzbdy;
+    std::vector<T> Su, Sv, Fqux, Fquy, Fqvx, Fqvy, Fhu, Fhv;
+    std::vector<T> dh, dhu, dhv;
+    std::vector<T> zb, cf, dtmax;
+    std::vector<int> activeCell;
+    int bndtype;                 // type of the remainder segment (aoibnd)
+    Side left, right, top, bot;  // boundary block lists of the remainder segment
+};
+
+// ---------------------------------------------------------------- halo fill
+// src/Halo.cu:2083-2286
+template <class T> void fillLeft(const Prm& P, int ib, const Blk<T>& B, T* z) {
+    (void)P;
+    int jj, bb, read, write, ii, ir, it, itr;
+    const int* lev = B.level.data();
+    if (B.LeftBot[ib] == ib) {
+        for (int j = 0; j < BW / 2; j++) { read = memloc(0, j, ib); write = memloc(-1, j, ib); z[write] = z[read]; }
+        if (B.LeftTop[ib] == ib) {
+            for (int j = BW / 2; j < BW; j++) { read = memloc(0, j, ib); write = memloc(-1, j, ib); z[write] = z[read]; }
+        } else {
+            for (int j = BW / 2; j < BW; j++) {
+                write = memloc(-1, j, ib);
+                jj = (j - BW / 2) * 2;
+                ii = memloc(BW - 1, jj, B.LeftTop[ib]); ir = memloc(BW - 2, jj, B.LeftTop[ib]);
+                it = memloc(BW - 1, jj + 1, B.LeftTop[ib]); itr = memloc(BW - 2, jj + 1, B.LeftTop[ib]);
+                z[write] = T(0.25) * (z[ii] + z[ir] + z[it] + z[itr]);
+            }
+        }
+    } else if (lev[ib] == lev[B.LeftBot[ib]]) {
+        for (int j = 0; j < BW; j++) { write = memloc(-1, j, ib); read = memloc(BW - 1, j, B.LeftBot[ib]); z[write] = z[read]; }
+    } else if (lev[B.LeftBot[ib]] > lev[ib]) {
+        for (int j = 0; j < BW / 2; j++) {
+            write = memloc(-1, j, ib);
+            jj = j * 2; bb = B.LeftBot[ib];
+            ii = memloc(BW - 1, jj, bb); ir = memloc(BW - 2, jj, bb); it = memloc(BW - 1, jj + 1, bb); itr = memloc(BW - 2, jj + 1, bb);
+            z[write] = T(0.25) * (z[ii] + z[ir] + z[it] + z[itr]);
+        }
+        if (B.LeftTop[ib] == ib) {
+            for (int j = BW / 2; j < BW; j++) { read = memloc(0, j, ib); write = memloc(-1, j, ib); z[write] = z[read]; }
+        } else {
+            for (int j = BW / 2; j < BW; j++) {
+                jj = (j - BW / 2) * 2; bb = B.LeftTop[ib];
+                write = memloc(-1, j, ib);
+                ii = memloc(BW - 1, jj, bb); ir = memloc(BW - 2, jj, bb); it = memloc(BW - 1, jj + 1, bb); itr = memloc(BW - 2, jj + 1, bb);
+                z[write] = T(0.25) * (z[ii] + z[ir] + z[it] + z[itr]);
+            }
+        }
+    } else if (lev[B.LeftBot[ib]] < lev[ib]) {
+        const int LB = B.LeftBot[ib];
+        for (int j = 0; j < BW; j++) {
+            write = memloc(-1, j, ib);
+            T w1, w2, w3;
+            int jjc = B.RightBot[LB] == ib ? ftoi(std::ceil(j * (T)0.5)) : ftoi(std::ceil(j * (T)0.5) + BW / 2);
+            w1 = T(1.0 / 3.0);
+            w2 = std::ceil(j * (T)0.5) * 2 > j ? T(1.0 / 6.0) : T(0.5);
+            w3 = std::ceil(j * (T)0.5) * 2 > j ? T(0.5) : T(1.0 / 6.0);
+            ii = memloc(0, j, ib);
+            ir = memloc(BW - 1, jjc, LB);
+            it = memloc(BW - 1, jjc - 1, LB);
+            if (B.RightBot[LB] == ib) {
+                if (j == 0) {
+                    const int n = B.BotRight[LB];
+                    if (n == LB) { w3 = T(0.5) * (T(1.0) - w1); w2 = w3; it = ir; }
+                    else if (lev[n] < lev[LB]) { w1 = T(4.0 / 10.0); w2 = T(5.0 / 10.0); w3 = T(1.0 / 10.0); it = memloc(BW - 1, BW - 1, n); }
+                    else if (lev[n] == lev[LB]) { it = memloc(BW - 1, BW - 1, n); }
+                    else if (lev[n] > lev[LB]) { w1 = T(1.0 / 4.0); w2 = T(1.0 / 2.0); w3 = T(1.0 / 4.0); it = memloc(BW - 1, BW - 1, n); }
+                }
+            } else {
+                if (j == BW - 1) {
+                    const int LT = B.LeftTop[ib];
+                    const int n = B.TopRight[LT];
+                    if (n == LT) { w3 = T(0.5 * (1.0 - w1)); w2 = w3; ir = it; }
+                    else if (lev[n] < lev[LT]) { w1 = T(4.0 / 10.0); w2 = T(1.0 / 10.0); w3 = T(5.0 / 10.0); ir = memloc(BW - 1, 0, n); }
+                    else if (lev[n] == lev[LT]) { ir = memloc(BW - 1, 0, n); }
+                    else if (lev[n] > lev[LT]) { w1 = T(1.0 / 4.0); w2 = T(1.0 / 2.0); w3 = T(1.0 / 4.0); ir = memloc(BW - 1, 0, n); }
+                }
+            }
+            z[write] = w1 * z[ii] + w2 * z[ir] + w3 * z[it];
+        }
+    }
+}
+
+// src/Halo.cu:2818-3036
+template <class T> void fillRight(const Prm& P, int ib, const Blk<T>& B, T* z) {
+    (void)P;
+    int jj, bb, read, write, ii, ir, it, itr;
+    const int* lev = B.level.data();
+    if (B.RightBot[ib] == ib) {
+        for (int j = 0; j < BW / 2; j++) { read = memloc(BW - 1, j, ib); write = memloc(BW, j, ib); z[write] = z[read]; }
+        if (B.RightTop[ib] == ib) {
+            for (int j = BW / 2; j < BW; j++) { read = memloc(BW - 1, j, ib); write = memloc(BW, j, ib); z[write] = z[read]; }
+        } else {
+            for (int j = BW / 2; j < BW; j++) {
+                write = memloc(BW, j, ib);
+                jj = (j - BW / 2) * 2;
+                ii = memloc(0, jj, B.RightTop[ib]); ir = memloc(1, jj, B.RightTop[ib]);
+                it = memloc(0, jj + 1, B.RightTop[ib]); itr = memloc(1, jj + 1, B.RightTop[ib]);
+                z[write] = T(0.25) * (z[ii] + z[ir] + z[it] + z[itr]);
+            }
+        }
+    } else if (lev[ib] == lev[B.RightBot[ib]]) {
+        for (int j = 0; j < BW; j++) { write = memloc(BW, j, ib); read = memloc(0, j, B.RightBot[ib]); z[write] = z[read]; }
+    } else if (lev[B.RightBot[ib]] > lev[ib]) {
+        for (int j = 0; j < BW / 2; j++) {
+            write = memloc(BW, j, ib);
+            jj = j * 2; bb = B.RightBot[ib];
+            ii = memloc(0, jj, bb); ir = memloc(1, jj, bb); it = memloc(0, jj + 1, bb); itr = memloc(1, jj + 1, bb);
+            z[write] = T(0.25) * (z[ii] + z[ir] + z[it] + z[itr]);
+        }
+        if (B.RightTop[ib] == ib) {
+            for (int j = BW / 2; j < BW; j++) { read = memloc(BW - 1, j, ib); write = memloc(BW, j, ib); z[write] = z[read]; }
+        } else {
+            for (int j = BW / 2; j < BW; j++) {
+                jj = (j - BW / 2) * 2; bb = B.RightTop[ib];
+                write = memloc(BW, j, ib);
+                ii = memloc(0, jj, bb); ir = memloc(1, jj, bb); it = memloc(0, jj + 1, bb); itr = memloc(1, jj + 1, bb);
+                z[write] = T(0.25) * (z[ii] + z[ir] + z[it] + z[itr]);
+            }
+        }
+    } else if (lev[B.RightBot[ib]] < lev[ib]) {
+        const int RB = B.RightBot[ib];
+        for (int j = 0; j < BW; j++) {
+            write = memloc(BW, j, ib);
+            T w1, w2, w3;
+            int jjc = B.LeftBot[RB] == ib ? ftoi(std::ceil(j * (T)0.5)) : ftoi(std::ceil(j * (T)0.5) + BW / 2);
+            w1 = T(1.0 / 3.0);
+            w2 = std::ceil(j * (T)0.5) * 2 > j ? T(1.0 / 6.0) : T(0.5);
+            w3 = std::ceil(j * (T)0.5) * 2 > j ? T(0.5) : T(1.0 / 6.0);
+            ii = memloc(BW - 1, j, ib);
+            ir = memloc(0, jjc, RB);
+            it = memloc(0, jjc - 1, RB);
+            if (B.LeftBot[RB] == ib) {
+                if (j == 0) {
+                    const int n = B.BotLeft[RB];
+                    if (n == RB) { w3 = T(0.5 * (1.0 - w1)); w2 = w3; it = ir; }
+                    else if (lev[n] < lev[RB]) { w1 = T(4.0 / 10.0); w2 = T(5.0 / 10.0); w3 = T(1.0 / 10.0); it = memloc(0, BW - 1, n); }
+                    else if (lev[n] == lev[RB]) { it = memloc(0, BW - 1, n); }
+                    else if (lev[n] > lev[RB]) { w1 = T(1.0 / 4.0); w2 = T(1.0 / 2.0); w3 = T(1.0 / 4.0); it = memloc(0, BW - 1, n); }
+                }
+            } else {
+                if (j == BW - 1) {
+                    const int RT = B.RightTop[ib];
+                    const int n = B.TopLeft[RT];
+                    if (n == RT) { w3 = T(0.5 * (1.0 - w1)); w2 = w3; ir = it; }
+                    else if (lev[n] < lev[RT]) { w1 = T(4.0 / 10.0); w2 = T(1.0 / 10.0); w3 = T(5.0 / 10.0); ir = memloc(0, 0, n); }
+                    else if (lev[n] == lev[RT]) { ir = memloc(0, 0, n); }
+                    else if (lev[n] > lev[RT]) { w1 = 1.0 / 4.0; w2 = 1.0 / 2.0; w3 = 1.0 / 4.0; ir = memloc(0, 0, n); }
+                }
+            }
+            z[write] = w1 * z[ii] + w2 * z[ir] + w3 * z[it];
+        }
+    }
+}
+
+// src/Halo.cu:3685-3900
+template <class T> void fillBot(const Prm& P, int ib, const Blk<T>& B, T* z) {
+    (void)P;
+    int jj, bb, read, write, ii, ir, it, itr;
+    const int* lev = B.level.data();
+    if (B.BotLeft[ib] == ib) {
+        for (int j = 0; j < BW / 2; j++) { read = memloc(j, 0, ib); write = memloc(j, -1, ib); z[write] = z[read]; }
+        if (B.BotRight[ib] == ib) {
+            for (int j = BW / 2; j < BW; j++) { read = memloc(j, 0, ib); write = memloc(j, -1, ib); z[write] = z[read]; }
+        } else {
+            for (int j = BW / 2; j < BW; j++) {
+                write = memloc(j, -1, ib);
+                jj = (j - 8) * 2;
+                ii = memloc(jj, BW - 1, B.BotRight[ib]); ir = memloc(jj, BW - 2, B.BotRight[ib]);
+                it = memloc(jj + 1, BW - 1, B.BotRight[ib]); itr = memloc(jj + 1, BW - 2, B.BotRight[ib]);
+                z[write] = T(0.25) * (z[ii] + z[ir] + z[it] + z[itr]);
+            }
+        }
+    } else if (lev[ib] == lev[B.BotLeft[ib]]) {
+        for (int j = 0; j < BW; j++) { write = memloc(j, -1, ib); read = memloc(j, BW - 1, B.BotLeft[ib]); z[write] = z[read]; }
+    } else if (lev[B.BotLeft[ib]] > lev[ib]) {
+        for (int j = 0; j < BW / 2; j++) {
+            write = memloc(j, -1, ib);
+            jj = j * 2; bb = B.BotLeft[ib];
+            ii = memloc(jj, BW - 1, bb); ir = memloc(jj, BW - 2, bb); it = memloc(jj + 1, BW - 1, bb); itr = memloc(jj + 1, BW - 2, bb);
+            z[write] = T(0.25) * (z[ii] + z[ir] + z[it] + z[itr]);
+        }
+        if (B.BotRight[ib] == ib) {
+            for (int j = BW / 2; j < BW; j++) { read = memloc(j, 0, ib); write = memloc(j, -1, ib); z[write] = z[read]; }
+        } else {
+            for (int j = BW / 2; j < BW; j++) {
+                jj = (j - 8) * 2; bb = B.BotRight[ib];
+                write = memloc(j, -1, ib);
+                ii = memloc(jj, BW - 1, bb); ir = memloc(jj, BW - 2, bb); it = memloc(jj + 1, BW - 1, bb); itr = memloc(jj + 1, BW - 2, bb);
+                z[write] = T(0.25) * (z[ii] + z[ir] + z[it] + z[itr]);
+            }
+        }
+    } else if (lev[B.BotLeft[ib]] < lev[ib]) {
+        const int BL = B.BotLeft[ib];
+        for (int j = 0; j < BW; j++) {
+            write = memloc(j, -1, ib);
+            T w1, w2, w3;
+            int jjc = B.TopLeft[BL] == ib ? ftoi(std::ceil(j * (T)0.5)) : ftoi(std::ceil(j * (T)0.5) + BW / 2);
+            w1 = T(1.0 / 3.0);
+            w2 = std::ceil(j * (T)0.5) * 2 > j ? T(1.0 / 6.0) : T(0.5);
+            w3 = std::ceil(j * (T)0.5) * 2 > j ? T(0.5) : T(1.0 / 6.0);
+            ii = memloc(j, 0, ib);
+            ir = memloc(jjc, BW - 1, BL);
+            it = memloc(jjc - 1, BW - 1, BL);
+            if (B.TopLeft[BL] == ib) {
+                if (j == 0) {
+                    const int n = B.LeftTop[BL];
+                    if (n == BL) { w3 = T(0.5 * (1.0 - w1)); w2 = w3; it = ir; }
+                    else if (lev[n] < lev[BL]) { w1 = T(4.0 / 10.0); w2 = T(5.0 / 10.0); w3 = T(1.0 / 10.0); it = memloc(BW - 1, BW - 1, n); }
+                    else if (lev[n] == lev[BL]) { it = memloc(BW - 1, BW - 1, n); }
+                    else if (lev[n] > lev[BL]) { w1 = T(1.0 / 4.0); w2 = T(1.0 / 2.0); w3 = T(1.0 / 4.0); it = memloc(BW - 1, BW - 1, n); }
+                }
+            } else {
+                if (j == BW - 1) {
+                    const int BR = B.BotRight[ib];
+                    const int n = B.RightTop[BR];
+                    if (n == BR) { w3 = T(0.5 * (1.0 - w1)); w2 = w3; ir = it; }
+                    else if (lev[n] < lev[BR]) { w1 = T(4.0 / 10.0); w2 = T(1.0 / 10.0); w3 = T(5.0 / 10.0); ir = memloc(0, BW - 1, n); }
+                    else if (lev[n] == lev[BR]) { ir = memloc(0, BW - 1, n); }
+                    else if (lev[n] > lev[BR]) { w1 = T(1.0 / 4.0); w2 = T(1.0 / 2.0); w3 = T(1.0 / 4.0); ir = memloc(0, BW - 1, n); }
+                }
+            }
+            z[write] = w1 * z[ii] + w2 * z[ir] + w3 * z[it];
+        }
+    }
+}
+
+// src/Halo.cu:4394-4609
+template <class T> void fillTop(const Prm& P, int ib, const Blk<T>& B, T* z) {
+    (void)P;
+    int jj, bb, read, write, ii, ir, it, itr;
+    const int* lev = B.level.data();
+    if (B.TopLeft[ib] == ib) {
+        for (int j = 0; j < BW / 2; j++) { read = memloc(j, BW - 1, ib); write = memloc(j, BW, ib); z[write] = z[read]; }
+        if (B.TopRight[ib] == ib) {
+            for (int j = BW / 2; j < BW; j++) { read = memloc(j, BW - 1, ib); write = memloc(j, BW, ib); z[write] = z[read]; }
+        } else {
+            for (int j = BW / 2; j < BW; j++) {
+                write = memloc(j, BW, ib);
+                jj = (j - BW / 2) * 2;
+                ii = memloc(jj, 0, B.TopRight[ib]); ir = memloc(jj, 1, B.TopRight[ib]);
+                it = memloc(jj + 1, 0, B.TopRight[ib]); itr = memloc(jj + 1, 1, B.TopRight[ib]);
+                z[write] = T(0.25) * (z[ii] + z[ir] + z[it] + z[itr]);
+            }
+        }
+    } else if (lev[ib] == lev[B.TopLeft[ib]]) {
+        for (int j = 0; j < BW; j++) { write = memloc(j, BW, ib); read = memloc(j, 0, B.TopLeft[ib]); z[write] = z[read]; }
+    } else if (lev[B.TopLeft[ib]] > lev[ib]) {
+        for (int j = 0; j < BW / 2; j++) {
+            write = memloc(j, BW, ib);
+            jj = j * 2; bb = B.TopLeft[ib];
+            ii = memloc(jj, 0, bb); ir = memloc(jj, 1, bb); it = memloc(jj + 1, 0, bb); itr = memloc(jj + 1, 1, bb);
+            z[write] = T(0.25) * (z[ii] + z[ir] + z[it] + z[itr]);
+        }
+        if (B.TopRight[ib] == ib) {
+            for (int j = BW / 2; j < BW; j++) { read = memloc(j, BW - 1, ib); write = memloc(j, BW, ib); z[write] = z[read]; }
+        } else {
+            for (int j = BW / 2; j < BW; j++) {
+                jj = (j - BW / 2) * 2; bb = B.TopRight[ib];
+                write = memloc(j, BW, ib);
+                ii = memloc(jj, 0, bb); ir = memloc(jj, 1, bb); it = memloc(jj + 1, 0, bb); itr = memloc(jj + 1, 1, bb);
+                z[write] = T(0.25) * (z[ii] + z[ir] + z[it] + z[itr]);
+            }
+        }
+    } else if (lev[B.TopLeft[ib]] < lev[ib]) {
+        const int TL = B.TopLeft[ib];
+        for (int j = 0; j < BW; j++) {
+            write = memloc(j, BW, ib);
+            T w1, w2, w3;
+            int jjc = B.BotLeft[TL] == ib ? ftoi(std::ceil(j * (T)0.5)) : ftoi(std::ceil(j * (T)0.5) + BW / 2);
+            w1 = T(1.0 / 3.0);
+            w2 = std::ceil(j * (T)0.5) * 2 > j ? T(1.0 / 6.0) : T(0.5);
+            w3 = std::ceil(j * (T)0.5) * 2 > j ? T(0.5) : T(1.0 / 6.0);
+            ii = memloc(j, BW - 1, ib);
+            ir = memloc(jjc, 0, TL);
+            it = memloc(jjc - 1, 0, TL);
+            if (B.BotLeft[TL] == ib) {
+                if (j == 0) {
+                    const int n = B.LeftBot[TL];
+                    if (n == TL) { w3 = T(0.5 * (1.0 - w1)); w2 = w3; it = ir; }
+                    else if (lev[n] < lev[TL]) { w1 = T(4.0 / 10.0); w2 = T(5.0 / 10.0); w3 = T(1.0 / 10.0); it = memloc(BW - 1, 0, n); }
+                    else if (lev[n] == lev[TL]) { it = memloc(BW - 1, 0, n); }
+                    else if (lev[n] > lev[TL]) { w1 = T(1.0 / 4.0); w2 = T(1.0 / 2.0); w3 = T(1.0 / 4.0); it = memloc(BW - 1, 0, n); }
+                }
+            } else {
+                if (j == BW - 1) {
+                    const int TR = B.TopRight[ib];
+                    const int n = B.RightBot[TR];
+                    if (n == TR) { w3 = T(0.5 * (1.0 - w1)); w2 = w3; ir = it; }
+                    else if (lev[n] < lev[TR]) { w1 = T(4.0 / 10.0); w2 = T(1.0 / 10.0); w3 = T(5.0 / 10.0); ir = memloc(0, 0, n); }
+                    else if (lev[n] == lev[TR]) { ir = memloc(0, 0, n); }
+                    else if (lev[n] > lev[TR]) { w1 = T(1.0 / 4.0); w2 = T(1.0 / 2.0); w3 = T(1.0 / 4.0); ir = memloc(0, 0, n); }
+                }
+            }
+            z[write] = w1 * z[ii] + w2 * z[ir] + w3 * z[it];
+        }
+    }
+}
+
+// src/Halo.cu:16-51 (fillHaloD / fillHaloC)
+template <class T> void fillHaloC(const Prm& P, const Blk<T>& B, T* z) {
+    for (int ibl = 0; ibl < P.nblk; ibl++) {
+        int ib = B.active[ibl];
+        fillLeft(P, ib, B, z);
+        fillRight(P, ib, B, z);
+        fillTop(P, ib, B, z);
+        fillBot(P, ib, B, z);
+    }
+}
+
+// src/ConserveElevation.cu:676-726
+template <class T> void wetdryrestriction(T eps, int ib, int ibn, int ihalo, int jhalo, int i, int j, T* h, T* zb) {
+    int write = memloc(ihalo, jhalo, ib);
+    int ii = memloc(i, j, ibn), ir = memloc(i + 1, j, ibn), it = memloc(i, j + 1, ibn), itr = memloc(i + 1, j + 1, ibn);
+    T hii = h[ii], hir = h[ir], hit = h[it], hitr = h[itr];
+    T zbw = zb[write];
+    T iiwet = hii > eps ? T(1.0) : T(0.0);
+    T irwet = hir > eps ? T(1.0) : T(0.0);
+    T itwet = hit > eps ? T(1.0) : T(0.0);
+    T itrwet = hitr > eps ? T(1.0) : T(0.0);
+    T cwet = (iiwet + irwet + itwet + itrwet);
+    T hwet = (iiwet * hii + irwet * hir + itwet * hit + itrwet * hitr);
+    T zswet = (iiwet * hii) * (zb[ii] + h[ii]) + (irwet * hir) * (zb[ir] + h[ir]) + (itwet * hit) * (zb[it] + h[it]) + (itrwet * hitr) * (zb[itr] + h[itr]);
+    if (cwet > T(0.0) && cwet < T(4.0)) {
+        zswet = zswet / hwet;
+        hwet = umax(T(0.0), zswet - zbw);
+        h[write] = hwet;
+    }
+}
+
+// src/ConserveElevation.cu:218-378
+template <class T> void WetDryRestriction(const Prm& P, const Blk<T>& B, Ev<T>& E, T* zb) {
+    const T eps = T(P.eps);
+    for (int ibl = 0; ibl < P.nblk; ibl++) {
+        int ib = B.active[ibl];
+        int lev = B.level[ib];
+        int ihalo, jhalo, ir, jr;
+        ir = BW - 2; ihalo = -1;
+        if (lev < B.level[B.LeftBot[ib]])
+            for (int iy = 0; iy < BW / 2; iy++) { jhalo = iy; jr = iy * 2; wetdryrestriction(eps, ib, B.LeftBot[ib], ihalo, jhalo, ir, jr, E.h.data(), zb); }
+        if (lev < B.level[B.LeftTop[ib]])
+            for (int iy = BW / 2; iy < BW; iy++) { jhalo = iy; jr = (iy - BW / 2) * 2; wetdryrestriction(eps, ib, B.LeftTop[ib], ihalo, jhalo, ir, jr, E.h.data(), zb); }
+        ihalo = BW; ir = 0;
+        if (lev < B.level[B.RightBot[ib]])
+            for (int iy = 0; iy < BW / 2; iy++) { jhalo = iy; jr = iy * 2; wetdryrestriction(eps, ib, B.RightBot[ib], ihalo, jhalo, ir, jr, E.h.data(), zb); }
+        if (lev < B.level[B.RightTop[ib]])
+            for (int iy = BW / 2; iy < BW; iy++) { jhalo = iy; jr = (iy - BW / 2) * 2; wetdryrestriction(eps, ib, B.RightTop[ib], ihalo, jhalo, ir, jr, E.h.data(), zb); }
+        jhalo = BW; jr = 0;
+        if (lev < B.level[B.TopLeft[ib]])
+            for (int ix = 0; ix < BW / 2; ix++) { ihalo = ix; ir = ix * 2; wetdryrestriction(eps, ib, B.TopLeft[ib], ihalo, jhalo, ir, jr, E.h.data(), zb); }
+        if (lev < B.level[B.TopRight[ib]])
+            for (int ix = BW / 2; ix < BW; ix++) { ihalo = ix; ir = (ix - BW / 2) * 2; wetdryrestriction(eps, ib, B.TopRight[ib], ihalo, jhalo, ir, jr, E.h.data(), zb); }
+        jhalo = -1; jr = BW - 2;
+        if (lev < B.level[B.BotLeft[ib]])
+            for (int ix = 0; ix < BW / 2; ix++) { ihalo = ix; ir = ix * 2; wetdryrestriction(eps, ib, B.BotLeft[ib], ihalo, jhalo, ir, jr, E.h.data(), zb); }
+        if (lev < B.level[B.BotRight[ib]])
+            for (int ix = BW / 2; ix < BW; ix++) { ihalo = ix; ir = (ix - BW / 2) * 2; wetdryrestriction(eps, ib, B.BotRight[ib], ihalo, jhalo, ir, jr, E.h.data(), zb); }
+    }
+}
+
+// src/Halo.cu:72-107
+template <class T> void RecalculateZs(const Prm& P, const Blk<T>& B, Ev<T>& E, const T* zb) {
+    for (int ibl = 0; ibl < P.nblk; ibl++) {
+        int ib = B.active[ibl];
+        for (int j = -1; j < BW + 1; j++)
+            for (int i = -1; i < BW + 1; i++) { int n = memloc(i, j, ib); E.zs[n] = zb[n] + E.h[n]; }
+    }
+}
+
+template <class F> void run_parallel(int nthreads, std::vector<F>& jobs) {
+    if (nthreads <= 1) { for (auto& f : jobs) f(); return; }
+    std::vector<std::thread> th;
+    for (auto& f : jobs) th.emplace_back(f);
+    for (auto& t : th) t.join();
+}
+
+// src/Halo.cu:586-616 (fillHalo(EvolvingP, zb)); maskbnd is a no-op without masked blocks
+template <class T> void fillHaloEv(Mdl<T>& M, Ev<T>& E) {
+    const Prm& P = M.p;
+    std::vector<std::function<void()>> jobs;
+    T* arrs[4] = {E.h.data(), E.zs.data(), E.u.data(), E.v.data()};
+    for (int k = 0; k < 4; k++) { T* z = arrs[k]; jobs.push_back([&P, &M, z]() { fillHaloC(P, M.b, z); }); }
+    run_parallel(P.nthreads, jobs);
+    if (P.wetdryfix) WetDryRestriction(P, M.b, E, M.zb.data());
+    RecalculateZs(P, M.b, E, M.zb.data());
+}
+
+// --------------------------------------------------------------- gradients
+// src/Gradients.cu:921-957
+template <class T> void gradientC(const Prm& P, const Blk<T>& B, const T* a, T* dadx, T* dady) {
+    for (int ibl = 0; ibl < P.nblk; ibl++) {
+        int ib = B.active[ibl];
+        T delta = calcres(T(P.delta), B.level[ib]);
+        for (int iy = 0; iy < BW; iy++)
+            for (int ix = 0; ix < BW; ix++) {
+                int i = memloc(ix, iy, ib);
+                int xplus = memloc(ix + 1, iy, ib), xminus = memloc(ix - 1, iy, ib);
+                int yplus = memloc(ix, iy + 1, ib), yminus = memloc(ix, iy - 1, ib);
+                dadx[i] = minmod2(T(P.theta), a[xminus], a[i], a[xplus]) / delta;
+                dady[i] = minmod2(T(P.theta), a[yminus], a[i], a[yplus]) / delta;
+            }
+    }
+}
+
+// src/Gradients.cu:2713-2843
+template <class T> void gradientHaloLeft(const Prm& P, const Blk<T>& B, int ib, int iy, const T* a, T* dadx) {
+    int ii, ir, it, itr, jj, read;
+    int i = memloc(-1, iy, ib);
+    T aright = a[memloc(0, iy, ib)];
+    T aleft = T(0);
+    T delta = calcres(T(P.delta), B.level[ib]);
+    const int* lev = B.level.data();
+    if (B.LeftBot[ib] == ib) {
+        if (iy < BW / 2) { read = memloc(0, iy, ib); aleft = a[read]; }
+        if (B.LeftTop[ib] == ib) {
+            if (iy >= BW / 2) { read = memloc(0, iy, ib); aleft = a[read]; }
+        } else if (iy >= BW / 2) {
+            jj = (iy - BW / 2) * 2; int n = B.LeftTop[ib];
+            ii = memloc(BW - 3, jj, n); ir = memloc(BW - 4, jj, n); it = memloc(BW - 3, jj + 1, n); itr = memloc(BW - 4, jj + 1, n);
+            aleft = T(0.25) * (a[ii] + a[ir] + a[it] + a[itr]);
+        }
+    } else if (lev[ib] == lev[B.LeftBot[ib]]) {
+        read = memloc(BW - 2, iy, B.LeftBot[ib]); aleft = a[read];
+    } else if (lev[B.LeftBot[ib]] > lev[ib]) {
+        if (iy < BW / 2) {
+            jj = iy * 2; int bb = B.LeftBot[ib];
+            ii = memloc(BW - 3, jj, bb); ir = memloc(BW - 4, jj, bb); it = memloc(BW - 3, jj + 1, bb); itr = memloc(BW - 4, jj + 1, bb);
+            aleft = T(0.25) * (a[ii] + a[ir] + a[it] + a[itr]);
+        }
+        if (B.LeftTop[ib] == ib) {
+            if (iy >= BW / 2) { read = memloc(0, iy, ib); aleft = a[read]; }
+        } else if (iy >= BW / 2) {
+            jj = (iy - BW / 2) * 2; int n = B.LeftTop[ib];
+            ii = memloc(BW - 3, jj, n); ir = memloc(BW - 4, jj, n); it = memloc(BW - 3, jj + 1, n); itr = memloc(BW - 4, jj + 1, n);
+            aleft = T(0.25) * (a[ii] + a[ir] + a[it] + a[itr]);
+        }
+    } else if (lev[B.LeftBot[ib]] < lev[ib]) {
+        int LB = B.LeftBot[ib];
+        jj = B.RightBot[LB] == ib ? ftoi(std::ceil(iy * (T)0.5)) : ftoi(std::ceil(iy * (T)0.5) + BW / 2);
+        T jr = std::ceil(iy * (T)0.5) * 2 > iy ? T(0.25) : T(0.75);
+        ii = memloc(BW - 1, jj, LB); ir = memloc(BW - 2, jj, LB); it = memloc(BW - 1, jj - 1, LB); itr = memloc(BW - 2, jj - 1, LB);
+        aleft = BilinearInterpolation(a[itr], a[ir], a[it], a[ii], T(0.0), T(1.0), T(0.0), T(1.0), T(0.75), jr);
+    }
+    dadx[i] = minmod2(T(P.theta), aleft, a[i], aright) / delta;
+}
+
+// src/Gradients.cu:2858-2990
+template <class T> void gradientHaloRight(const Prm& P, const Blk<T>& B, int ib, int iy, const T* a, T* dadx) {
+    int ii, ir, it, itr, jj, read;
+    int i = memloc(BW, iy, ib);
+    T aleft = a[memloc(BW - 1, iy, ib)];
+    T aright = T(0);
+    T delta = calcres(T(P.delta), B.level[ib]);
+    const int* lev = B.level.data();
+    if (B.RightBot[ib] == ib) {
+        if (iy < BW / 2) { read = memloc(BW - 1, iy, ib); aright = a[read]; }
+        if (B.RightTop[ib] == ib) {
+            if (iy >= BW / 2) { read = memloc(BW - 1, iy, ib); aright = a[read]; }
+        } else if (iy >= BW / 2) {
+            jj = (iy - BW / 2) * 2; int n = B.RightTop[ib];
+            ii = memloc(3, jj, n); ir = memloc(2, jj, n); it = memloc(3, jj + 1, n); itr = memloc(2, jj + 1, n);
+            aright = T(0.25) * (a[ii] + a[ir] + a[it] + a[itr]);
+        }
+    } else if (lev[ib] == lev[B.RightBot[ib]]) {
+        read = memloc(1, iy, B.RightBot[ib]); aright = a[read];
+    } else if (lev[B.RightBot[ib]] > lev[ib]) {
+        if (iy < BW / 2) {
+            jj = iy * 2; int bb = B.RightBot[ib];
+            ii = memloc(3, jj, bb); ir = memloc(2, jj, bb); it = memloc(3, jj + 1, bb); itr = memloc(2, jj + 1, bb);
+            aright = T(0.25) * (a[ii] + a[ir] + a[it] + a[itr]);
+        }
+        if (B.RightTop[ib] == ib) {
+            if (iy >= BW / 2) { read = memloc(BW - 1, iy, ib); aright = a[read]; }
+        } else if (iy >= BW / 2) {
+            jj = (iy - BW / 2) * 2; int n = B.RightTop[ib];
+            ii = memloc(3, jj, n); ir = memloc(2, jj, n); it = memloc(3, jj + 1, n); itr = memloc(2, jj + 1, n);
+            aright = T(0.25) * (a[ii] + a[ir] + a[it] + a[itr]);
+        }
+    } else if (lev[B.RightBot[ib]] < lev[ib]) {
+        int RB = B.RightBot[ib];
+        jj = B.LeftBot[RB] == ib ? ftoi(std::ceil(iy * (T)0.5)) : ftoi(std::ceil(iy * (T)0.5) + BW / 2);
+        T jr = std::ceil(iy * (T)0.5) * 2 > iy ? T(0.25) : T(0.75);
+        ii = memloc(0, jj, RB); ir = memloc(1, jj, RB); it = memloc(0, jj - 1, RB); itr = memloc(1, jj - 1, RB);
+        aright = BilinearInterpolation(a[it], a[ii], a[itr], a[ir], T(0.0), T(1.0), T(0.0), T(1.0), T(0.25), jr);
+    }
+    dadx[i] = minmod2(T(P.theta), aleft, a[i], aright) / delta;
+}
+
+// src/Gradients.cu:3002-3145
+template <class T> void gradientHaloBot(const Prm& P, const Blk<T>& B, int ib, int ix, const T* a, T* dady) {
+    int ii, ir, it, itr, jj, read;
+    int i = memloc(ix, -1, ib);
+    T atop = a[memloc(ix, 0, ib)];
+    T abot = T(0);
+    T delta = calcres(T(P.delta), B.level[ib]);
+    const int* lev = B.level.data();
+    if (B.BotLeft[ib] == ib) {
+        if (ix < BW / 2) { read = memloc(ix, 0, ib); abot = a[read]; }
+        if (B.BotRight[ib] == ib) {
+            if (ix >= BW / 2) { read = memloc(ix, 0, ib); abot = a[read]; }
+        } else if (ix >= BW / 2) {
+            jj = (ix - BW / 2) * 2; int n = B.BotRight[ib];
+            ii = memloc(jj, BW - 3, n); ir = memloc(jj, BW - 4, n); it = memloc(jj + 1, BW - 3, n); itr = memloc(jj + 1, BW - 4, n);
+            abot = T(0.25) * (a[ii] + a[ir] + a[it] + a[itr]);
+        }
+    } else if (lev[ib] == lev[B.BotLeft[ib]]) {
+        read = memloc(ix, BW - 2, B.BotLeft[ib]); abot = a[read];
+    } else if (lev[B.BotLeft[ib]] > lev[ib]) {
+        if (ix < BW / 2) {
+            jj = ix * 2; int bb = B.BotLeft[ib];
+            ii = memloc(jj, BW - 3, bb); ir = memloc(jj, BW - 4, bb); it = memloc(jj + 1, BW - 3, bb); itr = memloc(jj + 1, BW - 4, bb);
+            abot = T(0.25) * (a[ii] + a[ir] + a[it] + a[itr]);
+        }
+        if (B.BotRight[ib] == ib) {
+            if (ix >= BW / 2) { read = memloc(ix, 0, ib); abot = a[read]; }
+        } else if (ix >= BW / 2) {
+            jj = (ix - BW / 2) * 2; int n = B.BotRight[ib];
+            ii = memloc(jj, BW - 3, n); ir = memloc(jj, BW - 4, n); it = memloc(jj + 1, BW - 3, n); itr = memloc(jj + 1, BW - 4, n);
+            abot = T(0.25) * (a[ii] + a[ir] + a[it] + a[itr]);
+        }
+    } else if (lev[B.BotLeft[ib]] < lev[ib]) {
+        int BL = B.BotLeft[ib];
+        jj = B.TopLeft[BL] == ib ? ftoi(std::ceil(ix * (T)0.5)) : ftoi(std::ceil(ix * (T)0.5) + BW / 2);
+        T jr = std::ceil(ix * (T)0.5) * 2 > ix ? T(0.25) : T(0.75);
+        ii = memloc(jj, BW - 1, BL); ir = memloc(jj, BW - 2, BL); it = memloc(jj - 1, BW - 1, BL); itr = memloc(jj - 1, BW - 2, BL);
+        abot = BilinearInterpolation(a[itr], a[it], a[ir], a[ii], T(0.0), T(1.0), T(0.0), T(1.0), jr, T(0.75));
+    }
+    dady[i] = minmod2(T(P.theta), abot, a[i], atop) / delta;
+}
+
+// src/Gradients.cu:3149-3294
+template <class T> void gradientHaloTop(const Prm& P, const Blk<T>& B, int ib, int ix, const T* a, T* dady) {
+    int ii, ir, it, itr, jj, read;
+    int i = memloc(ix, BW, ib);
+    T abot = a[memloc(ix, BW - 1, ib)];
+    T atop = T(0);
+    T delta = calcres(T(P.delta), B.level[ib]);
+    const int* lev = B.level.data();
+    if (B.TopLeft[ib] == ib) {
+        if (ix < BW / 2) { read = memloc(ix, BW - 1, ib); atop = a[read]; }
+        if (B.TopRight[ib] == ib) {
+            if (ix >= BW / 2) { read = memloc(ix, BW - 1, ib); atop = a[read]; }
+        } else if (ix >= BW / 2) {
+            jj = (ix - BW / 2) * 2; int n = B.TopRight[ib];
+            ii = memloc(jj, 3, n); ir = memloc(jj, 2, n); it = memloc(jj + 1, 3, n); itr = memloc(jj + 1, 2, n);
+            atop = T(0.25) * (a[ii] + a[ir] + a[it] + a[itr]);
+        }
+    } else if (lev[ib] == lev[B.TopLeft[ib]]) {
+        read = memloc(ix, 1, B.TopLeft[ib]); atop = a[read];
+    } else if (lev[B.TopLeft[ib]] > lev[ib]) {
+        if (ix < BW / 2) {
+            jj = ix * 2; int bb = B.TopLeft[ib];
+            ii = memloc(jj, 3, bb); ir = memloc(jj, 2, bb); it = memloc(jj + 1, 3, bb); itr = memloc(jj + 1, 2, bb);
+            atop = T(0.25) * (a[ii] + a[ir] + a[it] + a[itr]);
+        }
+        if (B.TopRight[ib] == ib) {
+            if (ix >= BW / 2) { read = memloc(ix, BW - 1, ib); atop = a[read]; }
+        } else if (ix >= BW / 2) {
+            jj = (ix - BW / 2) * 2; int n = B.TopRight[ib];
+            ii = memloc(jj, 3, n); ir = memloc(jj, 2, n); it = memloc(jj + 1, 3, n); itr = memloc(jj + 1, 2, n);
+            atop = T(0.25) * (a[ii] + a[ir] + a[it] + a[itr]);
+        }
+    } else if (lev[B.TopLeft[ib]] < lev[ib]) {
+        int TL = B.TopLeft[ib];
+        jj = B.BotLeft[TL] == ib ? ftoi(std::ceil(ix * (T)0.5)) : ftoi(std::ceil(ix * (T)0.5) + BW / 2);
+        T jr = std::ceil(ix * (T)0.5) * 2 > ix ? T(0.25) : T(0.75);
+        ii = memloc(jj, 0, TL); ir = memloc(jj, 1, TL); it = memloc(jj - 1, 0, TL); itr = memloc(jj - 1, 1, TL);
+        atop = BilinearInterpolation(a[it], a[itr], a[ii], a[ir], T(0.0), T(1.0), T(0.0), T(1.0), jr, T(0.25));
+    }
+    dady[i] = minmod2(T(P.theta), abot, a[i], atop) / delta;
+}
+
+// src/Gradients.cu:2584-2605
+template <class T> void gradientHalo(const Prm& P, const Blk<T>& B, const T* a, T* dadx, T* dady) {
+    for (int ibl = 0; ibl < P.nblk; ibl++) {
+        int ib = B.active[ibl];
+        for (int iy = 0; iy < BW; iy++) {
+            gradientHaloLeft(P, B, ib, iy, a, dadx);
+            gradientHaloRight(P, B, ib, iy, a, dadx);
+        }
+        for (int ix = 0; ix < BW; ix++) {
+            gradientHaloBot(P, B, ib, ix, a, dady);
+            gradientHaloTop(P, B, ib, ix, a, dady);
+        }
+    }
+}
+
+// src/Halo.cu:1044-1066, 1138-1160, 1227-1249, 1313-1335 (refine_linear_{Left,Right,Bot,Top})
+template <class T> void refine_linear(const Prm& P, const Blk<T>& B, T* z, const T* dzdx, const T* dzdy) {
+    const int* lev = B.level.data();
+    for (int ibl = 0; ibl < P.nblk; ibl++) {
+        int ib = B.active[ibl];
+        if (lev[B.LeftBot[ib]] < lev[ib]) {  // Left: ilevdx in double
+            double ilevdx = calcres(P.delta, lev[ib]) * T(0.5);
+            int LB = B.LeftBot[ib];
+            for (int j = 0; j < BW; j++) {
+                int jj = B.RightBot[LB] == ib ? ftoi(std::floor(j * (T)0.5)) : ftoi(std::floor(j * (T)0.5) + BW / 2);
+                int il = memloc(BW - 1, jj, LB);
+                int write = memloc(-1, j, ib);
+                T faclr = T(1.0);
+                T facbt = std::floor(j * (T)0.5) * T(2.0) < (j - T(0.01)) ? 1.0 : -1.0;
+                T newz = z[il] + (faclr * dzdx[il] + facbt * dzdy[il]) * ilevdx;
+                z[write] = newz;
+            }
+        }
+        if (lev[B.RightBot[ib]] < lev[ib]) {  // Right: ilevdx in T
+            T ilevdx = calcres(T(P.delta), lev[ib]) * T(0.5);
+            int RB = B.RightBot[ib];
+            for (int j = 0; j < BW; j++) {
+                int jj = B.LeftBot[RB] == ib ? ftoi(std::floor(j * (T)0.5)) : ftoi(std::floor(j * (T)0.5) + BW / 2);
+                int il = memloc(0, jj, RB);
+                int write = memloc(BW, j, ib);
+                T faclr = T(-1.0);
+                T facbt = std::floor(j * (T)0.5) * T(2.0) < (j - T(0.01)) ? 1.0 : -1.0;
+                T newz = z[il] + (faclr * dzdx[il] + facbt * dzdy[il]) * ilevdx;
+                z[write] = newz;
+            }
+        }
+        if (lev[B.TopLeft[ib]] < lev[ib]) {  // Top: ilevdx in double
+            double ilevdx = calcres(P.delta, lev[ib]) * T(0.5);
+            int TL = B.TopLeft[ib];
+            for (int i = 0; i < BW; i++) {
+                int ii = B.BotLeft[TL] == ib ? ftoi(std::floor(i * (T)0.5)) : ftoi(std::floor(i * (T)0.5) + BW / 2);
+                int jl = memloc(ii, 0, TL);
+                int write = memloc(i, BW, ib);
+                T facbt = T(-1.0);
+                T faclr = std::floor(i * (T)0.5) * T(2.0) < (i - T(0.01)) ? 1.0 : -1.0;
+                T newz = z[jl] + (faclr * dzdx[jl] + facbt * dzdy[jl]) * ilevdx;
+                z[write] = newz;
+            }
+        }
+        if (lev[B.BotLeft[ib]] < lev[ib]) {  // Bot: ilevdx in T
+            T ilevdx = calcres(T(P.delta), lev[ib]) * T(0.5);
+            int BL = B.BotLeft[ib];
+            for (int i = 0; i < BW; i++) {
+                int ii = B.TopLeft[BL] == ib ? ftoi(std::floor(i * (T)0.5)) : ftoi(std::floor(i * (T)0.5) + BW / 2);
+                int jl = memloc(ii, BW - 1, BL);
+                int write = memloc(i, -1, ib);
+                T facbt = T(1.0);
+                T faclr = std::floor(i * (T)0.5) * T(2.0) < (i - T(0.01)) ? 1.0 : -1.0;
+                T newz = z[jl] + (faclr * dzdx[jl] + facbt * dzdy[jl]) * ilevdx;
+                z[write] = newz;
+            }
+        }
+    }
+}
+
+// src/ConserveElevation.cu:451-482
+template <class T> void ProlongationElevation(T eps, int ib, int ibn, int ihalo, int jhalo, int ip, int jp, T* h, T* zs, T* zb) {
+    int halo = memloc(ihalo, jhalo, ib);
+    int ii = memloc(ip, jp, ibn), left = memloc(ip - 1, jp, ibn), right = memloc(ip + 1, jp, ibn);
+    int top = memloc(ip, jp + 1, ibn), bot = memloc(ip, jp - 1, ibn);
+    if (!(h[ii] > eps && h[left] > eps && h[right] > eps && h[top] > eps && h[bot] > eps)) {
+        h[halo] = h[ii];
+        zb[halo] = zb[ii];
+        zs[halo] = zs[ii];
+    }
+}
+
+// src/ConserveElevation.cu:80-205 (sequential block order matters: a block's halo
+// written here may be read by a later block's test)
+template <class T> void WetDryProlongation(const Prm& P, const Blk<T>& B, Ev<T>& E, T* zb) {
+    const T eps = T(P.eps);
+    const int* lev = B.level.data();
+    for (int ibl = 0; ibl < P.nblk; ibl++) {
+        int ib = B.active[ibl];
+        int ibLB = B.LeftBot[ib], ibRB = B.RightBot[ib], ibTL = B.TopLeft[ib], ibBL = B.BotLeft[ib];
+        if (lev[ib] > lev[ibLB])
+            for (int j = 0; j < BW; j++) {
+                int jp = B.RightBot[ibLB] == ib ? ftoi(std::floor(j * T(0.5))) : ftoi(std::floor(j * T(0.5)) + BW / 2);
+                ProlongationElevation(eps, ib, ibLB, -1, j, BW - 1, jp, E.h.data(), E.zs.data(), zb);
+            }
+        if (lev[ib] > lev[ibRB])
+            for (int j = 0; j < BW; j++) {
+                int jp = B.LeftBot[ibRB] == ib ? ftoi(std::floor(j * T(0.5))) : ftoi(std::floor(j * T(0.5)) + BW / 2);
+                ProlongationElevation(eps, ib, ibRB, BW, j, 0, jp, E.h.data(), E.zs.data(), zb);
+            }
+        if (lev[ib] > lev[ibTL])
+            for (int i = 0; i < BW; i++) {
+                int ip = B.BotLeft[ibTL] == ib ? ftoi(std::floor(i * T(0.5))) : ftoi(std::floor(i * T(0.5)) + BW / 2);
+                ProlongationElevation(eps, ib, ibTL, i, BW, ip, 0, E.h.data(), E.zs.data(), zb);
+            }
+        if (lev[ib] > lev[ibBL])
+            for (int i = 0; i < BW; i++) {
+                int ip = B.TopLeft[ibBL] == ib ? ftoi(std::floor(i * T(0.5))) : ftoi(std::floor(i * T(0.5)) + BW / 2);
+                ProlongationElevation(eps, ib, ibBL, i, -1, ip, BW - 1, E.h.data(), E.zs.data(), zb);
+            }
+    }
+}
+
+// src/Gradients.cu:971-1054 (engine 2: no Wetsloperest; conserveElevation=false)
+template <class T> void gradientCPU(Mdl<T>& M, Ev<T>& E) {
+    const Prm& P = M.p;
+    auto grads = [&]() {
+        std::vector<std::function<void()>> jobs;
+        jobs.push_back([&]() { gradientC(P, M.b, E.h.data(), M.dhdx.data(), M.dhdy.data()); });
+        jobs.push_back([&]() { gradientC(P, M.b, E.zs.data(), M.dzsdx.data(), M.dzsdy.data()); });
+        jobs.push_back([&]() { gradientC(P, M.b, E.u.data(), M.dudx.data(), M.dudy.data()); });
+        jobs.push_back([&]() { gradientC(P, M.b, E.v.data(), M.dvdx.data(), M.dvdy.data()); });
+        run_parallel(P.nthreads, jobs);
+        gradientHalo(P, M.b, E.h.data(), M.dhdx.data(), M.dhdy.data());
+        gradientHalo(P, M.b, E.zs.data(), M.dzsdx.data(), M.dzsdy.data());
+        gradientHalo(P, M.b, E.u.data(), M.dudx.data(), M.dudy.data());
+        gradientHalo(P, M.b, E.v.data(), M.dvdx.data(), M.dvdy.data());
+    };
+    grads();
+    refine_linear(P, M.b, E.h.data(), M.dhdx.data(), M.dhdy.data());
+    refine_linear(P, M.b, E.u.data(), M.dudx.data(), M.dudy.data());
+    refine_linear(P, M.b, E.v.data(), M.dvdx.data(), M.dvdy.data());
+    if (P.wetdryfix) WetDryProlongation(P, M.b, E, M.zb.data());
+    RecalculateZs(P, M.b, E, M.zb.data());
+    grads();
+}
+
+// ------------------------------------------------------------------ fluxes
+// src/Kurganov.cu:474-641
+template <class T> void updateKurgXCPU(Mdl<T>& M, const Ev<T>& E) {
+    const Prm& P = M.p;
+    const Blk<T>& B = M.b;
+    T g = T(P.g), CFL = T(P.CFL);
+    T epsi = std::nextafter(T(1.0), T(2.0)) - T(1.0);
+    T eps = T(P.eps) + epsi;
+    const T* zb = M.zb.data();
+    T* dtmax = M.dtmax.data();
+    for (int ibl = 0; ibl < P.nblk; ibl++) {
+        int ib = B.active[ibl];
+        int lev = B.level[ib];
+        T delta = calcres(T(P.delta), lev);
+        int RB = B.RightBot[ib], levRB = B.level[RB], LBRB = B.LeftBot[RB];
+        int LB = B.LeftBot[ib], levLB = B.level[LB], RBLB = B.RightBot[LB];
+        for (int iy = 0; iy < BW; iy++)
+            for (int ix = 0; ix < BW + HW; ix++) {
+                int i = memloc(ix, iy, ib), ileft = memloc(ix - 1, iy, ib);
+                T dhdxi = M.dhdx[i], dhdxmin = M.dhdx[ileft];
+                T cm = T(1.0), fmu = T(1.0);
+                T hi = E.h[i], hn = E.h[ileft];
+                if (hi > eps || hn > eps) {
+                    T dx, zi, zl, zn, zr, zlr, hl, up, hp, hr, um, hm, ga;
+                    dx = delta * T(0.5);
+                    zi = E.zs[i] - hi;
+                    zl = zi - dx * (M.dzsdx[i] - dhdxi);
+                    zn = E.zs[ileft] - hn;
+                    zr = zn + dx * (M.dzsdx[ileft] - dhdxmin);
+                    zlr = std::max(zl, zr);
+                    hl = hi - dx * dhdxi;
+                    up = E.u[i] - dx * M.dudx[i];
+                    hp = std::max(T(0.0), hl + zl - zlr);
+                    hr = hn + dx * dhdxmin;
+                    um = E.u[ileft] + dx * M.dudx[ileft];
+                    hm = std::max(T(0.0), hr + zr - zlr);
+                    ga = g * T(0.5);
+                    T fh, fu, fv, sl, sr, dt;
+                    dt = KurgSolver(g, delta, epsi, CFL, cm, fmu, hp, hm, up, um, fh, fu);
+                    if (dt < dtmax[i]) dtmax[i] = dt;
+                    if (fh > T(0.0)) fv = (E.v[ileft] + dx * M.dvdx[ileft]) * fh;
+                    else fv = (E.v[i] - dx * M.dvdx[i]) * fh;
+                    if ((ix == BW) && levRB < lev) {
+                        int jj = LBRB == ib ? ftoi(std::floor(iy * (T)0.5)) : ftoi(std::floor(iy * (T)0.5) + BW / 2);
+                        int iright = memloc(0, jj, RB);
+                        hi = E.h[iright]; zi = zb[iright];
+                    }
+                    if ((ix == 0) && levLB < lev) {
+                        int jj = RBLB == ib ? ftoi(std::floor(iy * (T)0.5)) : ftoi(std::floor(iy * (T)0.5) + BW / 2);
+                        int ilc = memloc(BW - 1, jj, LB);
+                        hn = E.h[ilc]; zn = zb[ilc];
+                    }
+                    sl = ga * (sq(hp) - sq(hl) + (hl + hi) * (zi - zl));
+                    sr = ga * (sq(hm) - sq(hr) + (hr + hn) * (zn - zr));
+                    M.Fhu[i] = fmu * fh;
+                    M.Fqux[i] = fmu * (fu - sl);
+                    M.Su[i] = fmu * (fu - sr);
+                    M.Fqvx[i] = fmu * fv;
+                } else {
+                    dtmax[i] = T(1.0) / epsi;
+                    M.Fhu[i] = T(0.0); M.Fqux[i] = T(0.0); M.Su[i] = T(0.0); M.Fqvx[i] = T(0.0);
+                }
+            }
+    }
+}
+
+// src/Kurganov.cu:1404-1546
+template <class T> void updateKurgYCPU(Mdl<T>& M, const Ev<T>& E) {
+    const Prm& P = M.p;
+    const Blk<T>& B = M.b;
+    T epsi = std::nextafter(T(1.0), T(2.0)) - T(1.0);
+    T eps = T(P.eps) + epsi;
+    T g = T(P.g), CFL = T(P.CFL);
+    const T* zb = M.zb.data();
+    T* dtmax = M.dtmax.data();
+    for (int ibl = 0; ibl < P.nblk; ibl++) {
+        int ib = B.active[ibl];
+        int TL = B.TopLeft[ib], levTL = B.level[TL], BLTL = B.BotLeft[TL];
+        int BL = B.BotLeft[ib], levBL = B.level[BL], TLBL = B.TopLeft[BL];
+        int lev = B.level[ib];
+        T delta = T(calcres(P.delta, lev));
+        for (int iy = 0; iy < BW + HW; iy++)
+            for (int ix = 0; ix < BW; ix++) {
+                int i = memloc(ix, iy, ib), ibot = memloc(ix, iy - 1, ib);
+                T cm = T(1.0), fmv = T(1.0);
+                T dhdyi = M.dhdy[i], dhdymin = M.dhdy[ibot];
+                T hi = E.h[i], hn = E.h[ibot];
+                T dx, zi, zl, zn, zr, zlr, hl, up, hp, hr, um, hm, ga;
+                if (hi > eps || hn > eps) {
+                    hn = E.h[ibot];
+                    dx = delta * T(0.5);
+                    zi = E.zs[i] - hi;
+                    zl = zi - dx * (M.dzsdy[i] - dhdyi);
+                    zn = E.zs[ibot] - hn;
+                    zr = zn + dx * (M.dzsdy[ibot] - dhdymin);
+                    zlr = std::max(zl, zr);
+                    hl = hi - dx * dhdyi;
+                    up = E.v[i] - dx * M.dvdy[i];
+                    hp = std::max(T(0.0), hl + zl - zlr);
+                    hr = hn + dx * dhdymin;
+                    um = E.v[ibot] + dx * M.dvdy[ibot];
+                    hm = std::max(T(0.0), hr + zr - zlr);
+                    ga = g * T(0.5);
+                    T fh, fu, fv, sl, sr, dt;
+                    dt = KurgSolver(g, delta, epsi, CFL, cm, fmv, hp, hm, up, um, fh, fu);
+                    if (dt < dtmax[i]) dtmax[i] = dt;
+                    if (fh > T(0.0)) fv = (E.u[ibot] + dx * M.dudy[ibot]) * fh;
+                    else fv = (E.u[i] - dx * M.dudy[i]) * fh;
+                    if ((iy == BW) && levTL < lev) {
+                        int jj = BLTL == ib ? ftoi(std::floor(ix * (T)0.5)) : ftoi(std::floor(ix * (T)0.5) + BW / 2);
+                        int itop = memloc(jj, 0, TL);
+                        hi = E.h[itop]; zi = zb[itop];
+                    }
+                    if ((iy == 0) && levBL < lev) {
+                        int jj = TLBL == ib ? ftoi(std::floor(ix * (T)0.5)) : ftoi(std::floor(ix * (T)0.5) + BW / 2);
+                        int ibc = memloc(jj, BW - 1, BL);
+                        hn = E.h[ibc]; zn = zb[ibc];
+                    }
+                    sl = ga * (sq(hp) - sq(hl) + (hl + hi) * (zi - zl));
+                    sr = ga * (sq(hm) - sq(hr) + (hr + hn) * (zn - zr));
+                    M.Fhv[i] = fmv * fh;
+                    M.Fqvy[i] = fmv * (fu - sl);
+                    M.Sv[i] = fmv * (fu - sr);
+                    M.Fquy[i] = fmv * fv;
+                } else {
+                    dtmax[i] = T(1.0) / epsi;
+                    M.Fhv[i] = T(0.0); M.Fqvy[i] = T(0.0); M.Sv[i] = T(0.0); M.Fquy[i] = T(0.0);
+                }
+            }
+    }
+}
+
+// src/Halo.cu:1441-1529
+template <class T> void HaloFluxCPULR(const Blk<T>& B, int ib, T* z) {
+    const int* lev = B.level.data();
+    int jj, i, il, itl;
+    if (lev[B.LeftBot[ib]] > lev[ib])
+        for (int j = 0; j < BW / 2; j++) { i = memloc(0, j, ib); jj = j * 2; il = memloc(BW, jj, B.LeftBot[ib]); itl = memloc(BW, jj + 1, B.LeftBot[ib]); z[i] = T(0.5) * (z[il] + z[itl]); }
+    if (lev[B.LeftTop[ib]] > lev[ib])
+        for (int j = BW / 2; j < BW; j++) { i = memloc(0, j, ib); jj = (j - BW / 2) * 2; il = memloc(BW, jj, B.LeftTop[ib]); itl = memloc(BW, jj + 1, B.LeftTop[ib]); z[i] = T(0.5) * (z[il] + z[itl]); }
+    if (lev[B.RightBot[ib]] > lev[ib])
+        for (int j = 0; j < BW / 2; j++) { i = memloc(BW, j, ib); jj = j * 2; il = memloc(0, jj, B.RightBot[ib]); itl = memloc(0, jj + 1, B.RightBot[ib]); z[i] = T(0.5) * (z[il] + z[itl]); }
+    if (lev[B.RightTop[ib]] > lev[ib])
+        for (int j = BW / 2; j < BW; j++) { jj = (j - BW / 2) * 2; i = memloc(BW, j, ib); il = memloc(0, jj, B.RightTop[ib]); itl = memloc(0, jj + 1, B.RightTop[ib]); z[i] = T(0.5) * (z[il] + z[itl]); }
+}
+
+// src/Halo.cu:1766-1862
+template <class T> void HaloFluxCPUBT(const Blk<T>& B, int ib, T* z) {
+    const int* lev = B.level.data();
+    int jj, i, il, itl;
+    if (lev[B.BotLeft[ib]] > lev[ib])
+        for (int j = 0; j < BW / 2; j++) { i = memloc(j, 0, ib); jj = j * 2; il = memloc(jj, BW, B.BotLeft[ib]); itl = memloc(jj + 1, BW, B.BotLeft[ib]); z[i] = T(0.5) * (z[il] + z[itl]); }
+    if (lev[B.BotRight[ib]] > lev[ib])
+        for (int j = BW / 2; j < BW; j++) { i = memloc(j, 0, ib); jj = (j - BW / 2) * 2; il = memloc(jj, BW, B.BotRight[ib]); itl = memloc(jj + 1, BW, B.BotRight[ib]); z[i] = T(0.5) * (z[il] + z[itl]); }
+    if (lev[B.TopLeft[ib]] > lev[ib])
+        for (int j = 0; j < BW / 2; j++) { i = memloc(j, BW, ib); jj = j * 2; il = memloc(jj, 0, B.TopLeft[ib]); itl = memloc(jj + 1, 0, B.TopLeft[ib]); z[i] = T(0.5) * (z[il] + z[itl]); }
+    if (lev[B.TopRight[ib]] > lev[ib])
+        for (int j = BW / 2; j < BW; j++) { jj = (j - BW / 2) * 2; i = memloc(j, BW, ib); il = memloc(jj, 0, B.TopRight[ib]); itl = memloc(jj + 1, 0, B.TopRight[ib]); z[i] = T(0.5) * (z[il] + z[itl]); }
+}
+
+// src/Halo.cu:849-884 and :376-427
+template <class T> void fillHaloFlux(Mdl<T>& M) {
+    const Prm& P = M.p;
+    std::vector<std::function<void()>> jobs;
+    T* lr[4] = {M.Fhu.data(), M.Fqux.data(), M.Su.data(), M.Fqvx.data()};
+    T* bt[4] = {M.Fhv.data(), M.Fqvy.data(), M.Sv.data(), M.Fquy.data()};
+    for (int k = 0; k < 4; k++) {
+        T* z = lr[k];
+        jobs.push_back([&M, &P, z]() { for (int ibl = 0; ibl < P.nblk; ibl++) HaloFluxCPULR(M.b, M.b.active[ibl], z); });
+        T* w = bt[k];
+        jobs.push_back([&M, &P, w]() { for (int ibl = 0; ibl < P.nblk; ibl++) HaloFluxCPUBT(M.b, M.b.active[ibl], w); });
+    }
+    run_parallel(P.nthreads, jobs);
+}
+
+// src/Boundary.cu:2183-2220
+int Inside(int isright, int istop, int ix, int iy, int ib) {
+    if (isright < 0) return memloc(ix + 1, iy, ib);
+    if (isright > 0) return memloc(ix - 1, iy, ib);
+    if (istop < 0) return memloc(ix, iy + 1, ib);
+    return memloc(ix, iy - 1, ib);
+}
+
+// src/Boundary.cu:587-760 restricted to type 0 (wall, noslipbndQ :1887-1891).
+// Type 1 (Neumann) segments are skipped by FlowbndFlux (src/Boundary.cu:154).
+template <class T> void bndFluxSideWall(const Side& side, T* Fh, T* Fq, T* Ss) {
+    for (size_t ibl = 0; ibl < side.blk.size(); ibl++) {
+        int ib = side.blk[ibl];
+        for (int tx = 0; tx < BW; tx++) {
+            int ix, iy;
+            if (side.isright == 0) { ix = tx; iy = side.istop < 0 ? 0 : BW; }
+            else { iy = tx; ix = side.isright < 0 ? 0 : BW; }
+            int i = memloc(ix, iy, ib);
+            int inside = Inside(side.isright, side.istop, ix, iy, ib);
+            T F, G, S;
+            if (side.isright < 0 || side.istop < 0) { F = Fh[i]; G = Fq[i]; S = Ss[inside]; }
+            else { F = Fh[i]; G = Ss[i]; S = Fq[inside]; }
+            F = T(0.0); S = G;  // noslipbndQ
+            if (side.isright < 0 || side.istop < 0) { Fh[i] = F; Fq[i] = G; Ss[inside] = S; }
+            else { Fh[i] = F; Ss[i] = G; Fq[inside] = S; }
+        }
+    }
+}
+
+// src/Boundary.cu:108-195
+template <class T> void FlowbndFlux(Mdl<T>& M) {
+    if (M.bndtype == 1) return;
+    if (M.bndtype != 0) { std::fprintf(stderr, "oracle: boundary type %d not restated\n", M.bndtype); std::abort(); }
+    bndFluxSideWall(M.left, M.Fhu.data(), M.Fqux.data(), M.Su.data());
+    bndFluxSideWall(M.right, M.Fhu.data(), M.Fqux.data(), M.Su.data());
+    bndFluxSideWall(M.top, M.Fhv.data(), M.Fqvy.data(), M.Sv.data());
+    bndFluxSideWall(M.bot, M.Fhv.data(), M.Fqvy.data(), M.Sv.data());
+}
+
+// --------------------------------------------------------------- time step
+// src/Advection.cu:470-498
+template <class T> T timestepreductionCPU(const Mdl<T>& M) {
+    T epsi = std::nextafter(T(1.0), T(2.0)) - T(1.0);
+    T dt = T(1.0) / epsi;
+    for (int ibl = 0; ibl < M.p.nblk; ibl++) {
+        int ib = M.b.active[ibl];
+        for (int iy = 0; iy < BW; iy++)
+            for (int ix = 0; ix < BW; ix++) dt = umin(dt, M.dtmax[memloc(ix, iy, ib)]);
+    }
+    return dt;
+}
+
+// src/Advection.cu:511-535 (the clamp and the output-time landing)
+template <class T> T timestep_clamp(T dt, const orc_loop& L) {
+    if (dt > (1.5 * L.dtmax)) dt = T(1.5 * L.dtmax);
+    if (std::ceil((L.nextoutputtime - L.totaltime) / dt) > 0.0)
+        dt = T((L.nextoutputtime - L.totaltime) / std::ceil((L.nextoutputtime - L.totaltime) / dt));
+    return dt;
+}
+
+// ------------------------------------------------------------- advance etc.
+// src/Advection.cu:160-254
+template <class T> void updateEVCPU(Mdl<T>& M, const Ev<T>& E) {
+    const Prm& P = M.p;
+    const Blk<T>& B = M.b;
+    T g = T(P.g);
+    for (int ibl = 0; ibl < P.nblk; ibl++) {
+        int ib = B.active[ibl];
+        int lev = B.level[ib];
+        T delta = calcres(T(P.delta), lev);
+        for (int iy = 0; iy < BW; iy++)
+            for (int ix = 0; ix < BW; ix++) {
+                T fc = T(std::sin(T(P.lat * kPi / 180.0)) * kPi / T(21600.0));
+                int i = memloc(ix, iy, ib), iright = memloc(ix + 1, iy, ib), itop = memloc(ix, iy + 1, ib);
+                T cm = T(1.0), fmu = T(1.0), fmv = T(1.0), fmup = T(1.0), fmvp = T(1.0);
+                T hi = E.h[i], uui = E.u[i], vvi = E.v[i];
+                T cmdinv = T(1.0) / (cm * delta);
+                T ga = T(0.5) * g;
+                M.dh[i] = T(-1.0) * (M.Fhu[iright] - M.Fhu[i] + M.Fhv[itop] - M.Fhv[i]) * cmdinv;
+                T dmdl = (fmup - fmu) / (cm * delta);
+                T dmdt = (fmvp - fmv) / (cm * delta);
+                T fG = vvi * dmdl - uui * dmdt;
+                M.dhu[i] = (M.Fqux[i] + M.Fquy[i] - M.Su[iright] - M.Fquy[itop]) * cmdinv + fc * hi * vvi;
+                M.dhv[i] = (M.Fqvy[i] + M.Fqvx[i] - M.Sv[itop] - M.Fqvx[iright]) * cmdinv - fc * hi * uui;
+                M.dhu[i] += hi * (ga * hi * dmdl + fG * vvi);
+                M.dhv[i] += hi * (ga * hi * dmdt - fG * uui);
+            }
+    }
+}
+
+// src/Advection.cu:332-385 (the drying limiter mixes T and double exactly as here)
+template <class T> void AdvkernelCPU(Mdl<T>& M, T dt, const Ev<T>& E, Ev<T>& Eo) {
+    const Prm& P = M.p;
+    T eps = T(P.eps);
+    for (int ibl = 0; ibl < P.nblk; ibl++) {
+        int ib = M.b.active[ibl];
+        for (int iy = 0; iy < BW; iy++)
+            for (int ix = 0; ix < BW; ix++) {
+                int i = memloc(ix, iy, ib);
+                T hold = E.h[i];
+                T ho, uo, vo, dhi = M.dh[i];
+                T edt = P.ForceMassConserve ? dt
+                        : dhi >= T(0.0) ? dt
+                                        : T(std::min((double)dt, std::max((double)hold, P.eps) / (double)std::abs(dhi)));
+                ho = hold + edt * dhi;
+                if (ho > eps) {
+                    uo = (hold * E.u[i] + edt * M.dhu[i]) / ho;
+                    vo = (hold * E.v[i] + edt * M.dhv[i]) / ho;
+                } else {
+                    uo = T(0.0); vo = T(0.0);
+                }
+                Eo.zs[i] = M.zb[i] + ho;
+                Eo.h[i] = ho;
+                Eo.u[i] = uo;
+                Eo.v[i] = vo;
+            }
+    }
+}
+
+// src/Friction.cu:317-334
+template <class T> T smartfriction(T hi, T zo) {
+    T cfi;
+    T ee = T(2.71828182845905);
+    T Hbar = hi / zo;
+    if (Hbar <= ee) cfi = T(1.0) / (T(0.46) * Hbar);
+    else cfi = T(1.0) / (T(2.5) * (std::log(Hbar) - T(1.0) + T(1.359) / Hbar));
+    return cfi * cfi;
+}
+
+// src/Friction.cu:347-351
+template <class T> T manningfriction(T g, T hi, T n) { return g * n * n / std::cbrt(hi); }
+
+// src/Friction.cu:89-148
+template <class T> void bottomfrictionCPU(Mdl<T>& M, T dt, Ev<T>& E) {
+    const Prm& P = M.p;
+    T eps = T(P.eps), g = T(P.g);
+    for (int ibl = 0; ibl < P.nblk; ibl++) {
+        int ib = M.b.active[ibl];
+        for (int iy = 0; iy < BW; iy++)
+            for (int ix = 0; ix < BW; ix++) {
+                int i = memloc(ix, iy, ib);
+                T hi = E.h[i], ui = E.u[i], vi = E.v[i];
+                if (hi > eps) {
+                    T normu = std::sqrt(ui * ui + vi * vi);
+                    T cfi = T(0);
+                    if (P.frictionmodel == 0) cfi = M.cf[i];
+                    else if (P.frictionmodel == 1) cfi = smartfriction(hi, M.cf[i]);
+                    else if (P.frictionmodel == -1) cfi = manningfriction(g, hi, M.cf[i]);
+                    T tb = cfi * normu / hi * dt;
+                    E.u[i] = ui / (T(1.0) + tb);
+                    E.v[i] = vi / (T(1.0) + tb);
+                }
+            }
+    }
+}
+
+// src/Advection.cu:431-456 (cleanupCPU(evolv_o -> evolv))
+template <class T> void cleanupCPU(Mdl<T>& M, const Ev<T>& Eo, Ev<T>& E) {
+    for (int ibl = 0; ibl < M.p.nblk; ibl++) {
+        int ib = M.b.active[ibl];
+        for (int iy = 0; iy < BW; iy++)
+            for (int ix = 0; ix < BW; ix++) {
+                int i = memloc(ix, iy, ib);
+                E.h[i] = Eo.h[i]; E.zs[i] = Eo.zs[i]; E.u[i] = Eo.u[i]; E.v[i] = Eo.v[i];
+            }
+    }
+}
+
+// src/Updateforcing.cu:530-581 (uniform rain)
+template <class T> void AddrainforcingImplicitCPU(Mdl<T>& M, const orc_loop& L, Ev<T>& E) {
+    const Prm& P = M.p;
+    for (int ibl = 0; ibl < P.nblk; ibl++) {
+        int ib = M.b.active[ibl];
+        for (int iy = 0; iy < BW; iy++)
+            for (int ix = 0; ix < BW; ix++) {
+                int i = memloc(ix, iy, ib);
+                T hi = E.h[i];
+                T Rainhh = T(P.rain);
+                Rainhh = std::max(Rainhh / T(1000.0) / T(3600.0) * T(L.dt), T(0.0)) * M.activeCell[i];
+                T qvol = hi / (hi + Rainhh);
+                E.h[i] = hi + Rainhh;
+                E.zs[i] += Rainhh;
+                if (hi > P.eps) {
+                    E.u[i] = E.u[i] * qvol;
+                    E.v[i] = E.v[i] * qvol;
+                }
+            }
+    }
+}
+
+// src/FlowCPU.cu:15-253 (engine 2; no atmospheric pressure, wind, rivers,
+// infiltration, velocity threshold or conserveElevation)
+template <class T> void FlowCPU(Mdl<T>& M, orc_loop& L) {
+    fillHaloEv(M, M.ev);
+    std::fill(M.dtmax.begin(), M.dtmax.end(), std::numeric_limits<T>::max());
+    gradientCPU(M, M.ev);
+    updateKurgXCPU(M, M.ev);
+    updateKurgYCPU(M, M.ev);
+    fillHaloFlux(M);
+    FlowbndFlux(M);
+    L.dt = double(timestep_clamp(timestepreductionCPU(M), L));
+    L.dtmax = L.dt;
+    T dt = T(L.dt);
+    updateEVCPU(M, M.ev);
+    AdvkernelCPU(M, dt * T(0.5), M.ev, M.evo);
+
+    fillHaloEv(M, M.evo);
+    gradientCPU(M, M.evo);
+    updateKurgXCPU(M, M.evo);
+    updateKurgYCPU(M, M.evo);
+    fillHaloFlux(M);
+    FlowbndFlux(M);
+    updateEVCPU(M, M.evo);
+    AdvkernelCPU(M, dt, M.ev, M.evo);
+    bottomfrictionCPU(M, dt, M.evo);
+    cleanupCPU(M, M.evo, M.ev);
+    if (M.p.rain_on) AddrainforcingImplicitCPU(M, L, M.ev);
+}
+
+// src/Mainloop.cu:569-589
+template <class T> void CalcInitdtCPU(Mdl<T>& M) {
+    for (int ibl = 0; ibl < M.p.nblk; ibl++) {
+        int ib = M.b.active[ibl];
+        T delta = calcres(T(M.p.delta), M.b.level[ib]);
+        for (int iy = 0; iy < BW; iy++)
+            for (int ix = 0; ix < BW; ix++) {
+                int i = memloc(ix, iy, ib);
+                M.dtmax[i] = delta / std::sqrt(T(M.p.g) * std::max(M.ev.h[i], T(M.p.eps)));
+            }
+    }
+}
+
+// src/Mainloop.cu:515-554
+template <class T> double initdt(Mdl<T>& M, orc_loop L) {
+    if (M.p.dtinit > 0) L.dtmax = M.p.dtinit / 1.5;
+    else L.dtmax = calcres(M.p.delta, M.p.maxlevel) / (std::sqrt(M.p.g * 10.0));
+    CalcInitdtCPU(M);
+    return double(timestep_clamp(timestepreductionCPU(M), L));
+}
+
+// src/InitialConditions.cu:135-147 (zb halo; the zb gradients feed only engine 1)
+template <class T> void InitzbHalo(Mdl<T>& M) { fillHaloC(M.p, M.b, M.zb.data()); }
+
+// src/InitialConditions.cu:1168-1290: all boundary blocks join the remainder segment
+template <class T> void Initbndblks(Mdl<T>& M) {
+    const Blk<T>& B = M.b;
+    M.left = {{}, -1, 0}; M.right = {{}, 1, 0}; M.top = {{}, 0, 1}; M.bot = {{}, 0, -1};
+    for (int ibl = 0; ibl < M.p.nblk; ibl++) {
+        int ib = B.active[ibl];
+        if ((B.BotLeft[ib] == ib) || (B.BotRight[ib] == ib)) M.bot.blk.push_back(ib);
+        if ((B.TopLeft[ib] == ib) || (B.TopRight[ib] == ib)) M.top.blk.push_back(ib);
+        if ((B.LeftBot[ib] == ib) || (B.LeftTop[ib] == ib)) M.left.blk.push_back(ib);
+        if ((B.RightBot[ib] == ib) || (B.RightTop[ib] == ib)) M.right.blk.push_back(ib);
+    }
+}
+
+// src/InitialConditions.cu:1560-1593 (no AOI; the rainbnd strip uses the legacy
+// left/right/top/bot lists, which stay empty with boundary segments)
+template <class T> void calcactiveCellCPU(Mdl<T>& M) {
+    for (int ibl = 0; ibl < M.p.nblk; ibl++) {
+        int ib = M.b.active[ibl];
+        for (int j = 0; j < BW; j++)
+            for (int i = 0; i < BW; i++) {
+                int n = memloc(i, j, ib);
+                M.activeCell[n] = (M.zb[n] < M.p.mask) ? 1 : 0;
+            }
+    }
+}
+
+// src/Mesh.cu:257-347 + 351-433 (uniform grid, no mask, no AOI)
+template <class T> int mesh_uniform(const Prm& P, int* active, int* level, double* xo, double* yo,
+                                    int* LB, int* LT, int* RB, int* RT, int* TL, int* TR, int* BL, int* BR) {
+    double levdx = calcres(P.dx, P.initlevel);
+    int maxnbx = ftoi(std::ceil(P.nx / (double)BW));
+    int maxnby = ftoi(std::ceil(P.ny / (double)BW));
+    int nb = maxnbx * maxnby;
+    std::vector<T> bxo(nb), byo(nb);
+    int blkid = 0;
+    for (int nblky = 0; nblky < maxnby; nblky++)
+        for (int nblkx = 0; nblkx < maxnbx; nblkx++) {
+            bxo[blkid] = nblkx * ((T)BW) * (T)levdx + T(0.5) * (T)levdx;
+            byo[blkid] = nblky * ((T)BW) * (T)levdx + T(0.5) * (T)levdx;
+            active[blkid] = blkid;
+            level[blkid] = P.initlevel;
+            blkid++;
+        }
+    for (int ibl = 0; ibl < nb; ibl++) {
+        int bl = active[ibl];
+        T espdist = (T)levdx / 3;
+        T leftxo = bxo[bl] - ((T)BW) * (T)levdx, leftyo = byo[bl];
+        T rightxo = bxo[bl] + ((T)BW) * (T)levdx, rightyo = byo[bl];
+        T topxo = bxo[bl], topyo = byo[bl] + ((T)BW) * (T)levdx;
+        T botxo = bxo[bl], botyo = byo[bl] - ((T)BW) * (T)levdx;
+        LB[bl] = LT[bl] = RB[bl] = RT[bl] = TL[bl] = TR[bl] = BL[bl] = BR[bl] = bl;
+        for (int iblb = 0; iblb < nb; iblb++) {
+            int blb = active[iblb];
+            if (std::abs(bxo[blb] - leftxo) < espdist && std::abs(byo[blb] - leftyo) < espdist) { LB[bl] = blb; LT[bl] = blb; }
+            if (std::abs(bxo[blb] - rightxo) < espdist && std::abs(byo[blb] - rightyo) < espdist) { RB[bl] = blb; RT[bl] = blb; }
+            if (std::abs(bxo[blb] - topxo) < espdist && std::abs(byo[blb] - topyo) < espdist) { TL[bl] = blb; TR[bl] = blb; }
+            if (std::abs(bxo[blb] - botxo) < espdist && std::abs(byo[blb] - botyo) < espdist) { BL[bl] = blb; BR[bl] = blb; }
+        }
+    }
+    for (int i = 0; i < nb; i++) { xo[i] = (double)bxo[i]; yo[i] = (double)byo[i]; }
+    return nb;
+}
+
+Prm to_prm(const orc_param* p) {
+    Prm q;
+    q.g = p->g; q.eps = p->eps; q.CFL = p->CFL; q.theta = p->theta; q.cf = p->cf;
+    q.dx = p->dx; q.delta = p->delta; q.xo = p->xo; q.yo = p->yo; q.lat = p->lat;
+    q.zsinit = p->zsinit; q.zsoffset = p->zsoffset; q.dtinit = p->dtinit; q.mask = p->mask;
+    q.frictionmodel = p->frictionmodel; q.aoibnd = p->aoibnd; q.ForceMassConserve = p->ForceMassConserve;
+    q.wetdryfix = p->wetdryfix; q.nblk = p->nblk; q.nblkmem = p->nblkmem; q.initlevel = p->initlevel;
+    q.minlevel = p->minlevel; q.maxlevel = p->maxlevel; q.nx = p->nx; q.ny = p->ny;
+    q.rain_on = p->rain_on; q.rain = p->rain; q.nthreads = p->nthreads;
+    return q;
+}
+
+struct AnyModel {
+    int dbl;
+    Mdl<double>* d;
+    Mdl<float>* f;
+};
+
+template <class T> std::vector<T>* array_by_name(Mdl<T>& M, const std::string& n) {
+    if (n == "h") return &M.ev.h;
+    if (n == "zs") return &M.ev.zs;
+    if (n == "u") return &M.ev.u;
+    if (n == "v") return &M.ev.v;
+    if (n == "ho") return &M.evo.h;
+    if (n == "zso") return &M.evo.zs;
+    if (n == "uo") return &M.evo.u;
+    if (n == "vo") return &M.evo.v;
+    if (n == "zb") return &M.zb;
+    if (n == "cf") return &M.cf;
+    if (n == "dtmax") return &M.dtmax;
+    if (n == "dhdx") return &M.dhdx;
+    if (n == "dhdy") return &M.dhdy;
+    if (n == "dzsdx") return &M.dzsdx;
+    if (n == "dzsdy") return &M.dzsdy;
+    if (n == "dudx") return &M.dudx;
+    if (n == "dudy") return &M.dudy;
+    if (n == "dvdx") return &M.dvdx;
+    if (n == "dvdy") return &M.dvdy;
+    if (n == "Fhu") return &M.Fhu;
+    if (n == "Fhv") return &M.Fhv;
+    if (n == "Fqux") return &M.Fqux;
+    if (n == "Fqvy") return &M.Fqvy;
+    if (n == "Fquy") return &M.Fquy;
+    if (n == "Fqvx") return &M.Fqvx;
+    if (n == "Su") return &M.Su;
+    if (n == "Sv") return &M.Sv;
+    if (n == "dh") return &M.dh;
+    if (n == "dhu") return &M.dhu;
+    if (n == "dhv") return &M.dhv;
+    return nullptr;
+}
+
+template <class T> Mdl<T>* create(const orc_param* p, const orc_blocks* b) {
+    auto* M = new Mdl<T>();
+    M->p = to_prm(p);
+    size_t S = (size_t)p->nblkmem * BS;
+    int nbm = p->nblkmem;
+    Blk<T>& B = M->b;
+    auto cp = [nbm](std::vector<int>& dst, const int* src) { dst.assign(src, src + nbm); };
+    cp(B.active, b->active); cp(B.level, b->level);
+    cp(B.LeftBot, b->LeftBot); cp(B.LeftTop, b->LeftTop); cp(B.RightBot, b->RightBot); cp(B.RightTop, b->RightTop);
+    cp(B.TopLeft, b->TopLeft); cp(B.TopRight, b->TopRight); cp(B.BotLeft, b->BotLeft); cp(B.BotRight, b->BotRight);
+    B.xo.resize(nbm); B.yo.resize(nbm);
+    for (int i = 0; i < nbm; i++) { B.xo[i] = T(b->xo[i]); B.yo[i] = T(b->yo[i]); }
+    for (auto* v : {&M->ev.zs, &M->ev.h, &M->ev.u, &M->ev.v, &M->evo.zs, &M->evo.h, &M->evo.u, &M->evo.v,
+                    &M->dzsdx, &M->dhdx, &M->dudx, &M->dvdx, &M->dzsdy, &M->dhdy, &M->dudy, &M->dvdy, &M->dzbdx, &M->dzbdy,
+                    &M->Su, &M->Sv, &M->Fqux, &M->Fquy, &M->Fqvx, &M->Fqvy, &M->Fhu, &M->Fhv,
+                    &M->dh, &M->dhu, &M->dhv, &M->zb, &M->cf, &M->dtmax})
+        v->assign(S, T(0));
+    M->activeCell.assign(S, 0);
+    M->bndtype = p->aoibnd;
+    Initbndblks(*M);
+    return M;
+}
+
+template <class T> int set_state(Mdl<T>& M, const void* zb, const void* h, const void* u, const void* v, const void* cf) {
+    size_t S = (size_t)M.p.nblkmem * BS;
+    std::memcpy(M.zb.data(), zb, S * sizeof(T));
+    std::memcpy(M.ev.h.data(), h, S * sizeof(T));
+    std::memcpy(M.ev.u.data(), u, S * sizeof(T));
+    std::memcpy(M.ev.v.data(), v, S * sizeof(T));
+    std::memcpy(M.cf.data(), cf, S * sizeof(T));
+    InitzbHalo(M);
+    for (size_t i = 0; i < S; i++) M.ev.zs[i] = M.zb[i] + M.ev.h[i];
+    M.evo = M.ev;
+    calcactiveCellCPU(M);
+    return 0;
+}
+
+template <class T> double volume(const Mdl<T>& M) {
+    double vol = 0.0;
+    for (int ibl = 0; ibl < M.p.nblk; ibl++) {
+        int ib = M.b.active[ibl];
+        double d = calcres(M.p.delta, M.b.level[ib]);
+        for (int iy = 0; iy < BW; iy++)
+            for (int ix = 0; ix < BW; ix++) vol += (double)M.ev.h[memloc(ix, iy, ib)] * d * d;
+    }
+    return vol;
+}
+
+}  // namespace
+
+struct orc_model_s { AnyModel m; };
+
+extern "C" {
+
+int orc_mesh_uniform(const orc_param* p, int* active, int* level, double* xo, double* yo,
+                     int* LB, int* LT, int* RB, int* RT, int* TL, int* TR, int* BL, int* BR) {
+    Prm q = to_prm(p);
+    if (p->doubleprecision) return mesh_uniform<double>(q, active, level, xo, yo, LB, LT, RB, RT, TL, TR, BL, BR);
+    return mesh_uniform<float>(q, active, level, xo, yo, LB, LT, RB, RT, TL, TR, BL, BR);
+}
+
+orc_model orc_create(const orc_param* p, const orc_blocks* b) {
+    auto* h = new orc_model_s();
+    h->m.dbl = p->doubleprecision;
+    h->m.d = nullptr; h->m.f = nullptr;
+    if (p->doubleprecision) h->m.d = create<double>(p, b);
+    else h->m.f = create<float>(p, b);
+    return h;
+}
+
+void orc_destroy(orc_model m) {
+    if (!m) return;
+    delete m->m.d;
+    delete m->m.f;
+    delete m;
+}
+
+int orc_set_state(orc_model m, const void* zb, const void* h, const void* u, const void* v, const void* cf) {
+    return m->m.dbl ? set_state(*m->m.d, zb, h, u, v, cf) : set_state(*m->m.f, zb, h, u, v, cf);
+}
+
+int orc_get_array(orc_model m, const char* name, void* out) {
+    if (m->m.dbl) {
+        auto* a = array_by_name(*m->m.d, name);
+        if (!a) return -1;
+        std::memcpy(out, a->data(), a->size() * sizeof(double));
+    } else {
+        auto* a = array_by_name(*m->m.f, name);
+        if (!a) return -1;
+        std::memcpy(out, a->data(), a->size() * sizeof(float));
+    }
+    return 0;
+}
+
+int orc_set_array(orc_model m, const char* name, const void* in) {
+    if (m->m.dbl) {
+        auto* a = array_by_name(*m->m.d, name);
+        if (!a) return -1;
+        std::memcpy(a->data(), in, a->size() * sizeof(double));
+    } else {
+        auto* a = array_by_name(*m->m.f, name);
+        if (!a) return -1;
+        std::memcpy(a->data(), in, a->size() * sizeof(float));
+    }
+    return 0;
+}
+
+int orc_init_loop(orc_model m, orc_loop* loop) {
+    loop->dtmax = m->m.dbl ? initdt(*m->m.d, *loop) : initdt(*m->m.f, *loop);
+    return 0;
+}
+
+int orc_step(orc_model m, orc_loop* loop) {
+    if (m->m.dbl) FlowCPU(*m->m.d, *loop);
+    else FlowCPU(*m->m.f, *loop);
+    return 0;
+}
+
+// Main-loop time keeping of src/Mainloop.cu:33-60 around the step.
+int orc_run(orc_model m, orc_loop* loop, int nsteps, double* seconds) {
+    auto t0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < nsteps; k++) {
+        orc_step(m, loop);
+        loop->totaltime = loop->totaltime + loop->dt;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    return 0;
+}
+
+double orc_volume(orc_model m) { return m->m.dbl ? volume(*m->m.d) : volume(*m->m.f); }
+
+double orc_minmod2_d(double theta, double s0, double s1, double s2) { return minmod2(theta, s0, s1, s2); }
+float orc_minmod2_f(float theta, float s0, float s1, float s2) { return minmod2(theta, s0, s1, s2); }
+double orc_kurgsolver_d(double g, double delta, double epsi, double CFL, double cm, double fm,
+                        double hp, double hm, double up, double um, double* fh, double* fu) {
+    return KurgSolver(g, delta, epsi, CFL, cm, fm, hp, hm, up, um, *fh, *fu);
+}
+float orc_kurgsolver_f(float g, float delta, float epsi, float CFL, float cm, float fm,
+                       float hp, float hm, float up, float um, float* fh, float* fu) {
+    return KurgSolver(g, delta, epsi, CFL, cm, fm, hp, hm, up, um, *fh, *fu);
+}
+double orc_calcres_d(double dx, int level) { return calcres(dx, level); }
+int orc_ftoi_d(double v) { return ftoi(v); }
+int orc_memloc(int i, int j, int ib) { return memloc(i, j, ib); }
+double orc_timestep_d(double dtmin_cells, const orc_loop* loop) { return timestep_clamp(dtmin_cells, *loop); }
+float orc_timestep_f(float dtmin_cells, const orc_loop* loop) { return timestep_clamp(dtmin_cells, *loop); }
+
+}  // extern "C"

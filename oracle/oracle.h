@@ -1,0 +1,83 @@
+/*
+ * oracle.h -- C ABI of the test oracle.
+ *
+ * The oracle is TEST INFRASTRUCTURE ONLY: a CPU restatement of BG_Flood's
+ * FlowCPU<T> step (engine 2, Kurganov-Petrova) and of the mesh / initial
+ * condition helpers it relies on.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it, and only as a checker or as the
+ * timed CPU baseline.  The product library (bg_flood_amd) never links it.
+ *
+ * Every function in oracle.cpp cites the reference file:line it restates.
+ * Parity status of each function is listed at the top of oracle.cpp.
+ */
+#ifndef BGF_ORACLE_H
+#define BGF_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Subset of class Param (src/Param.h:10-239) used by the Kurganov step. */
+typedef struct {
+    double g, eps, CFL, theta, cf;
+    double dx, delta, xo, yo, lat;
+    double zsinit, zsoffset, dtinit, mask;
+    int frictionmodel;        /* -1 Manning, 0 quadratic, 1 Smart */
+    int aoibnd;               /* type of the remainder boundary segment: 0 wall, 1 Neumann */
+    int ForceMassConserve;
+    int wetdryfix;
+    int nblk, nblkmem;
+    int initlevel, minlevel, maxlevel;
+    int nx, ny;               /* grid size at initlevel (multiples of 16) */
+    int rain_on;              /* uniform rain forcing present */
+    float rain;               /* Rain.nowvalue in mm/h */
+    int doubleprecision;      /* 1: T=double, 0: T=float */
+    int nthreads;             /* host threads for per-variable halo/gradient work (reference: 4) */
+} orc_param;
+
+/* Block description, BlockP<T> (src/Arrays.h:233-249); arrays of nblkmem ints/doubles. */
+typedef struct {
+    const int *active, *level;
+    const int *LeftBot, *LeftTop, *RightBot, *RightTop;
+    const int *TopLeft, *TopRight, *BotLeft, *BotRight;
+    const double *xo, *yo;
+} orc_blocks;
+
+/* Loop<T> time keeping subset (src/Arrays.h:358-404). */
+typedef struct {
+    double nextoutputtime, dt, dtmax, totaltime;
+} orc_loop;
+
+typedef struct orc_model_s* orc_model;
+
+int orc_mesh_uniform(const orc_param* p, int* active, int* level, double* xo, double* yo,
+                     int* LB, int* LT, int* RB, int* RT, int* TL, int* TR, int* BL, int* BR);
+
+orc_model orc_create(const orc_param* p, const orc_blocks* b);
+void orc_destroy(orc_model m);
+/* Arrays are nblkmem*324 elements of T (double if doubleprecision else float). */
+int orc_set_state(orc_model m, const void* zb, const void* h, const void* u, const void* v, const void* cf);
+int orc_get_array(orc_model m, const char* name, void* out);
+int orc_set_array(orc_model m, const char* name, const void* in);
+int orc_init_loop(orc_model m, orc_loop* loop);
+int orc_step(orc_model m, orc_loop* loop);
+int orc_run(orc_model m, orc_loop* loop, int nsteps, double* seconds);
+double orc_volume(orc_model m);
+
+/* Primitive restatements for unit tests. */
+double orc_minmod2_d(double theta, double s0, double s1, double s2);
+float  orc_minmod2_f(float theta, float s0, float s1, float s2);
+double orc_kurgsolver_d(double g, double delta, double epsi, double CFL, double cm, double fm,
+                        double hp, double hm, double up, double um, double* fh, double* fu);
+float  orc_kurgsolver_f(float g, float delta, float epsi, float CFL, float cm, float fm,
+                        float hp, float hm, float up, float um, float* fh, float* fu);
+double orc_calcres_d(double dx, int level);
+int    orc_ftoi_d(double v);
+int    orc_memloc(int i, int j, int ib);
+double orc_timestep_d(double dtmin_cells, const orc_loop* loop);
+float  orc_timestep_f(float dtmin_cells, const orc_loop* loop);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

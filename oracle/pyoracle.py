@@ -1,0 +1,209 @@
+"""ctypes binding of the test oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, by __graft_entry__.smoke() as
+the checker and by bench.py for the timed CPU baseline; never by the product
+package.  The oracle restates BG_Flood's FlowCPU<T> step (engine 2); see
+oracle.cpp for the per-function reference citations and parity pins.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+BW = 16
+MW = 18
+BS = MW * MW
+
+
+class OrcParam(ctypes.Structure):
+    _fields_ = [
+        ("g", ctypes.c_double), ("eps", ctypes.c_double), ("CFL", ctypes.c_double),
+        ("theta", ctypes.c_double), ("cf", ctypes.c_double),
+        ("dx", ctypes.c_double), ("delta", ctypes.c_double), ("xo", ctypes.c_double),
+        ("yo", ctypes.c_double), ("lat", ctypes.c_double),
+        ("zsinit", ctypes.c_double), ("zsoffset", ctypes.c_double), ("dtinit", ctypes.c_double),
+        ("mask", ctypes.c_double),
+        ("frictionmodel", ctypes.c_int), ("aoibnd", ctypes.c_int), ("ForceMassConserve", ctypes.c_int),
+        ("wetdryfix", ctypes.c_int), ("nblk", ctypes.c_int), ("nblkmem", ctypes.c_int),
+        ("initlevel", ctypes.c_int), ("minlevel", ctypes.c_int), ("maxlevel", ctypes.c_int),
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+        ("rain_on", ctypes.c_int), ("rain", ctypes.c_float),
+        ("doubleprecision", ctypes.c_int), ("nthreads", ctypes.c_int),
+    ]
+
+
+class OrcBlocks(ctypes.Structure):
+    _fields_ = [(n, ctypes.POINTER(ctypes.c_int)) for n in
+                ("active", "level", "LeftBot", "LeftTop", "RightBot", "RightTop",
+                 "TopLeft", "TopRight", "BotLeft", "BotRight")] + \
+               [("xo", ctypes.POINTER(ctypes.c_double)), ("yo", ctypes.POINTER(ctypes.c_double))]
+
+
+class OrcLoop(ctypes.Structure):
+    _fields_ = [("nextoutputtime", ctypes.c_double), ("dt", ctypes.c_double),
+                ("dtmax", ctypes.c_double), ("totaltime", ctypes.c_double)]
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.orc_mesh_uniform.restype = ctypes.c_int
+        L.orc_mesh_uniform.argtypes = [ctypes.POINTER(OrcParam), ip, ip, dp, dp] + [ip] * 8
+        L.orc_create.restype = ctypes.c_void_p
+        L.orc_create.argtypes = [ctypes.POINTER(OrcParam), ctypes.POINTER(OrcBlocks)]
+        L.orc_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_set_state.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 5
+        L.orc_get_array.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p]
+        L.orc_set_array.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p]
+        L.orc_init_loop.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcLoop)]
+        L.orc_step.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcLoop)]
+        L.orc_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcLoop), ctypes.c_int, dp]
+        L.orc_volume.restype = ctypes.c_double
+        L.orc_volume.argtypes = [ctypes.c_void_p]
+        L.orc_minmod2_d.restype = ctypes.c_double
+        L.orc_minmod2_d.argtypes = [ctypes.c_double] * 4
+        L.orc_minmod2_f.restype = ctypes.c_float
+        L.orc_minmod2_f.argtypes = [ctypes.c_float] * 4
+        L.orc_kurgsolver_d.restype = ctypes.c_double
+        L.orc_kurgsolver_d.argtypes = [ctypes.c_double] * 10 + [dp, dp]
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.orc_kurgsolver_f.restype = ctypes.c_float
+        L.orc_kurgsolver_f.argtypes = [ctypes.c_float] * 10 + [fp, fp]
+        L.orc_calcres_d.restype = ctypes.c_double
+        L.orc_calcres_d.argtypes = [ctypes.c_double, ctypes.c_int]
+        L.orc_ftoi_d.restype = ctypes.c_int
+        L.orc_ftoi_d.argtypes = [ctypes.c_double]
+        L.orc_memloc.restype = ctypes.c_int
+        L.orc_memloc.argtypes = [ctypes.c_int] * 3
+        L.orc_timestep_d.restype = ctypes.c_double
+        L.orc_timestep_d.argtypes = [ctypes.c_double, ctypes.POINTER(OrcLoop)]
+        L.orc_timestep_f.restype = ctypes.c_float
+        L.orc_timestep_f.argtypes = [ctypes.c_float, ctypes.POINTER(OrcLoop)]
+        _lib = L
+    return _lib
+
+
+def _iptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def make_param(**kw):
+    p = OrcParam()
+    defaults = dict(g=9.81, eps=1e-4, CFL=0.5, theta=1.3, cf=1e-4, dx=1.0, delta=1.0, xo=0.0, yo=0.0,
+                    lat=0.0, zsinit=0.0, zsoffset=0.0, dtinit=-1.0, mask=9999.0, frictionmodel=0,
+                    aoibnd=0, ForceMassConserve=0, wetdryfix=1, nblk=0, nblkmem=0, initlevel=0,
+                    minlevel=0, maxlevel=0, nx=0, ny=0, rain_on=0, rain=0.0, doubleprecision=1,
+                    nthreads=4)
+    defaults.update(kw)
+    for k, v in defaults.items():
+        setattr(p, k, v)
+    return p
+
+
+def mesh_uniform(p):
+    """Reference InitBlockxoyo + InitBlockneighbours (O(n^2) coordinate search)."""
+    nbx = -(-p.nx // BW)
+    nby = -(-p.ny // BW)
+    nb = nbx * nby
+    out = {k: np.zeros(nb, np.int32) for k in ("active", "level", "LeftBot", "LeftTop", "RightBot",
+                                                  "RightTop", "TopLeft", "TopRight", "BotLeft", "BotRight")}
+    out["xo"] = np.zeros(nb)
+    out["yo"] = np.zeros(nb)
+    n = lib().orc_mesh_uniform(ctypes.byref(p), _iptr(out["active"]), _iptr(out["level"]), _dptr(out["xo"]),
+                               _dptr(out["yo"]), _iptr(out["LeftBot"]), _iptr(out["LeftTop"]),
+                               _iptr(out["RightBot"]), _iptr(out["RightTop"]), _iptr(out["TopLeft"]),
+                               _iptr(out["TopRight"]), _iptr(out["BotLeft"]), _iptr(out["BotRight"]))
+    assert n == nb
+    return out
+
+
+class Oracle:
+    """One oracle model (FlowCPU state) in the reference's BUQ memory layout."""
+
+    def __init__(self, p, blocks):
+        self.p = p
+        self.dtype = np.float64 if p.doubleprecision else np.float32
+        nbm = p.nblkmem
+        self._keep = {}
+        b = OrcBlocks()
+        for k in ("active", "level", "LeftBot", "LeftTop", "RightBot", "RightTop", "TopLeft", "TopRight",
+                  "BotLeft", "BotRight"):
+            a = np.zeros(nbm, np.int32)
+            src = np.asarray(blocks[k], np.int32)
+            a[:len(src)] = src
+            if k == "active":
+                a[len(src):] = -1
+            self._keep[k] = a
+            setattr(b, k, _iptr(a))
+        for k in ("xo", "yo"):
+            a = np.zeros(nbm)
+            src = np.asarray(blocks[k], np.float64)
+            a[:len(src)] = src
+            self._keep[k] = a
+            setattr(b, k, _dptr(a))
+        self.h = lib().orc_create(ctypes.byref(p), ctypes.byref(b))
+        self.size = nbm * BS
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def _c(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.size
+        return a
+
+    def set_state(self, zb, h, u, v, cf):
+        arrs = [self._c(x) for x in (zb, h, u, v, cf)]
+        lib().orc_set_state(self.h, *[a.ctypes.data for a in arrs])
+
+    def get(self, name):
+        out = np.empty(self.size, self.dtype)
+        rc = lib().orc_get_array(self.h, name.encode(), out.ctypes.data)
+        if rc != 0:
+            raise KeyError(name)
+        return out
+
+    def set(self, name, arr):
+        a = self._c(arr)
+        rc = lib().orc_set_array(self.h, name.encode(), a.ctypes.data)
+        if rc != 0:
+            raise KeyError(name)
+
+    def init_loop(self, loop):
+        lib().orc_init_loop(self.h, ctypes.byref(loop))
+        return loop
+
+    def step(self, loop):
+        lib().orc_step(self.h, ctypes.byref(loop))
+        return loop
+
+    def run(self, loop, nsteps):
+        secs = ctypes.c_double(0.0)
+        lib().orc_run(self.h, ctypes.byref(loop), nsteps, ctypes.byref(secs))
+        return secs.value
+
+    def volume(self):
+        return lib().orc_volume(self.h)
