@@ -1,0 +1,190 @@
+"""Benchmark of the BG_Flood Kurganov timestep on one MI355X.
+
+Workload (default): rain on terrain, 8192 x 8192 cells (262,144 blocks of 16x16), dx = 5 m,
+bed = 20 m x 6-octave seeded value noise + 0.002 planar slope, initially dry, uniform rain
+50 mm/h, Manning n = 0.03, outflow (Neumann) boundaries, FP32 state.
+
+A step is one predictor-corrector timestep of the whole domain (FlowGPU).  W warm-up steps
+run first (they also wet the domain), then exactly K steps are timed between device
+synchronisations.  value = cell-updates/s over the K steps (interior cells x steps / s),
+reported in Gcell-updates/s; inputs are resident in HBM before the timed region.
+
+roofline: the dominant kernel's algorithmic bytes per launch (predictor: h,u,v,zb read,
+dh,dhu,dhv written = 7 words/cell; corrector: h,u,v,dh,dhu,dhv,zb,cf read, h,u,v written =
+11 words/cell) over its average duration measured with HIP events on the library's stream.
+cpu_baseline: the oracle (C++ restatement of the reference FlowCPU, FP64 as the reference
+runs rain cases) on a 2048 x 2048 window of the warmed state for a few steps.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gpu_sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def measure(sc, precision, warmup, steps, timing_steps):
+    import numpy as np
+    import bg_flood_amd as bgf
+    p = dict(sc["param"], precision=precision)
+    T = np.float64 if precision == "fp64" else np.float32
+    blocks = sc["blocks"] if precision != "fp64" else bgf.mesh_uniform(sc["nbx"], sc["nby"], p["dx"], 0, True)
+    d = bgf.Domain(p, blocks)
+    d.set_state(*[np.asarray(sc[k], T) for k in ("zb", "h", "u", "v", "cf")])
+    loop = bgf.Loop(nextoutputtime=1e30, dt=0, dtmax=0, totaltime=0)
+    d.init_dt(loop)
+    if warmup:
+        d.run(loop, warmup)
+    gpu_sync()
+    t0 = time.perf_counter()
+    d.run_async(loop, steps)
+    d.sync(loop)
+    gpu_sync()
+    t1 = time.perf_counter()
+    secs = t1 - t0
+    cells = p["nblk"] * 256
+    # per-kernel durations from HIP events on the library stream
+    d.set_timing(True)
+    d.run(loop, timing_steps)
+    kt = d.kernel_times()
+    d.set_timing(False)
+    return d, loop, dict(secs=secs, cells=cells, steps=steps, kt=kt,
+                         gcups=cells * steps / secs / 1e9, ms_per_step=secs / steps * 1e3)
+
+
+def cpu_baseline(d, sc, window=2048, nsteps=6):
+    """Oracle FP64 on a window x window corner of the warmed GPU state (Neumann edges)."""
+    import numpy as np
+    import bg_flood_amd as bgf
+    from bg_flood_amd import scenarios
+    from oracle import pyoracle as po
+    st = d.get_state()
+    nbx, nby = sc["nbx"], sc["nby"]
+    p = sc["param"]
+    grids = {k: scenarios.grid_from_buq(st[k], nbx, nby)[:window, :window].astype(np.float64) for k in ("h", "u", "v")}
+    grids["zb"] = scenarios.grid_from_buq(sc["zb"], nbx, nby)[:window, :window].astype(np.float64)
+    grids["cf"] = np.full((window, window), p["cf"])
+    pw, blocks, wx, wy = scenarios.uniform(window, window, p["dx"], "fp64", aoibnd=1, frictionmodel=-1,
+                                           cf=p["cf"], rain_on=1, rain=p["rain"])
+    sub = dict(param=pw, blocks=blocks, nbx=wx, nby=wy)
+    for k in ("h", "u", "v", "zb", "cf"):
+        sub[k] = scenarios.buq_from_grid(grids[k], pw["nblkmem"], np.float64)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from tests._util import oracle_for
+    o = oracle_for(sub, nthreads=4)
+    loop = po.OrcLoop(nextoutputtime=1e30, dt=0, dtmax=0, totaltime=0)
+    o.init_loop(loop)
+    secs = o.run(loop, nsteps)
+    cells = pw["nblk"] * 256
+    return dict(value=cells * nsteps / secs / 1e9, unit="Gcell-updates/s", cores=4, kind="port",
+                sample="oracle FlowCPU restatement, FP64, %dx%d window of the warmed 8192^2 state, %d steps "
+                       "(%.1f s); 4 std::threads for per-variable halo/gradient work as in the reference"
+                       % (window, window, nsteps, secs))
+
+
+def load_traffic(name):
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % name)
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--n", type=int, default=8192, help="domain side in cells")
+    ap.add_argument("--precision", default="fp32_fast", choices=["fp32_fast", "fp32", "fp64"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU sharding is described in DESIGN.md "
+              "and not launched (requested --gpus %d)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from bg_flood_amd import scenarios
+    t = time.time()
+    sc = scenarios.rain_terrain(n=args.n, precision="fp32")
+    log("scenario built in %.1f s: %d blocks" % (time.time() - t, sc["param"]["nblk"]))
+
+    d, loop, r = measure(sc, args.precision, args.warmup, args.steps, min(args.steps, 20))
+    log("%s: %.3f Gcell/s, %.3f ms/step, kernels %s, t=%.1f s" % (args.precision, r["gcups"], r["ms_per_step"],
+                                                                 r["kt"], loop.totaltime))
+    es = 8 if args.precision == "fp64" else 4
+    words = {"k_predictor": 7, "k_corrector": 11}
+    dom = max(words, key=lambda k: r["kt"].get(k, 0.0))
+    kms = r["kt"][dom]
+    alg_bytes = words[dom] * es * r["cells"]
+    achieved = alg_bytes / (kms * 1e-3) / 1e9
+    tr = load_traffic("rain%d_%s" % (args.n, args.precision)).get(dom)
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=tr, kernel=dom,
+                kernel_ms=round(kms, 4), alg_bytes_per_launch=alg_bytes,
+                step_frac=round(r["gcups"] * 1e9 * 18 * es / (HBM_PEAK_GBS * 1e9), 4))
+
+    variants = {}
+    if not args.no_variants:
+        for prec in ("fp32", "fp64"):
+            if prec == args.precision:
+                continue
+            dv, lv, rv = measure(sc, prec, args.warmup, max(args.steps // 4, 10), 5)
+            variants[prec] = dict(gcups=round(rv["gcups"], 3), ms_per_step=round(rv["ms_per_step"], 3),
+                                  kernels_ms={k: round(v, 4) for k, v in rv["kt"].items()})
+            log("%s: %.3f Gcell/s" % (prec, rv["gcups"]))
+            dv.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(d, sc)
+            log("cpu baseline: %.4f Gcell/s" % cpu["value"])
+        except Exception as e:  # report, never hide
+            cpu = dict(value=None, unit="Gcell-updates/s", cores=4, kind="port", sample="failed: %r" % (e,))
+
+    out = {
+        "metric": "Gcell-updates/s + %HBM roofline; batched-ensemble p50/p99 scenario latency",
+        "value": round(r["gcups"], 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32",
+        "data": "synthetic",
+        "config": {"workload": "rain on terrain %dx%d cells (%d blocks of 16x16), dx=5 m, seeded 6-octave "
+                               "value-noise bed 20 m + 0.002 slope, rain 50 mm/h, Manning n=0.03, outflow "
+                               "boundaries, initially dry" % (args.n, args.n, sc["param"]["nblk"]),
+                   "precision": args.precision, "simulated_time_s": round(loop.totaltime, 3),
+                   "kernels_ms": {k: round(v, 4) for k, v in r["kt"].items()}},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "variants": variants,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

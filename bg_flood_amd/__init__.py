@@ -1,0 +1,254 @@
+"""bg_flood_amd: MI355X-native BG_Flood Kurganov timestep (FlowGPU drop-in).
+
+Python binding of libbgflood_amd.so through its C ABI (include/bgflood_amd.h).
+The GPU library is the only compute path: importing works without a GPU, but
+creating a Domain fails loudly when the library or the device is missing.
+There is no CPU fallback.
+
+Names follow the reference: Param keys (src/Param.h), the BUQ block list
+(BlockP, src/Arrays.h:233-249), Loop time keeping (src/Arrays.h:358-366) and
+FlowGPU's step (src/FlowGPU.cu:14-315).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbgflood_amd.so")
+
+BW = 16
+MW = 18
+BS = MW * MW
+
+FP64, FP32, FP32_FAST = 0, 1, 2
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "fp32_fast": FP32_FAST}
+
+
+class BgfError(RuntimeError):
+    pass
+
+
+class BgfParam(ctypes.Structure):
+    _fields_ = [
+        ("g", ctypes.c_double), ("eps", ctypes.c_double), ("CFL", ctypes.c_double),
+        ("theta", ctypes.c_double), ("dx", ctypes.c_double), ("delta", ctypes.c_double),
+        ("xo", ctypes.c_double), ("yo", ctypes.c_double), ("lat", ctypes.c_double),
+        ("mask", ctypes.c_double),
+        ("frictionmodel", ctypes.c_int), ("engine", ctypes.c_int), ("aoibnd", ctypes.c_int),
+        ("ForceMassConserve", ctypes.c_int), ("wetdryfix", ctypes.c_int), ("spherical", ctypes.c_int),
+        ("nblk", ctypes.c_int), ("nblkmem", ctypes.c_int), ("minlevel", ctypes.c_int),
+        ("maxlevel", ctypes.c_int), ("rain_on", ctypes.c_int), ("rain", ctypes.c_double),
+        ("precision", ctypes.c_int),
+    ]
+
+
+class BgfBlocks(ctypes.Structure):
+    _fields_ = [(n, ctypes.POINTER(ctypes.c_int)) for n in
+                ("active", "level", "LeftBot", "LeftTop", "RightBot", "RightTop",
+                 "TopLeft", "TopRight", "BotLeft", "BotRight")] + \
+               [("xo", ctypes.POINTER(ctypes.c_double)), ("yo", ctypes.POINTER(ctypes.c_double))]
+
+
+class Loop(ctypes.Structure):
+    """Loop<T> time keeping: nextoutputtime, dt, dtmax, totaltime."""
+    _fields_ = [("nextoutputtime", ctypes.c_double), ("dt", ctypes.c_double),
+                ("dtmax", ctypes.c_double), ("totaltime", ctypes.c_double)]
+
+    def __repr__(self):
+        return "Loop(t=%r, dt=%r, dtmax=%r, next=%r)" % (self.totaltime, self.dt, self.dtmax, self.nextoutputtime)
+
+
+BLOCK_KEYS = ("active", "level", "LeftBot", "LeftTop", "RightBot", "RightTop", "TopLeft", "TopRight",
+              "BotLeft", "BotRight")
+
+_lib = None
+
+
+def library_path():
+    return LIB_PATH
+
+
+def lib():
+    """Loads libbgflood_amd.so (raises BgfError if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BgfError("libbgflood_amd.so is not built: run __graft_entry__.build() or "
+                           "python bg_flood_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        ip = ctypes.POINTER(ctypes.c_int)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.bgf_last_error.restype = ctypes.c_char_p
+        L.bgf_version.restype = ctypes.c_char_p
+        L.bgf_device_count.restype = ctypes.c_int
+        L.bgf_create.argtypes = [ctypes.POINTER(BgfParam), ctypes.POINTER(BgfBlocks), ctypes.POINTER(vp)]
+        L.bgf_destroy.argtypes = [vp]
+        L.bgf_set_state.argtypes = [vp] * 6
+        L.bgf_get_state.argtypes = [vp] * 5
+        L.bgf_init_dt.argtypes = [vp, ctypes.c_double, ctypes.POINTER(Loop)]
+        L.bgf_step.argtypes = [vp, ctypes.POINTER(Loop)]
+        L.bgf_run.argtypes = [vp, ctypes.POINTER(Loop), ctypes.c_int, ctypes.c_double]
+        L.bgf_run_async.argtypes = [vp, ctypes.POINTER(Loop), ctypes.c_int, ctypes.c_double]
+        L.bgf_sync.argtypes = [vp, ctypes.POINTER(Loop)]
+        L.bgf_volume.argtypes = [vp, dp]
+        L.bgf_device_array.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.bgf_set_timing.argtypes = [vp, ctypes.c_int]
+        L.bgf_kernel_times.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), dp, ip]
+        L.bgf_mesh_uniform.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                       ip, ip, dp, dp] + [ip] * 8
+        for name in ("bgf_create", "bgf_destroy", "bgf_set_state", "bgf_get_state", "bgf_init_dt", "bgf_step",
+                     "bgf_run", "bgf_run_async", "bgf_sync", "bgf_volume", "bgf_device_array", "bgf_set_timing",
+                     "bgf_kernel_times", "bgf_mesh_uniform"):
+            getattr(L, name).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise BgfError("bgflood_amd error %d: %s" % (rc, lib().bgf_last_error().decode()))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def mesh_uniform(nbx, nby, dx, level=0, dbl=True):
+    """Block list of a uniform nbx x nby block grid (InitBlockxoyo/InitBlockneighbours order)."""
+    nb = nbx * nby
+    out = {k: np.zeros(nb, np.int32) for k in BLOCK_KEYS}
+    out["xo"] = np.zeros(nb)
+    out["yo"] = np.zeros(nb)
+    _check(lib().bgf_mesh_uniform(nbx, nby, dx, level, int(dbl), _iptr(out["active"]), _iptr(out["level"]),
+                                  _dptr(out["xo"]), _dptr(out["yo"]), _iptr(out["LeftBot"]), _iptr(out["LeftTop"]),
+                                  _iptr(out["RightBot"]), _iptr(out["RightTop"]), _iptr(out["TopLeft"]),
+                                  _iptr(out["TopRight"]), _iptr(out["BotLeft"]), _iptr(out["BotRight"])))
+    return out
+
+
+def default_param(**kw):
+    """Param defaults of src/Param.h:15-97 for the keys the step reads."""
+    p = dict(g=9.81, eps=1e-4, CFL=0.5, theta=1.3, dx=1.0, delta=None, xo=0.0, yo=0.0, lat=0.0, mask=9999.0,
+             frictionmodel=0, engine=2, aoibnd=0, ForceMassConserve=0, wetdryfix=1, spherical=0, nblk=0,
+             nblkmem=0, minlevel=0, maxlevel=0, rain_on=0, rain=0.0, precision="fp64", cf=1e-4,
+             dtinit=-1.0)
+    p.update(kw)
+    if p["delta"] is None:
+        p["delta"] = p["dx"]
+    return p
+
+
+class Domain:
+    """One BUQ domain resident on the GPU (a bgflood_amd handle)."""
+
+    def __init__(self, param, blocks):
+        prec = param["precision"]
+        self.precision = PRECISIONS[prec] if isinstance(prec, str) else int(prec)
+        self.dtype = np.float64 if self.precision == FP64 else np.float32
+        self.param = dict(param)
+        nblk = int(param["nblk"])
+        nbm = int(param["nblkmem"])
+        self.nblk, self.nblkmem = nblk, nbm
+        self.size = nbm * BS
+        P = BgfParam()
+        for name, _t in BgfParam._fields_:
+            if name == "precision":
+                P.precision = self.precision
+            else:
+                setattr(P, name, param[name])
+        self._keep = {}
+        B = BgfBlocks()
+        for k in BLOCK_KEYS:
+            a = np.zeros(nbm, np.int32)
+            src = np.asarray(blocks[k], np.int32)
+            a[:len(src)] = src
+            if k != "active":
+                # unused blocks point at themselves
+                a[len(src):] = np.arange(len(src), nbm) if k != "level" else 0
+            self._keep[k] = a
+            setattr(B, k, _iptr(a))
+        for k in ("xo", "yo"):
+            a = np.zeros(nbm)
+            a[:len(blocks[k])] = blocks[k]
+            self._keep[k] = a
+            setattr(B, k, _dptr(a))
+        self.active = np.asarray(blocks["active"][:nblk], np.int64)
+        h = ctypes.c_void_p()
+        _check(lib().bgf_create(ctypes.byref(P), ctypes.byref(B), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bgf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.size != self.size:
+            raise BgfError("array has %d values, expected nblkmem*324 = %d" % (a.size, self.size))
+        return a
+
+    def set_state(self, zb, h, u, v, cf):
+        arrs = [self._c(x) for x in (zb, h, u, v, cf)]
+        _check(lib().bgf_set_state(self.h, *[a.ctypes.data for a in arrs]))
+
+    def get_state(self):
+        out = {k: np.zeros(self.size, self.dtype) for k in ("zs", "h", "u", "v")}
+        _check(lib().bgf_get_state(self.h, out["zs"].ctypes.data, out["h"].ctypes.data, out["u"].ctypes.data,
+                                   out["v"].ctypes.data))
+        return out
+
+    def init_dt(self, loop, dtinit=-1.0):
+        _check(lib().bgf_init_dt(self.h, dtinit, ctypes.byref(loop)))
+        return loop
+
+    def step(self, loop):
+        """One FlowGPU step: sets loop.dt and loop.dtmax; the caller advances totaltime."""
+        _check(lib().bgf_step(self.h, ctypes.byref(loop)))
+        return loop
+
+    def run(self, loop, nsteps, endtime=1e300):
+        """nsteps of {step; totaltime += dt} on the GPU without host round trips."""
+        _check(lib().bgf_run(self.h, ctypes.byref(loop), int(nsteps), float(endtime)))
+        return loop
+
+    def run_async(self, loop, nsteps, endtime=1e300):
+        _check(lib().bgf_run_async(self.h, ctypes.byref(loop), int(nsteps), float(endtime)))
+
+    def sync(self, loop):
+        _check(lib().bgf_sync(self.h, ctypes.byref(loop)))
+        return loop
+
+    def volume(self):
+        v = ctypes.c_double()
+        _check(lib().bgf_volume(self.h, ctypes.byref(v)))
+        return v.value
+
+    def set_timing(self, on):
+        _check(lib().bgf_set_timing(self.h, int(on)))
+
+    def kernel_times(self):
+        names = (ctypes.c_char_p * 8)()
+        ms = (ctypes.c_double * 8)()
+        n = ctypes.c_int()
+        _check(lib().bgf_kernel_times(self.h, 8, names, ms, ctypes.byref(n)))
+        return {names[i].decode(): ms[i] for i in range(n.value)}
+
+
+def interior_mask(nblkmem, active):
+    """Boolean mask of the interior cells of the active blocks in an nblkmem*324 array."""
+    m = np.zeros((nblkmem, MW, MW), bool)
+    m[np.asarray(active), 1:MW - 1, 1:MW - 1] = True
+    return m.reshape(-1)

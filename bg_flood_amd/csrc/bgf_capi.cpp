@@ -1,0 +1,478 @@
+// bgf_capi.cpp -- extern "C" entry points of libbgflood_amd.so (declared in
+// include/bgflood_amd.h).  A handle owns the device Model<T> of one domain:
+// the state in two buffers (evolv and evolv_o of src/Arrays.h:300-354, used
+// as a ping-pong pair), the static bed and friction arrays, the predictor
+// tendencies (Model.adv), the block description and a device-resident loop
+// state.  Between API calls the current state is always in buffer 0
+// (evolv).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bgflood_amd.h"
+#include "bgf_kernels.h"
+
+namespace {
+
+constexpr int BW = 16;
+constexpr int MW = 18;
+constexpr int BS = MW * MW;
+
+thread_local std::string g_err;
+
+bgf_status fail(bgf_status s, const std::string& msg) {
+    g_err = msg;
+    return s;
+}
+
+#define HIPCHK(call)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess) return fail(BGF_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+inline int memloc(int i, int j, int ib) { return (i + 1) + (j + 1) * MW + ib * BS; }
+
+}  // namespace
+
+struct bgf_handle_s {
+    bgf_param p;
+    int prec;
+    size_t es;  // element size
+    int nblk, nblkmem;
+    size_t S;   // elements per array
+    std::vector<int> active;
+    int *d_active = nullptr, *d_level = nullptr;
+    int *d_nb[8] = {nullptr};  // LeftBot LeftTop RightBot RightTop TopLeft TopRight BotLeft BotRight
+    void *d_h[2] = {nullptr, nullptr}, *d_u[2] = {nullptr, nullptr}, *d_v[2] = {nullptr, nullptr};
+    void *d_zb = nullptr, *d_cf = nullptr, *d_dh = nullptr, *d_dhu = nullptr, *d_dhv = nullptr;
+    void* d_blockmin = nullptr;
+    double *d_volpart = nullptr, *d_volout = nullptr;
+    bgf::DevLoop* d_loop = nullptr;
+    bgf::KParams kp;
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double t_pred = 0, t_red = 0, t_corr = 0;
+    int t_n = 0;
+    int pending_steps = 0;  // steps enqueued by bgf_run_async
+    bool state_set = false;
+    bgf::DevLoop loop_host;  // staging copy of the loop state for uploads
+};
+
+namespace {
+
+bgf_status check_param(const bgf_param* p) {
+    if (!p) return fail(BGF_ERR_INVALID_ARG, "null param");
+    if (p->engine != 2) return fail(BGF_ERR_UNSUPPORTED, "only engine 2 (Kurganov) is provided");
+    if (p->spherical) return fail(BGF_ERR_UNSUPPORTED, "spherical grids are not provided");
+    if (p->nblk <= 0 || p->nblkmem < p->nblk) return fail(BGF_ERR_INVALID_ARG, "need 0 < nblk <= nblkmem");
+    if (p->precision < 0 || p->precision > 2) return fail(BGF_ERR_INVALID_ARG, "bad precision");
+    if (p->aoibnd != 0 && p->aoibnd != 1)
+        return fail(BGF_ERR_UNSUPPORTED, "remainder boundary type must be 0 (wall) or 1 (Neumann)");
+    if (p->frictionmodel < -1 || p->frictionmodel > 1) return fail(BGF_ERR_INVALID_ARG, "bad frictionmodel");
+    if (!(p->delta > 0.0) || !(p->g > 0.0) || !(p->eps > 0.0)) return fail(BGF_ERR_INVALID_ARG, "bad g/eps/delta");
+    return BGF_OK;
+}
+
+template <class T> double coriolis(double lat) {
+    // src/Advection.cu:187 (Cartesian): sin(T(lat*pi/180))*pi/T(21600) with pi = 3.14159265
+    const double pi = 3.14159265;
+    return (double)T(std::sin(T(lat * pi / 180.0)) * pi / T(21600.0));
+}
+
+void free_handle(bgf_handle h) {
+    auto F = [](void* p) { if (p) (void)hipFree(p); };
+    F(h->d_active); F(h->d_level);
+    for (int k = 0; k < 8; k++) F(h->d_nb[k]);
+    for (int k = 0; k < 2; k++) { F(h->d_h[k]); F(h->d_u[k]); F(h->d_v[k]); }
+    F(h->d_zb); F(h->d_cf); F(h->d_dh); F(h->d_dhu); F(h->d_dhv); F(h->d_blockmin);
+    F(h->d_volpart); F(h->d_volout); F(h->d_loop);
+    for (int k = 0; k < 4; k++) if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+bgf_status upload_loop(bgf_handle h, const bgf_loop* loop, double endtime) {
+    bgf::DevLoop& L = h->loop_host;
+    std::memset(&L, 0, sizeof(L));
+    L.nextoutputtime = loop->nextoutputtime;
+    L.dt = loop->dt;
+    L.dtmax = loop->dtmax;
+    L.totaltime = loop->totaltime;
+    L.done = loop->totaltime >= endtime ? 1 : 0;
+    HIPCHK(hipMemcpyAsync(h->d_loop, &L, sizeof(L), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return BGF_OK;
+}
+
+bgf_status download_loop(bgf_handle h, bgf::DevLoop* L) {
+    HIPCHK(hipMemcpyAsync(L, h->d_loop, sizeof(*L), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return BGF_OK;
+}
+
+void fill_buffers(bgf_handle h, int in, void* out_storage) {
+    int out = 1 - in;
+    if (h->prec == 0) {
+        auto* B = (bgf::StepBuffers<double>*)out_storage;
+        B->h = (const double*)h->d_h[in]; B->u = (const double*)h->d_u[in]; B->v = (const double*)h->d_v[in];
+        B->ho = (double*)h->d_h[out]; B->uo = (double*)h->d_u[out]; B->vo = (double*)h->d_v[out];
+        B->zb = (const double*)h->d_zb; B->cf = (const double*)h->d_cf;
+        B->dh = (double*)h->d_dh; B->dhu = (double*)h->d_dhu; B->dhv = (double*)h->d_dhv;
+        B->blockmin = (double*)h->d_blockmin;
+    } else {
+        auto* B = (bgf::StepBuffers<float>*)out_storage;
+        B->h = (const float*)h->d_h[in]; B->u = (const float*)h->d_u[in]; B->v = (const float*)h->d_v[in];
+        B->ho = (float*)h->d_h[out]; B->uo = (float*)h->d_u[out]; B->vo = (float*)h->d_v[out];
+        B->zb = (const float*)h->d_zb; B->cf = (const float*)h->d_cf;
+        B->dh = (float*)h->d_dh; B->dhu = (float*)h->d_dhu; B->dhv = (float*)h->d_dhv;
+        B->blockmin = (float*)h->d_blockmin;
+    }
+}
+
+bgf_status enqueue_steps(bgf_handle h, int nsteps, double endtime, int advance_time) {
+    alignas(16) unsigned char storage[sizeof(bgf::StepBuffers<double>)];
+    for (int k = 0; k < nsteps; k++) {
+        fill_buffers(h, k & 1, storage);
+        bgf::launch_step(h->prec, h->kp, storage, h->d_loop, endtime, advance_time, h->stream,
+                         h->timing ? h->ev : nullptr);
+        if (h->timing) {
+            HIPCHK(hipEventSynchronize(h->ev[3]));
+            float a = 0, b = 0, c = 0;
+            HIPCHK(hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
+            HIPCHK(hipEventElapsedTime(&b, h->ev[1], h->ev[2]));
+            HIPCHK(hipEventElapsedTime(&c, h->ev[2], h->ev[3]));
+            h->t_pred += a; h->t_red += b; h->t_corr += c; h->t_n += 1;
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return BGF_OK;
+}
+
+// After a run, bring the state back to buffer 0 if it ended in buffer 1.
+bgf_status settle(bgf_handle h, int steps_done) {
+    if (steps_done & 1) {
+        const void* src[3] = {h->d_h[1], h->d_u[1], h->d_v[1]};
+        void* dst[3] = {h->d_h[0], h->d_u[0], h->d_v[0]};
+        bgf::launch_copy_state(h->prec, h->kp, src, dst, h->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    return BGF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bgf_last_error(void) { return g_err.c_str(); }
+
+const char* bgf_version(void) { return "bgflood_amd 0.1 (gfx950)"; }
+
+int bgf_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) {
+    if (!out || !b) return fail(BGF_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    bgf_status s = check_param(p);
+    if (s != BGF_OK) return s;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(BGF_ERR_NO_DEVICE, "no HIP device");
+    const int nblk = p->nblk, nbm = p->nblkmem;
+    // The one-level kernels need each side to be a boundary or one same-level neighbour.
+    for (int ibl = 0; ibl < nblk; ibl++) {
+        int ib = b->active[ibl];
+        if (ib < 0 || ib >= nbm) return fail(BGF_ERR_INVALID_ARG, "active block index out of range");
+        const int* pairs[4][2] = {{b->LeftBot, b->LeftTop}, {b->RightBot, b->RightTop},
+                                  {b->TopLeft, b->TopRight}, {b->BotLeft, b->BotRight}};
+        for (auto& pr : pairs) {
+            int n0 = pr[0][ib], n1 = pr[1][ib];
+            if (n0 < 0 || n0 >= nbm || n1 < 0 || n1 >= nbm) return fail(BGF_ERR_INVALID_ARG, "neighbour out of range");
+            if (n0 != n1 || (n0 != ib && b->level[n0] != b->level[ib]))
+                return fail(BGF_ERR_UNSUPPORTED, "multi-level block lists need the AMR kernels");
+        }
+        if (b->level[ib] != b->level[b->active[0]]) return fail(BGF_ERR_UNSUPPORTED, "blocks of different levels");
+    }
+    auto* h = new bgf_handle_s();
+    h->p = *p;
+    h->prec = p->precision;
+    h->es = p->precision == 0 ? sizeof(double) : sizeof(float);
+    h->nblk = nblk;
+    h->nblkmem = nbm;
+    h->S = (size_t)nbm * BS;
+    h->active.assign(b->active, b->active + nblk);
+    auto A = [&](void** ptr, size_t bytes) -> bool {
+        if (hipMalloc(ptr, bytes) != hipSuccess) return false;
+        return hipMemset(*ptr, 0, bytes) == hipSuccess;
+    };
+    bool ok = true;
+    ok = ok && A((void**)&h->d_active, sizeof(int) * nblk);
+    ok = ok && A((void**)&h->d_level, sizeof(int) * nbm);
+    for (int k = 0; k < 8; k++) ok = ok && A((void**)&h->d_nb[k], sizeof(int) * nbm);
+    for (int k = 0; k < 2; k++) {
+        ok = ok && A(&h->d_h[k], h->es * h->S);
+        ok = ok && A(&h->d_u[k], h->es * h->S);
+        ok = ok && A(&h->d_v[k], h->es * h->S);
+    }
+    ok = ok && A(&h->d_zb, h->es * h->S) && A(&h->d_cf, h->es * h->S);
+    ok = ok && A(&h->d_dh, h->es * h->S) && A(&h->d_dhu, h->es * h->S) && A(&h->d_dhv, h->es * h->S);
+    ok = ok && A(&h->d_blockmin, h->es * nblk);
+    ok = ok && A((void**)&h->d_volpart, sizeof(double) * nblk) && A((void**)&h->d_volout, sizeof(double));
+    ok = ok && A((void**)&h->d_loop, sizeof(bgf::DevLoop));
+    if (!ok) {
+        free_handle(h);
+        (void)hipGetLastError();
+        return fail(BGF_ERR_OOM, "device allocation failed");
+    }
+    const int* nbs[8] = {b->LeftBot, b->LeftTop, b->RightBot, b->RightTop, b->TopLeft, b->TopRight, b->BotLeft, b->BotRight};
+    bool cp = hipMemcpy(h->d_active, b->active, sizeof(int) * nblk, hipMemcpyHostToDevice) == hipSuccess;
+    cp = cp && hipMemcpy(h->d_level, b->level, sizeof(int) * nbm, hipMemcpyHostToDevice) == hipSuccess;
+    for (int k = 0; k < 8; k++) cp = cp && hipMemcpy(h->d_nb[k], nbs[k], sizeof(int) * nbm, hipMemcpyHostToDevice) == hipSuccess;
+    cp = cp && hipStreamCreate(&h->stream) == hipSuccess;
+    for (int k = 0; k < 4; k++) cp = cp && hipEventCreate(&h->ev[k]) == hipSuccess;
+    if (!cp) {
+        free_handle(h);
+        return fail(BGF_ERR_HIP, "device upload failed");
+    }
+    bgf::KParams& kp = h->kp;
+    kp.g = p->g; kp.eps = p->eps; kp.CFL = p->CFL; kp.theta = p->theta; kp.delta = p->delta;
+    kp.fc = p->precision == 0 ? coriolis<double>(p->lat) : coriolis<float>(p->lat);
+    kp.mask = p->mask;
+    kp.rain_mmh = (double)(float)p->rain;  // Rain.nowvalue is a float (src/Forcing.h)
+    kp.frictionmodel = p->frictionmodel;
+    kp.aoibnd = p->aoibnd;
+    kp.ForceMassConserve = p->ForceMassConserve;
+    kp.rain_on = p->rain_on;
+    kp.nblk = nblk;
+    kp.active = h->d_active;
+    kp.LeftBot = h->d_nb[0];
+    kp.RightBot = h->d_nb[2];
+    kp.TopLeft = h->d_nb[4];
+    kp.BotLeft = h->d_nb[6];
+    *out = h;
+    g_err.clear();
+    return BGF_OK;
+}
+
+bgf_status bgf_destroy(bgf_handle h) {
+    if (!h) return fail(BGF_ERR_INVALID_ARG, "null handle");
+    (void)hipStreamSynchronize(h->stream);
+    free_handle(h);
+    return BGF_OK;
+}
+
+bgf_status bgf_set_state(bgf_handle h, const void* zb, const void* hh, const void* u, const void* v, const void* cf) {
+    if (!h || !zb || !hh || !u || !v || !cf) return fail(BGF_ERR_INVALID_ARG, "null argument");
+    size_t bytes = h->es * h->S;
+    HIPCHK(hipMemcpyAsync(h->d_zb, zb, bytes, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->d_h[0], hh, bytes, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->d_u[0], u, bytes, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->d_v[0], v, bytes, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->d_cf, cf, bytes, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->state_set = true;
+    return BGF_OK;
+}
+
+bgf_status bgf_get_state(bgf_handle h, void* zs, void* hh, void* u, void* v) {
+    if (!h) return fail(BGF_ERR_INVALID_ARG, "null handle");
+    if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
+    size_t bytes = h->es * h->S;
+    std::vector<unsigned char> tmp(bytes), tzb;
+    auto scatter = [&](void* dst) {
+        for (int ibl = 0; ibl < h->nblk; ibl++) {
+            int ib = h->active[ibl];
+            for (int j = 0; j < BW; j++) {
+                size_t off = (size_t)memloc(0, j, ib) * h->es;
+                std::memcpy((unsigned char*)dst + off, tmp.data() + off, BW * h->es);
+            }
+        }
+    };
+    if (hh || zs) {
+        HIPCHK(hipMemcpy(tmp.data(), h->d_h[0], bytes, hipMemcpyDeviceToHost));
+        if (hh) scatter(hh);
+        if (zs) {
+            tzb.resize(bytes);
+            HIPCHK(hipMemcpy(tzb.data(), h->d_zb, bytes, hipMemcpyDeviceToHost));
+            for (int ibl = 0; ibl < h->nblk; ibl++) {
+                int ib = h->active[ibl];
+                for (int j = 0; j < BW; j++)
+                    for (int i = 0; i < BW; i++) {
+                        size_t n = memloc(i, j, ib);
+                        if (h->prec == 0) ((double*)zs)[n] = ((double*)tzb.data())[n] + ((double*)tmp.data())[n];
+                        else ((float*)zs)[n] = ((float*)tzb.data())[n] + ((float*)tmp.data())[n];
+                    }
+            }
+        }
+    }
+    if (u) { HIPCHK(hipMemcpy(tmp.data(), h->d_u[0], bytes, hipMemcpyDeviceToHost)); scatter(u); }
+    if (v) { HIPCHK(hipMemcpy(tmp.data(), h->d_v[0], bytes, hipMemcpyDeviceToHost)); scatter(v); }
+    return BGF_OK;
+}
+
+bgf_status bgf_init_dt(bgf_handle h, double dtinit, bgf_loop* loop) {
+    if (!h || !loop) return fail(BGF_ERR_INVALID_ARG, "null argument");
+    if (!h->state_set) return fail(BGF_ERR_STATE, "state not set");
+    bgf_loop L = *loop;
+    // src/Mainloop.cu:524-536
+    if (dtinit > 0) L.dtmax = dtinit / 1.5;
+    else L.dtmax = (h->p.maxlevel < 0 ? h->p.delta * (1 << -h->p.maxlevel) : h->p.delta / (1 << h->p.maxlevel)) /
+                   std::sqrt(h->p.g * 10.0);
+    bgf_status s = upload_loop(h, &L, 1e300);
+    if (s != BGF_OK) return s;
+    bgf::launch_init_dt(h->prec, h->kp, h->d_h[0], h->d_blockmin, h->d_loop, h->stream);
+    HIPCHK(hipGetLastError());
+    bgf::DevLoop D;
+    s = download_loop(h, &D);
+    if (s != BGF_OK) return s;
+    loop->dtmax = D.dt;
+    return BGF_OK;
+}
+
+bgf_status bgf_step(bgf_handle h, bgf_loop* loop) {
+    if (!h || !loop) return fail(BGF_ERR_INVALID_ARG, "null argument");
+    if (!h->state_set) return fail(BGF_ERR_STATE, "state not set");
+    if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
+    bgf_status s = upload_loop(h, loop, 1e300);
+    if (s != BGF_OK) return s;
+    s = enqueue_steps(h, 1, 1e300, 0);
+    if (s != BGF_OK) return s;
+    bgf::DevLoop D;
+    s = download_loop(h, &D);
+    if (s != BGF_OK) return s;
+    s = settle(h, 1);  // cleanup: evolv_o -> evolv (src/FlowGPU.cu:282)
+    if (s != BGF_OK) return s;
+    loop->dt = D.dt;
+    loop->dtmax = D.dtmax;
+    if (D.bad) return fail(BGF_ERR_NONFINITE_DT, "non-finite or non-positive time step");
+    return BGF_OK;
+}
+
+bgf_status bgf_run_async(bgf_handle h, const bgf_loop* loop, int nsteps, double endtime) {
+    if (!h || !loop || nsteps < 0) return fail(BGF_ERR_INVALID_ARG, "bad argument");
+    if (!h->state_set) return fail(BGF_ERR_STATE, "state not set");
+    if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
+    bgf_status s = upload_loop(h, loop, endtime);
+    if (s != BGF_OK) return s;
+    s = enqueue_steps(h, nsteps, endtime, 1);
+    if (s != BGF_OK) return s;
+    h->pending_steps = nsteps > 0 ? nsteps : -1;
+    return BGF_OK;
+}
+
+bgf_status bgf_sync(bgf_handle h, bgf_loop* loop) {
+    if (!h) return fail(BGF_ERR_INVALID_ARG, "null handle");
+    if (!h->pending_steps) return fail(BGF_ERR_STATE, "nothing pending");
+    bgf::DevLoop D;
+    h->pending_steps = 0;
+    bgf_status s = download_loop(h, &D);
+    if (s != BGF_OK) return s;
+    s = settle(h, D.steps_done);
+    if (s != BGF_OK) return s;
+    if (loop) {
+        loop->dt = D.dt;
+        loop->dtmax = D.dtmax;
+        loop->totaltime = D.totaltime;
+    }
+    if (D.bad) return fail(BGF_ERR_NONFINITE_DT, "non-finite or non-positive time step");
+    return BGF_OK;
+}
+
+bgf_status bgf_run(bgf_handle h, bgf_loop* loop, int nsteps, double endtime) {
+    bgf_status s = bgf_run_async(h, loop, nsteps, endtime);
+    if (s != BGF_OK) return s;
+    return bgf_sync(h, loop);
+}
+
+bgf_status bgf_volume(bgf_handle h, double* volume) {
+    if (!h || !volume) return fail(BGF_ERR_INVALID_ARG, "null argument");
+    if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
+    bgf::launch_volume(h->prec, h->kp, h->d_h[0], h->d_volpart, h->d_volout, h->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(volume, h->d_volout, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return BGF_OK;
+}
+
+bgf_status bgf_device_array(bgf_handle h, const char* name, void** devptr) {
+    if (!h || !name || !devptr) return fail(BGF_ERR_INVALID_ARG, "null argument");
+    std::string n(name);
+    void* p = nullptr;
+    if (n == "h") p = h->d_h[0];
+    else if (n == "u") p = h->d_u[0];
+    else if (n == "v") p = h->d_v[0];
+    else if (n == "ho") p = h->d_h[1];
+    else if (n == "uo") p = h->d_u[1];
+    else if (n == "vo") p = h->d_v[1];
+    else if (n == "zb") p = h->d_zb;
+    else if (n == "cf") p = h->d_cf;
+    else if (n == "dh") p = h->d_dh;
+    else if (n == "dhu") p = h->d_dhu;
+    else if (n == "dhv") p = h->d_dhv;
+    else return fail(BGF_ERR_INVALID_ARG, "unknown array " + n);
+    *devptr = p;
+    return BGF_OK;
+}
+
+bgf_status bgf_set_timing(bgf_handle h, int enable) {
+    if (!h) return fail(BGF_ERR_INVALID_ARG, "null handle");
+    h->timing = enable != 0;
+    h->t_pred = h->t_red = h->t_corr = 0;
+    h->t_n = 0;
+    return BGF_OK;
+}
+
+bgf_status bgf_kernel_times(bgf_handle h, int n, const char** names, double* ms, int* count) {
+    if (!h || !count) return fail(BGF_ERR_INVALID_ARG, "null argument");
+    static const char* kn[3] = {"k_predictor", "k_reduce_dt", "k_corrector"};
+    double v[3] = {h->t_pred, h->t_red, h->t_corr};
+    int c = 0;
+    for (int k = 0; k < 3 && k < n; k++) {
+        if (names) names[k] = kn[k];
+        if (ms) ms[k] = h->t_n ? v[k] / h->t_n : 0.0;
+        c++;
+    }
+    *count = c;
+    return BGF_OK;
+}
+
+// Uniform block list with the raster order of InitBlockxoyo and the
+// neighbour convention of InitBlockneighbours (src/Mesh.cu:257-433): block
+// (bx, by) is number by*nbx + bx and a missing neighbour is the block itself.
+bgf_status bgf_mesh_uniform(int nbx, int nby, double dx, int level, int dbl, int* active, int* lev, double* xo,
+                            double* yo, int* LeftBot, int* LeftTop, int* RightBot, int* RightTop, int* TopLeft,
+                            int* TopRight, int* BotLeft, int* BotRight) {
+    if (nbx <= 0 || nby <= 0 || !active || !lev || !xo || !yo) return fail(BGF_ERR_INVALID_ARG, "bad mesh arguments");
+    double levdx = level < 0 ? dx * (1 << -level) : dx / (1 << level);
+    for (int by = 0; by < nby; by++)
+        for (int bx = 0; bx < nbx; bx++) {
+            int ib = by * nbx + bx;
+            active[ib] = ib;
+            lev[ib] = level;
+            if (dbl) {
+                xo[ib] = bx * ((double)BW) * levdx + 0.5 * levdx;
+                yo[ib] = by * ((double)BW) * levdx + 0.5 * levdx;
+            } else {
+                xo[ib] = (double)(bx * ((float)BW) * (float)levdx + 0.5f * (float)levdx);
+                yo[ib] = (double)(by * ((float)BW) * (float)levdx + 0.5f * (float)levdx);
+            }
+            int l = bx > 0 ? ib - 1 : ib, r = bx < nbx - 1 ? ib + 1 : ib;
+            int bt = by > 0 ? ib - nbx : ib, t = by < nby - 1 ? ib + nbx : ib;
+            LeftBot[ib] = LeftTop[ib] = l;
+            RightBot[ib] = RightTop[ib] = r;
+            BotLeft[ib] = BotRight[ib] = bt;
+            TopLeft[ib] = TopRight[ib] = t;
+        }
+    return BGF_OK;
+}
+
+}  // extern "C"

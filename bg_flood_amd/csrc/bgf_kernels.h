@@ -1,0 +1,53 @@
+// bgf_kernels.h -- types shared by the HIP kernels and the host library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace bgf {
+
+// Device-resident Loop<T> time keeping (src/Arrays.h:358-366) plus run flags.
+struct DevLoop {
+    double nextoutputtime;
+    double dt;
+    double dtmax;
+    double totaltime;
+    int done;        // totaltime reached the end time: later steps are no-ops
+    int bad;         // a non-finite or non-positive dt was produced
+    int steps_done;  // steps executed by the current run
+    int pad;
+};
+
+// Scalar parameters and block neighbour arrays of one domain.
+struct KParams {
+    double g, eps, CFL, theta, delta, fc, mask, rain_mmh;
+    int frictionmodel, aoibnd, ForceMassConserve, rain_on, nblk;
+    const int* active;
+    const int* LeftBot;
+    const int* RightBot;
+    const int* BotLeft;
+    const int* TopLeft;
+};
+
+template <class T> struct StepBuffers {
+    const T* h;
+    const T* u;
+    const T* v;
+    T* ho;
+    T* uo;
+    T* vo;
+    const T* zb;
+    const T* cf;
+    T* dh;
+    T* dhu;
+    T* dhv;
+    T* blockmin;
+};
+
+// ev (optional): four events recorded before the predictor, after it, after
+// the reduction and after the corrector.
+void launch_step(int precision, const KParams& P, const void* buffers, DevLoop* L, double endtime, int advance_time,
+                 hipStream_t st, hipEvent_t* ev);
+void launch_init_dt(int precision, const KParams& P, const void* h, void* blockmin, DevLoop* L, hipStream_t st);
+void launch_volume(int precision, const KParams& P, const void* h, double* part, double* out, hipStream_t st);
+void launch_copy_state(int precision, const KParams& P, const void* const* src, void* const* dst, hipStream_t st);
+
+}  // namespace bgf

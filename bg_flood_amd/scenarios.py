@@ -1,0 +1,168 @@
+"""Synthetic inputs in the reference's BUQ layout for the benchmark workloads.
+
+A scenario is a dict with the Param keys (see bg_flood_amd.default_param),
+the block list (BlockP arrays) and the initial arrays zb, h, u, v, cf, each
+nblkmem*324 long in memloc order (src/MemManagement.cu:37-40).  Initial
+water follows coldstart (src/InitEvolv.cu:131-165): zs = max(zsinit, zb),
+h = max(zs - zb, 0), u = v = 0.
+"""
+import math
+
+import numpy as np
+
+from . import BS, BW, MW, default_param, mesh_uniform
+
+
+def nblkmem_for(nblk, membuffer=1.05):
+    """nblkmem = ceil(nblk * membuffer) (src/Mesh.cu:145)."""
+    return int(math.ceil(nblk * membuffer))
+
+
+def buq_from_grid(field, nblkmem, dtype):
+    """Scatters an (ny, nx) raster into BUQ interior cells of a raster-ordered uniform block list."""
+    ny, nx = field.shape
+    nby, nbx = ny // BW, nx // BW
+    out = np.zeros((nblkmem, MW, MW), dtype)
+    blocks = field.reshape(nby, BW, nbx, BW).transpose(0, 2, 1, 3).reshape(nby * nbx, BW, BW)
+    out[:nby * nbx, 1:MW - 1, 1:MW - 1] = blocks
+    return out.reshape(-1)
+
+
+def grid_from_buq(arr, nbx, nby):
+    """Inverse of buq_from_grid for the interior cells."""
+    a = np.asarray(arr).reshape(-1, MW, MW)[:nbx * nby, 1:MW - 1, 1:MW - 1]
+    return a.reshape(nby, nbx, BW, BW).transpose(0, 2, 1, 3).reshape(nby * BW, nbx * BW)
+
+
+def cell_coords(blocks, nblkmem, xo, yo, delta, dtype):
+    """x = xo + blk.xo + ix*delta, y likewise, evaluated in T like the reference tests."""
+    T = dtype
+    nb = len(blocks["active"])
+    ix = np.arange(BW, dtype=T)
+    X = np.zeros((nblkmem, MW, MW), T)
+    Y = np.zeros((nblkmem, MW, MW), T)
+    bxo = np.asarray(blocks["xo"][:nb]).astype(T)
+    byo = np.asarray(blocks["yo"][:nb]).astype(T)
+    X[:nb, 1:MW - 1, 1:MW - 1] = (T(xo) + bxo[:, None, None]) + ix[None, None, :] * T(delta)
+    Y[:nb, 1:MW - 1, 1:MW - 1] = (T(yo) + byo[:, None, None]) + ix[None, :, None] * T(delta)
+    return X.reshape(-1), Y.reshape(-1)
+
+
+def uniform(nx, ny, dx, precision="fp64", xo=0.0, yo=0.0, **kw):
+    """Param + block list of a uniform one-level nx x ny grid (multiples of 16)."""
+    assert nx % BW == 0 and ny % BW == 0
+    nbx, nby = nx // BW, ny // BW
+    dbl = precision == "fp64"
+    blocks = mesh_uniform(nbx, nby, dx, 0, dbl)
+    nblk = nbx * nby
+    p = default_param(dx=dx, delta=dx, xo=xo, yo=yo, nblk=nblk, nblkmem=nblkmem_for(nblk), precision=precision,
+                      **kw)
+    return p, blocks, nbx, nby
+
+
+def gaussian_hump(n=256, dx=10.0, depth=10.0, amp=1.0, radius=200.0, precision="fp64", aoibnd=0):
+    """Uniform n x n grid, flat bed zb=-depth, zs = amp*exp(-r^2/(2*radius^2)) at the centre,
+    u = v = 0, frictionless, walls: the small plain-method case."""
+    p, blocks, nbx, nby = uniform(n, n, dx, precision, aoibnd=aoibnd, frictionmodel=0, cf=0.0)
+    T = np.float64 if precision == "fp64" else np.float32
+    S = p["nblkmem"] * BS
+    X, Y = cell_coords(blocks, p["nblkmem"], 0.0, 0.0, dx, T)
+    xc = T(n * dx / 2.0)
+    zb = np.full(S, T(-depth), T)
+    zs = np.maximum(T(0.0), zb) + T(amp) * np.exp(T(-1.0) * ((X - xc) * (X - xc) + (Y - xc) * (Y - xc))
+                                                   / (T(2.0) * T(radius) * T(radius)))
+    h = np.maximum(zs - zb, T(0.0)).astype(T)
+    inter = np.zeros((p["nblkmem"], MW, MW), bool)
+    inter[:p["nblk"], 1:MW - 1, 1:MW - 1] = True
+    h[~inter.reshape(-1)] = 0
+    z = np.zeros(S, T)
+    return dict(param=p, blocks=blocks, nbx=nbx, nby=nby, zb=zb, h=h, u=z.copy(), v=z.copy(), cf=z.copy())
+
+
+def reference_hump(precision="fp64", aoibnd=1):
+    """The reference's Gaussian hump verification setup (src/Testing.cu:418-600): 256^2 cells of
+    dx = 1/256 on [-0.5,0.5]^2, zb = 0, zsinit = 0.1, bump a = 1, cc = 0.05, frictionless,
+    run to t = 0.1 s."""
+    dx = 1.0 / 256
+    p, blocks, nbx, nby = uniform(256, 256, dx, precision, xo=-0.5, yo=-0.5, aoibnd=aoibnd, frictionmodel=0,
+                                  cf=0.0)
+    T = np.float64 if precision == "fp64" else np.float32
+    S = p["nblkmem"] * BS
+    X, Y = cell_coords(blocks, p["nblkmem"], -0.5, -0.5, dx, T)
+    zb = np.zeros(S, T)
+    cc = T(0.05)
+    zs = np.maximum(T(0.1) + T(0.0), zb) + T(1.0) * np.exp(T(-1.0) * (X * X + Y * Y) / (T(2.0) * cc * cc))
+    h = np.maximum(zs - zb, T(0.0)).astype(T)
+    z = np.zeros(S, T)
+    return dict(param=p, blocks=blocks, nbx=nbx, nby=nby, zb=zb, h=h, u=z.copy(), v=z.copy(), cf=z.copy(),
+                nextoutputtime=0.1)
+
+
+def value_noise(nx, ny, seed, octaves=6, base=4, lacunarity=2, gain=0.5):
+    """Sum of `octaves` bilinearly interpolated seeded lattices (base x base cells across the
+    domain at the first octave), normalised to [0, 1]."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros((ny, nx), np.float64)
+    amp = 1.0
+    cells = base
+    ys = (np.arange(ny) + 0.5) / ny
+    xs = (np.arange(nx) + 0.5) / nx
+    for _ in range(octaves):
+        lat = rng.random((cells + 1, cells + 1))
+        fy = ys * cells
+        fx = xs * cells
+        iy = np.minimum(fy.astype(np.int64), cells - 1)
+        ix = np.minimum(fx.astype(np.int64), cells - 1)
+        ty = (fy - iy)[:, None]
+        tx = (fx - ix)[None, :]
+        # bilinear interpolation, separable: rows first (ny x cells+1), then columns
+        rows = lat[iy] * (1 - ty) + lat[iy + 1] * ty
+        if nx % cells == 0:
+            rep = nx // cells
+            left = np.repeat(rows[:, :-1], rep, axis=1)
+            right = np.repeat(rows[:, 1:], rep, axis=1)
+        else:
+            left, right = rows[:, ix], rows[:, ix + 1]
+        left *= amp * (1 - tx)
+        right *= amp * tx
+        out += left
+        out += right
+        del left, right
+        amp *= gain
+        cells *= lacunarity
+    out -= out.min()
+    out /= max(out.max(), 1e-300)
+    return out
+
+
+def rain_terrain(n=8192, dx=5.0, amp=20.0, slope=0.002, rain=50.0, manning=0.03, seed=1, precision="fp32",
+                 aoibnd=1):
+    """Rain on a seeded value-noise bed plus a planar slope, initially dry, Manning friction,
+    outflow (Neumann) boundaries: the HBM-scale wet/dry workload."""
+    p, blocks, nbx, nby = uniform(n, n, dx, precision, aoibnd=aoibnd, frictionmodel=-1, cf=manning, rain_on=1,
+                                  rain=rain)
+    T = np.float64 if precision == "fp64" else np.float32
+    bed = amp * value_noise(n, n, seed)
+    x = (np.arange(n) + 0.5) * dx
+    bed = bed + slope * x[None, :]
+    zb = buq_from_grid(bed.astype(T), p["nblkmem"], T)
+    del bed
+    S = p["nblkmem"] * BS
+    h = np.zeros(S, T)
+    cf = np.full(S, T(manning), T)
+    return dict(param=p, blocks=blocks, nbx=nbx, nby=nby, zb=zb, h=h, u=np.zeros(S, T), v=np.zeros(S, T), cf=cf)
+
+
+def lake_at_rest(n=64, dx=1.0, level=0.5, amp=1.0, seed=3, precision="fp64", aoibnd=0):
+    """Still water at zs = level over a rough bed that emerges above it (wet/dry)."""
+    p, blocks, nbx, nby = uniform(n, n, dx, precision, aoibnd=aoibnd, frictionmodel=0, cf=0.0)
+    T = np.float64 if precision == "fp64" else np.float32
+    bed = amp * value_noise(n, n, seed, octaves=4) - 0.2
+    zb = buq_from_grid(bed.astype(T), p["nblkmem"], T)
+    S = p["nblkmem"] * BS
+    h = np.maximum(T(level) - zb, T(0)).astype(T)
+    inter = np.zeros((p["nblkmem"], MW, MW), bool)
+    inter[:p["nblk"], 1:MW - 1, 1:MW - 1] = True
+    h[~inter.reshape(-1)] = 0
+    z = np.zeros(S, T)
+    return dict(param=p, blocks=blocks, nbx=nbx, nby=nby, zb=zb, h=h, u=z.copy(), v=z.copy(), cf=z.copy())

@@ -1,0 +1,162 @@
+/*
+ * bgflood_amd.h -- C ABI of the MI355X-native BG_Flood timestep library.
+ *
+ * The library replaces the per-timestep shallow-water update of BG_Flood,
+ * FlowGPU<T> / FlowCPU<T> (src/FlowGPU.cu:14-315, src/FlowCPU.cu:15-253), on
+ * its block-uniform quadtree: 16x16-cell blocks stored with a one-cell halo
+ * in 18x18 tiles, index (i+1) + (j+1)*18 + ib*324 (src/MemManagement.cu:37-40).
+ * All arguments are plain pointers and sizes; no C++ or torch types cross it.
+ * Every entry point returns a bgf_status; none calls exit().
+ */
+#ifndef BGFLOOD_AMD_H
+#define BGFLOOD_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes.  The reference has none: CUDA_CHECK prints and exits
+ * (src/Setup_GPU.cu:169-182). */
+typedef enum {
+    BGF_OK = 0,
+    BGF_ERR_INVALID_ARG = 1,
+    BGF_ERR_OOM = 2,
+    BGF_ERR_HIP = 3,
+    BGF_ERR_NONFINITE_DT = 4,
+    BGF_ERR_UNSUPPORTED = 5,
+    BGF_ERR_NO_DEVICE = 6,
+    BGF_ERR_STATE = 7
+} bgf_status;
+
+/* Arithmetic of a handle.  The reference picks T from Param.doubleprecision
+ * (src/BG_Flood.cu:101-109). */
+typedef enum {
+    BGF_FP64 = 0,       /* T = double, operation-for-operation the reference CPU step */
+    BGF_FP32 = 1,       /* T = float,  operation-for-operation the reference CPU step */
+    BGF_FP32_FAST = 2   /* T = float, hardware reciprocal / square root, fused multiply-add */
+} bgf_precision;
+
+/* Subset of class Param (src/Param.h:10-239) that the Kurganov step reads. */
+typedef struct {
+    double g;             /* Param.g */
+    double eps;           /* Param.eps, drying height */
+    double CFL;           /* Param.CFL */
+    double theta;         /* Param.theta, minmod2 limiter parameter */
+    double dx;            /* Param.dx */
+    double delta;         /* Param.delta (== dx on Cartesian grids) */
+    double xo, yo;        /* Param.xo, Param.yo */
+    double lat;           /* Param.lat (Coriolis on Cartesian grids) */
+    double mask;          /* Param.mask: cells with zb >= mask get no rain */
+    int frictionmodel;    /* Param.frictionmodel: -1 Manning, 0 quadratic, 1 Smart */
+    int engine;           /* Param.engine: only 2 (Kurganov) is provided */
+    int aoibnd;           /* Param.aoibnd: remainder boundary type, 0 wall, 1 Neumann */
+    int ForceMassConserve;/* Param.ForceMassConserve */
+    int wetdryfix;        /* Param.wetdryfix */
+    int spherical;        /* Param.spherical: must be 0 */
+    int nblk;             /* Param.nblk: active blocks */
+    int nblkmem;          /* Param.nblkmem: allocated blocks (>= nblk) */
+    int minlevel, maxlevel;
+    int rain_on;          /* a uniform rain forcing is present (Forcing.Rain) */
+    double rain;          /* Rain.nowvalue, mm/h */
+    int precision;        /* bgf_precision */
+} bgf_param;
+
+/* BlockP<T> (src/Arrays.h:233-249): nblkmem entries each; active[ibl] for
+ * ibl < nblk gives the block index ib.  A neighbour equal to the block itself
+ * means "no neighbour" (src/Mesh.cu:390-397).  xo/yo are block origins. */
+typedef struct {
+    const int* active;
+    const int* level;
+    const int *LeftBot, *LeftTop, *RightBot, *RightTop;
+    const int *TopLeft, *TopRight, *BotLeft, *BotRight;
+    const double *xo, *yo;
+} bgf_blocks;
+
+/* Loop<T> time keeping (src/Arrays.h:358-366). */
+typedef struct {
+    double nextoutputtime;
+    double dt;
+    double dtmax;
+    double totaltime;
+} bgf_loop;
+
+typedef struct bgf_handle_s* bgf_handle;
+
+/* Last error message of the calling thread ("" if none). */
+const char* bgf_last_error(void);
+
+/* Library version string. */
+const char* bgf_version(void);
+
+/* Number of visible GPUs (0 without one). */
+int bgf_device_count(void);
+
+/* Allocates the device Model<T> for one domain on the current device and
+ * uploads the block description.  Replaces AllocateGPU + CopytoGPU of
+ * SetupGPU (src/Setup_GPU.cu:15-157). */
+bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out);
+
+/* Frees every device buffer of the handle. */
+bgf_status bgf_destroy(bgf_handle h);
+
+/* Uploads the state.  Arrays hold nblkmem*324 values of T (double for
+ * BGF_FP64, float otherwise) in the reference layout; halo cells are ignored.
+ * zb and cf are static: zb is the bed (Model.zb), cf the friction coefficient
+ * (Model.cf).  zs is not an input: it is zb + h (src/Halo.cu:72-107). */
+bgf_status bgf_set_state(bgf_handle h, const void* zb, const void* hh, const void* u, const void* v,
+                         const void* cf);
+
+/* Downloads zs, h, u, v (any pointer may be NULL) for the interior cells;
+ * halo cells of the outputs are left untouched. */
+bgf_status bgf_get_state(bgf_handle h, void* zs, void* hh, void* u, void* v);
+
+/* Computes the first time step like initdt (src/Mainloop.cu:515-554):
+ * loop->dtmax is set from dtinit (if > 0) and the per-cell
+ * delta/sqrt(g*max(h,eps)) minimum, with the clamp of CalctimestepCPU. */
+bgf_status bgf_init_dt(bgf_handle h, double dtinit, bgf_loop* loop);
+
+/* One predictor-corrector step: FlowGPU<T>(XParam, XLoop, XForcing, XModel)
+ * (src/FlowGPU.cu:14-315) with the CPU semantics of FlowCPU
+ * (src/FlowCPU.cu:15-253).  Reads loop->totaltime, nextoutputtime and
+ * dtmax; writes loop->dt and loop->dtmax like the reference.  The caller
+ * advances totaltime (src/Mainloop.cu:60).  Synchronous. */
+bgf_status bgf_step(bgf_handle h, bgf_loop* loop);
+
+/* nsteps iterations of {bgf_step; totaltime += dt} that stay on the GPU
+ * (device-resident loop state, no host round trip per step).  Steps stop
+ * early, as no-ops, once totaltime >= endtime.  Returns the final loop. */
+bgf_status bgf_run(bgf_handle h, bgf_loop* loop, int nsteps, double endtime);
+
+/* Enqueues nsteps steps like bgf_run without waiting for them, on the
+ * handle's stream.  bgf_sync waits and returns the loop state. */
+bgf_status bgf_run_async(bgf_handle h, const bgf_loop* loop, int nsteps, double endtime);
+bgf_status bgf_sync(bgf_handle h, bgf_loop* loop);
+
+/* Sum of h * delta^2 over active cells, accumulated in FP64 on the GPU. */
+bgf_status bgf_volume(bgf_handle h, double* volume);
+
+/* Device pointers of the handle's Model arrays by name ("h", "u", "v", "zb",
+ * "cf", "dh", "dhu", "dhv"), for callers that keep Model<T> views. */
+bgf_status bgf_device_array(bgf_handle h, const char* name, void** devptr);
+
+/* Average duration, in milliseconds, of each kernel of the last bgf_run
+ * call, measured with HIP events on the handle's stream when timing was
+ * enabled with bgf_set_timing(h, 1).  names/ms receive up to n entries;
+ * *count receives the number written. */
+bgf_status bgf_set_timing(bgf_handle h, int enable);
+bgf_status bgf_kernel_times(bgf_handle h, int n, const char** names, double* ms, int* count);
+
+/* Builds the block list of a uniform one-level grid of nbx x nby blocks at
+ * level `level`, in the raster order and with the neighbour convention of
+ * InitBlockxoyo / InitBlockneighbours (src/Mesh.cu:257-433), in O(n).
+ * xo/yo are computed in T (double if dbl, float otherwise) like the
+ * reference and returned widened to double.  All arrays have nbx*nby
+ * entries. */
+bgf_status bgf_mesh_uniform(int nbx, int nby, double dx, int level, int dbl, int* active, int* lev,
+                            double* xo, double* yo, int* LeftBot, int* LeftTop, int* RightBot,
+                            int* RightTop, int* TopLeft, int* TopRight, int* BotLeft, int* BotRight);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,64 @@
+"""C-ABI library checks that need no GPU: it loads, exports every declared symbol, and its
+O(n) uniform mesh builder reproduces the reference's O(n^2) block/neighbour construction."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import bg_flood_amd as bgf
+from oracle import pyoracle as po
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for hdr in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(hdr).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(bgf_[a-z_0-9]+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    L = bgf.lib()
+    names = declared_symbols()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_version_and_error_strings():
+    L = bgf.lib()
+    assert b"gfx950" in L.bgf_version()
+    assert isinstance(L.bgf_last_error(), bytes)
+
+
+@pytest.mark.parametrize("nbx,nby,dx,dbl", [(16, 16, 10.0, True), (16, 16, 10.0, False), (5, 3, 0.7, True),
+                                             (1, 7, 5.0, False), (7, 1, 1.0 / 256, True), (23, 17, 5.0, False)])
+def test_mesh_uniform_matches_reference_construction(nbx, nby, dx, dbl):
+    mine = bgf.mesh_uniform(nbx, nby, dx, 0, dbl)
+    p = po.make_param(dx=dx, nx=nbx * 16, ny=nby * 16, doubleprecision=int(dbl))
+    ref = po.mesh_uniform(p)
+    for k in bgf.BLOCK_KEYS:
+        assert np.array_equal(mine[k], ref[k]), k
+    assert np.array_equal(mine["xo"], ref["xo"])
+    assert np.array_equal(mine["yo"], ref["yo"])
+
+
+def test_mesh_uniform_rejects_bad_sizes():
+    with pytest.raises(bgf.BgfError):
+        bgf.mesh_uniform(0, 3, 1.0)
+
+
+def test_domain_without_device_fails_loudly():
+    if bgf.lib().bgf_device_count() > 0:
+        pytest.skip("a GPU is present")
+    from bg_flood_amd import scenarios
+    sc = scenarios.gaussian_hump(n=32)
+    with pytest.raises(bgf.BgfError):
+        bgf.Domain(sc["param"], sc["blocks"])

