@@ -107,6 +107,18 @@ def load_traffic(name):
     return {}
 
 
+def copy_probe(nbytes=1 << 30, reps=5):
+    """A plain device-to-device copy of a known size: the FETCH_SIZE / WRITE_SIZE counters of
+    this kernel give the factor between counted and actual HBM bytes."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log("copy probe: %d bytes read and %d bytes written per copy, %d copies" % (nbytes, nbytes, reps))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,7 +128,12 @@ def main():
     ap.add_argument("--precision", default="fp32_fast", choices=["fp32_fast", "fp32", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (scale reference for HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is described in DESIGN.md "
               "and not launched (requested --gpus %d)" % args.gpus, file=sys.stderr)

@@ -50,7 +50,6 @@ struct bgf_handle_s {
     int *d_nb[8] = {nullptr};  // LeftBot LeftTop RightBot RightTop TopLeft TopRight BotLeft BotRight
     void *d_h[2] = {nullptr, nullptr}, *d_u[2] = {nullptr, nullptr}, *d_v[2] = {nullptr, nullptr};
     void *d_zb = nullptr, *d_cf = nullptr, *d_dh = nullptr, *d_dhu = nullptr, *d_dhv = nullptr;
-    void* d_blockmin = nullptr;
     double *d_volpart = nullptr, *d_volout = nullptr;
     bgf::DevLoop* d_loop = nullptr;
     bgf::KParams kp;
@@ -90,7 +89,7 @@ void free_handle(bgf_handle h) {
     F(h->d_active); F(h->d_level);
     for (int k = 0; k < 8; k++) F(h->d_nb[k]);
     for (int k = 0; k < 2; k++) { F(h->d_h[k]); F(h->d_u[k]); F(h->d_v[k]); }
-    F(h->d_zb); F(h->d_cf); F(h->d_dh); F(h->d_dhu); F(h->d_dhv); F(h->d_blockmin);
+    F(h->d_zb); F(h->d_cf); F(h->d_dh); F(h->d_dhu); F(h->d_dhv);
     F(h->d_volpart); F(h->d_volout); F(h->d_loop);
     for (int k = 0; k < 4; k++) if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -105,6 +104,10 @@ bgf_status upload_loop(bgf_handle h, const bgf_loop* loop, double endtime) {
     L.dtmax = loop->dtmax;
     L.totaltime = loop->totaltime;
     L.done = loop->totaltime >= endtime ? 1 : 0;
+    for (int k = 0; k < bgf::NSLOT; k++) {
+        L.slot32[k] = 0x7f800000u;
+        L.slot64[k] = 0x7ff0000000000000ull;
+    }
     HIPCHK(hipMemcpyAsync(h->d_loop, &L, sizeof(L), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return BGF_OK;
@@ -124,14 +127,12 @@ void fill_buffers(bgf_handle h, int in, void* out_storage) {
         B->ho = (double*)h->d_h[out]; B->uo = (double*)h->d_u[out]; B->vo = (double*)h->d_v[out];
         B->zb = (const double*)h->d_zb; B->cf = (const double*)h->d_cf;
         B->dh = (double*)h->d_dh; B->dhu = (double*)h->d_dhu; B->dhv = (double*)h->d_dhv;
-        B->blockmin = (double*)h->d_blockmin;
     } else {
         auto* B = (bgf::StepBuffers<float>*)out_storage;
         B->h = (const float*)h->d_h[in]; B->u = (const float*)h->d_u[in]; B->v = (const float*)h->d_v[in];
         B->ho = (float*)h->d_h[out]; B->uo = (float*)h->d_u[out]; B->vo = (float*)h->d_v[out];
         B->zb = (const float*)h->d_zb; B->cf = (const float*)h->d_cf;
         B->dh = (float*)h->d_dh; B->dhu = (float*)h->d_dhu; B->dhv = (float*)h->d_dhv;
-        B->blockmin = (float*)h->d_blockmin;
     }
 }
 
@@ -225,7 +226,6 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
     }
     ok = ok && A(&h->d_zb, h->es * h->S) && A(&h->d_cf, h->es * h->S);
     ok = ok && A(&h->d_dh, h->es * h->S) && A(&h->d_dhu, h->es * h->S) && A(&h->d_dhv, h->es * h->S);
-    ok = ok && A(&h->d_blockmin, h->es * nblk);
     ok = ok && A((void**)&h->d_volpart, sizeof(double) * nblk) && A((void**)&h->d_volout, sizeof(double));
     ok = ok && A((void**)&h->d_loop, sizeof(bgf::DevLoop));
     if (!ok) {
@@ -329,7 +329,7 @@ bgf_status bgf_init_dt(bgf_handle h, double dtinit, bgf_loop* loop) {
                    std::sqrt(h->p.g * 10.0);
     bgf_status s = upload_loop(h, &L, 1e300);
     if (s != BGF_OK) return s;
-    bgf::launch_init_dt(h->prec, h->kp, h->d_h[0], h->d_blockmin, h->d_loop, h->stream);
+    bgf::launch_init_dt(h->prec, h->kp, h->d_h[0], h->d_loop, h->stream);
     HIPCHK(hipGetLastError());
     bgf::DevLoop D;
     s = download_loop(h, &D);

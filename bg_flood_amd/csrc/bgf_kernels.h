@@ -4,7 +4,10 @@
 
 namespace bgf {
 
-// Device-resident Loop<T> time keeping (src/Arrays.h:358-366) plus run flags.
+constexpr int NSLOT = 64;  // block-minimum dt slots (atomic min, spread to avoid contention)
+
+// Device-resident Loop<T> time keeping (src/Arrays.h:358-366), run flags and
+// the dt reduction slots (bit patterns of positive floats; +inf when empty).
 struct DevLoop {
     double nextoutputtime;
     double dt;
@@ -14,6 +17,8 @@ struct DevLoop {
     int bad;         // a non-finite or non-positive dt was produced
     int steps_done;  // steps executed by the current run
     int pad;
+    unsigned int slot32[NSLOT];
+    unsigned long long slot64[NSLOT];
 };
 
 // Scalar parameters and block neighbour arrays of one domain.
@@ -39,14 +44,13 @@ template <class T> struct StepBuffers {
     T* dh;
     T* dhu;
     T* dhv;
-    T* blockmin;
 };
 
 // ev (optional): four events recorded before the predictor, after it, after
 // the reduction and after the corrector.
 void launch_step(int precision, const KParams& P, const void* buffers, DevLoop* L, double endtime, int advance_time,
                  hipStream_t st, hipEvent_t* ev);
-void launch_init_dt(int precision, const KParams& P, const void* h, void* blockmin, DevLoop* L, hipStream_t st);
+void launch_init_dt(int precision, const KParams& P, const void* h, DevLoop* L, hipStream_t st);
 void launch_volume(int precision, const KParams& P, const void* h, double* part, double* out, hipStream_t st);
 void launch_copy_state(int precision, const KParams& P, const void* const* src, void* const* dst, hipStream_t st);
 

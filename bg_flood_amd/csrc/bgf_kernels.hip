@@ -4,20 +4,22 @@
 // One workgroup of 256 threads (four wave64s) owns one 16x16 block; thread t
 // owns cell (t % 16, t / 16).  Per step:
 //
-//   k_predictor  gathers h, u, v, zb on a 20x20 tile (two cells of each
-//                neighbour block, or the block's own edge cell at a domain
-//                boundary) straight from the neighbours into LDS: this is the
+//   k_predictor  gathers h, zs = zb + h, u, v on a 20x20 tile (two cells of
+//                each neighbour block, or the block's own edge cell at a
+//                domain boundary) straight from the neighbours into LDS: the
 //                halo fill of src/Halo.cu:586-616 plus the distance-2 reads of
 //                gradientHalo (src/Gradients.cu:2713-3294) without a halo
 //                pass.  It then forms the theta-minmod gradients
 //                (src/Gradients.cu:921-957), the 17x16 X and 16x17 Y
 //                Kurganov-Petrova faces (src/Kurganov.cu:474-641,
-//                1404-1546), the wall patch (src/Boundary.cu:587-760), the
-//                tendencies (src/Advection.cu:160-254), and the block's
-//                minimum CFL step (src/Advection.cu:470-498).
-//   k_reduce_dt  one workgroup: global minimum, 1.5x growth clamp and
-//                output-time landing in FP64 (src/Advection.cu:511-535),
-//                then totaltime += dt (src/Mainloop.cu:60) on the GPU.
+//                1404-1546), the tendencies with the wall patch folded in
+//                (src/Advection.cu:160-254, src/Boundary.cu:587-760), and the
+//                block's minimum CFL step (src/Advection.cu:470-498), which
+//                it merges into one of 64 device slots with an atomic min.
+//   k_reduce_dt  one wave: minimum of the slots and 1/epsi, the 1.5x growth
+//                clamp and the output-time landing in FP64
+//                (src/Advection.cu:511-535), then totaltime += dt
+//                (src/Mainloop.cu:60), all on the GPU.
 //   k_corrector  rebuilds U* = U + dt/2 L(U) (src/Advection.cu:332-385) on
 //                its own 20x20 tile from U and the predictor tendencies, so
 //                no U* array is written; forms the corrector fluxes of U*,
@@ -30,11 +32,15 @@
 // dh, dhu, dhv; corrector reads h, u, v, dh, dhu, dhv, zb, cf and writes
 // h, u, v: 18 words.  Neighbour strips are re-read from cache.
 //
+// LDS holds array-of-structures quadruples, so a cell's state, a cell's
+// gradients and a face's four fluxes are each one 16-byte (FP32) access.
+//
 // EXACT kernels (FAST = false) perform every operation of the reference CPU
 // step in the same order and precision (this file is compiled with
 // -ffp-contract=off; divisions and square roots are IEEE), so they reproduce
 // the CPU step bit for bit.  FAST kernels use the hardware reciprocal and
-// square root and fused multiply-adds.
+// square root, fused multiply-adds, and carry the limited slopes as half-cell
+// increments (0.5*minmod instead of minmod/delta followed by *delta/2).
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -58,37 +64,37 @@ __device__ __forceinline__ int fyi(int x, int yf) { return yf * 16 + x; }       
 template <class T> __device__ __forceinline__ T dmax(T a, T b) { return a > b ? a : b; }
 template <class T> __device__ __forceinline__ T dmin(T a, T b) { return a < b ? a : b; }
 
+// A cell's (h, zs, u, v), a cell's four gradients, or a face's four fluxes.
+template <class T> struct alignas(4 * sizeof(T)) Q4 {
+    T a, b, c, d;
+};
+
 // Division and square root: IEEE in EXACT kernels, hardware approximations in FAST ones.
 template <class T, bool FAST> struct Ar {
-    __device__ static __forceinline__ T div(T a, T b) { return a / b; }
     __device__ static __forceinline__ T rcp(T a) { return T(1.0) / a; }
     __device__ static __forceinline__ T sqrt_(T a) { return sqrt(a); }
 };
 template <> struct Ar<float, true> {
-    __device__ static __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     __device__ static __forceinline__ float rcp(float a) { return __builtin_amdgcn_rcpf(a); }
     __device__ static __forceinline__ float sqrt_(float a) { return __builtin_amdgcn_sqrtf(a); }
 };
 
-// src/Util_CPU.cu:243-263
+// src/Util_CPU.cu:243-263 without branches.  s0 < s1 < s2 holds exactly when
+// theta*(s1-s0) > 0 and theta*(s2-s1) > 0 (theta >= 1, IEEE subtraction is
+// zero only for equal operands), and the reference's two-step min (max) of
+// d1, d2, d3 is their three-way min (max).
 template <class T> __device__ __forceinline__ T minmod2(T theta, T s0, T s1, T s2) {
-    if (s0 < s1 && s1 < s2) {
-        T d1 = theta * (s1 - s0);
-        T d2 = (s2 - s0) / T(2.0);
-        T d3 = theta * (s2 - s1);
-        if (d2 < d1) d1 = d2;
-        return dmin(d1, d3);
-    }
-    if (s0 > s1 && s1 > s2) {
-        T d1 = theta * (s1 - s0), d2 = (s2 - s0) / T(2.0), d3 = theta * (s2 - s1);
-        if (d2 > d1) d1 = d2;
-        return dmax(d1, d3);
-    }
-    return T(0.0);
+    T d1 = theta * (s1 - s0);
+    T d2 = (s2 - s0) / T(2.0);
+    T d3 = theta * (s2 - s1);
+    T mn = dmin(dmin(d1, d2), d3);
+    T mx = dmax(dmax(d1, d2), d3);
+    T r = (d1 < T(0.0) && d3 < T(0.0)) ? mx : T(0.0);
+    return (d1 > T(0.0) && d3 > T(0.0)) ? mn : r;
 }
 
 template <class T> struct Cst {
-    T g, ga, CFL, theta, epsi, inv_epsi, eps_w, eps, delta, dxh, cmdinv, fc, rainrate;
+    T g, ga, CFL, theta, epsi, inv_epsi, eps_w, eps, delta, dxh, cmdinv, fc, rainrate, cfl_delta, half;
     double eps_d;
 };
 
@@ -107,209 +113,231 @@ template <class T, bool FAST> __device__ __forceinline__ Cst<T> make_cst(const K
     c.dxh = c.delta * T(0.5);
     c.cmdinv = T(1.0) / (T(1.0) * c.delta);
     c.fc = T(P.fc);
-    // rain depth rate; the per-step depth is max(R/1000/3600*dt, 0)
     c.rainrate = T(P.rain_mmh);
+    c.cfl_delta = c.CFL * c.delta;
+    c.half = T(0.5);
     return c;
 }
 
 // Hydrostatic reconstruction and central-upwind flux at one face between the
-// "n" (left/bottom) cell and the "i" (right/top) cell.  Along Y the roles of
-// u and v swap.  src/Kurganov.cu:62-164 and 1865-1911.
-template <class T, bool FAST>
-__device__ __forceinline__ bool kurg_face(const Cst<T>& C, T hi, T hn, T zsi, T zsn, T dhi, T dhn, T dzsi, T dzsn,
-                                          T ui, T un, T dui, T dun, T vi, T vn, T dvi, T dvn,
-                                          T& Fh, T& Fq, T& S, T& Fv, T& dt) {
+// "n" (left/bottom) cell and the "i" (right/top) cell, src/Kurganov.cu:62-164
+// and 1865-1911.  Along Y the roles of u and v swap.  si/sn hold the cells'
+// limited slopes of (h, zs, un, ut): gradients in EXACT kernels, half-cell
+// increments in FAST ones.
+template <class T, bool FAST, bool WANT_DT>
+__device__ __forceinline__ bool kurg_face(const Cst<T>& C, T hi, T hn, T zsi, T zsn, T ui, T un, T vi, T vn,
+                                          const Q4<T>& si, const Q4<T>& sn, Q4<T>& F, T& dt) {
     typedef Ar<T, FAST> A;
-    if (hi > C.eps_w || hn > C.eps_w) {
+    const bool wet = hi > C.eps_w || hn > C.eps_w;
+    T zi = zsi - hi;
+    T zn = zsn - hn;
+    T zl, zr, hl, up, hr, um, dzi, dzn;
+    if (FAST) {
+        dzi = si.b - si.a;  // zi - zl
+        dzn = sn.b - sn.a;  // zr - zn
+        zl = zi - dzi;
+        zr = zn + dzn;
+        hl = hi - si.a;
+        up = ui - si.c;
+        hr = hn + sn.a;
+        um = un + sn.c;
+    } else {
         const T dx = C.dxh;
-        T zi = zsi - hi;
-        T zl = zi - dx * (dzsi - dhi);
-        T zn = zsn - hn;
-        T zr = zn + dx * (dzsn - dhn);
-        T zlr = dmax(zl, zr);
-        T hl = hi - dx * dhi;
-        T up = ui - dx * dui;
-        T hp = dmax(T(0.0), hl + zl - zlr);
-        T hr = hn + dx * dhn;
-        T um = un + dx * dun;
-        T hm = dmax(T(0.0), hr + zr - zlr);
-        // KurgSolver
-        T cp = A::sqrt_(C.g * hp);
-        T cmo = A::sqrt_(C.g * hm);
-        T ap = dmax(dmax(up + cp, um + cmo), T(0.0));
-        T am = dmin(dmin(up - cp, um - cmo), T(0.0));
-        T qm = hm * um;
-        T qp = hp * up;
-        T a = dmax(ap, -am);
-        T fh, fu;
-        if (a > C.epsi) {
-            T ad = A::rcp(ap - am);
-            T apm = ap * am;
-            fh = (ap * qm - am * qp + apm * (hp - hm)) * ad;
-            fu = (ap * (qm * um + C.ga * (hm * hm)) - am * (qp * up + C.ga * (hp * hp)) + apm * (qp - qm)) * ad;
-            dt = A::div(C.CFL * C.delta, a);
-        } else {
-            fh = T(0.0);
-            fu = T(0.0);
-            dt = C.inv_epsi;
-        }
-        T fv = (fh > T(0.0) ? vn + dx * dvn : vi - dx * dvi) * fh;
-        T sl = C.ga * (hp * hp - hl * hl + (hl + hi) * (zi - zl));
-        T sr = C.ga * (hm * hm - hr * hr + (hr + hn) * (zn - zr));
-        Fh = fh;
-        Fq = fu - sl;
-        S = fu - sr;
-        Fv = fv;
-        return true;
+        zl = zi - dx * (si.b - si.a);
+        zr = zn + dx * (sn.b - sn.a);
+        hl = hi - dx * si.a;
+        up = ui - dx * si.c;
+        hr = hn + dx * sn.a;
+        um = un + dx * sn.c;
+        dzi = zi - zl;
+        dzn = zr - zn;
     }
-    Fh = T(0.0);
-    Fq = T(0.0);
-    S = T(0.0);
-    Fv = T(0.0);
-    dt = C.inv_epsi;
-    return false;
-}
-
-// Source cell of tile position (x, y) for a one-level grid: the block's own
-// cell, the neighbour's cell, or (no neighbour) the block's own edge cell.
-__device__ __forceinline__ int src_cell(int x, int y, int ib, int nL, int nR, int nB, int nT) {
-    if (x < 0) return nL == ib ? memloc(0, y, ib) : memloc(BW + x, y, nL);
-    if (x >= BW) return nR == ib ? memloc(BW - 1, y, ib) : memloc(x - BW, y, nR);
-    if (y < 0) return nB == ib ? memloc(x, 0, ib) : memloc(x, BW + y, nB);
-    if (y >= BW) return nT == ib ? memloc(x, BW - 1, ib) : memloc(x, y - BW, nT);
-    return memloc(x, y, ib);
-}
-
-// The 128 strip positions (two deep on each side) of the 20x20 tile.
-__device__ __forceinline__ void strip_pos(int k, int& x, int& y) {
-    int s = k >> 4, j = k & 15;
-    switch (s) {
-        case 0: x = -1; y = j; break;
-        case 1: x = -2; y = j; break;
-        case 2: x = BW; y = j; break;
-        case 3: x = BW + 1; y = j; break;
-        case 4: x = j; y = -1; break;
-        case 5: x = j; y = -2; break;
-        case 6: x = j; y = BW; break;
-        default: x = j; y = BW + 1; break;
+    T zlr = dmax(zl, zr);
+    T hp = dmax(T(0.0), hl + zl - zlr);
+    T hm = dmax(T(0.0), hr + zr - zlr);
+    // KurgSolver
+    T cp = A::sqrt_(C.g * hp);
+    T cmo = A::sqrt_(C.g * hm);
+    T ap = dmax(dmax(up + cp, um + cmo), T(0.0));
+    T am = dmin(dmin(up - cp, um - cmo), T(0.0));
+    T qm = hm * um;
+    T qp = hp * up;
+    T a = dmax(ap, -am);
+    const bool flow = a > C.epsi;
+    T ad = A::rcp(ap - am);
+    T apm = ap * am;
+    T fh, fu;
+    if (FAST) {
+        T t = fma(-am, qp, ap * qm);
+        fh = fma(apm, hp - hm, t) * ad;
+        T Fm = fma(qm, um, (C.ga * hm) * hm);
+        T Fp = fma(qp, up, (C.ga * hp) * hp);
+        T s = fma(-am, Fp, ap * Fm);
+        fu = fma(apm, qp - qm, s) * ad;
+    } else {
+        fh = (ap * qm - am * qp + apm * (hp - hm)) * ad;
+        fu = (ap * (qm * um + C.ga * (hm * hm)) - am * (qp * up + C.ga * (hp * hp)) + apm * (qp - qm)) * ad;
     }
+    fh = flow ? fh : T(0.0);
+    fu = flow ? fu : T(0.0);
+    if (WANT_DT) {
+        T d = FAST ? C.cfl_delta * A::rcp(a) : C.cfl_delta / a;
+        dt = (flow && wet) ? d : C.inv_epsi;
+    }
+    T fv = (fh > T(0.0) ? vn + (FAST ? sn.d : C.dxh * sn.d) : vi - (FAST ? si.d : C.dxh * si.d)) * fh;
+    T sl, sr;
+    if (FAST) {
+        sl = C.ga * fma(hl + hi, dzi, (hp - hl) * (hp + hl));
+        sr = C.ga * fma(hr + hn, -dzn, (hm - hr) * (hm + hr));
+    } else {
+        sl = C.ga * (hp * hp - hl * hl + (hl + hi) * (zi - zl));
+        sr = C.ga * (hm * hm - hr * hr + (hr + hn) * (zn - zr));
+    }
+    F.a = wet ? fh : T(0.0);
+    F.b = wet ? fu - sl : T(0.0);
+    F.c = wet ? fu - sr : T(0.0);
+    F.d = wet ? fv : T(0.0);
+    return wet;
 }
 
 template <class T> struct Tile {
-    T h[TP * TP], zs[TP * TP], u[TP * TP], v[TP * TP];
-    T gx[4][16 * 18];  // d/dx of h, zs, u, v at x in [-1,16], y in [0,15]
-    T gy[4][18 * 16];  // d/dy at x in [0,15], y in [-1,16]
-    T fx[4][17 * 16];  // Fhu, Fqux, Su, Fqvx
-    T fy[4][16 * 17];  // Fhv, Fqvy, Sv, Fquy
+    Q4<T> q[TP * TP];   // (h, zs, u, v) at x, y in [-2, 17], corners unused
+    Q4<T> gx[16 * 18];  // slopes along x of (h, zs, u, v) at x in [-1,16], y in [0,15]
+    Q4<T> gy[18 * 16];  // slopes along y of (h, zs, v, u) at x in [0,15], y in [-1,16]
+    Q4<T> fx[17 * 16];  // (Fhu, Fqux, Su, Fqvx) at X faces
+    Q4<T> fy[16 * 17];  // (Fhv, Fqvy, Sv, Fquy) at Y faces
     T red[4];
 };
 
-// Gradients of the tile (interior pass by every thread, halo ring by wave 0).
-template <class T, bool FAST> __device__ __forceinline__ void tile_gradients(Tile<T>& s, const Cst<T>& C, int t) {
-    typedef Ar<T, FAST> A;
+// Tile position and source cell of strip slot k in [0, 128): waves 0 and 1
+// fetch the two-deep x strips (left, right) and y strips (bottom, top).  A
+// missing neighbour (the block itself) maps to the block's own edge cell.
+__device__ __forceinline__ void strip_slot(int k, int ib, int nL, int nR, int nB, int nT, int& c, int& src) {
+    const int j = k & 15;
+    const int s = (k >> 4) & 3;           // 0: -1, 1: -2, 2: 16, 3: 17
+    const int o = s < 2 ? -1 - s : 14 + s;
+    const bool lo = s < 2;
+    if (k < 64) {  // x strips
+        const int nb = lo ? nL : nR;
+        const int sx = nb == ib ? (lo ? 0 : BW - 1) : (lo ? BW + o : o - BW);
+        c = tix(o, j);
+        src = memloc(sx, j, nb);
+    } else {       // y strips
+        const int nb = lo ? nB : nT;
+        const int sy = nb == ib ? (lo ? 0 : BW - 1) : (lo ? BW + o : o - BW);
+        c = tix(j, o);
+        src = memloc(j, sy, nb);
+    }
+}
+
+template <class T, bool FAST>
+__device__ __forceinline__ Q4<T> slopes(const Cst<T>& C, const Q4<T>& m, const Q4<T>& c, const Q4<T>& p) {
+    Q4<T> g;
     const T th = C.theta;
-    const T dl = C.delta;
+    if (FAST) {
+        g.a = C.half * minmod2(th, m.a, c.a, p.a);
+        g.b = C.half * minmod2(th, m.b, c.b, p.b);
+        g.c = C.half * minmod2(th, m.c, c.c, p.c);
+        g.d = C.half * minmod2(th, m.d, c.d, p.d);
+    } else {
+        g.a = minmod2(th, m.a, c.a, p.a) / C.delta;
+        g.b = minmod2(th, m.b, c.b, p.b) / C.delta;
+        g.c = minmod2(th, m.c, c.c, p.c) / C.delta;
+        g.d = minmod2(th, m.d, c.d, p.d) / C.delta;
+    }
+    return g;
+}
+
+// Slopes of the tile: every thread its own cell along x and y, wave 0 the
+// halo ring (x slopes at x = -1, 16 and y slopes at y = -1, 16).
+template <class T, bool FAST> __device__ __forceinline__ void tile_gradients(Tile<T>& s, const Cst<T>& C, int t) {
     {
-        int x = t & 15, y = t >> 4;
-        int c = tix(x, y);
-        const T* arr[4] = {s.h, s.zs, s.u, s.v};
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const T* a = arr[k];
-            s.gx[k][gxi(x, y)] = A::div(minmod2(th, a[c - 1], a[c], a[c + 1]), dl);
-            s.gy[k][gyi(x, y)] = A::div(minmod2(th, a[c - TP], a[c], a[c + TP]), dl);
-        }
+        const int x = t & 15, y = t >> 4;
+        const int c = tix(x, y);
+        const Q4<T> q0 = s.q[c];
+        s.gx[gxi(x, y)] = slopes<T, FAST>(C, s.q[c - 1], q0, s.q[c + 1]);
+        Q4<T> m = s.q[c - TP], p = s.q[c + TP];
+        Q4<T> g = slopes<T, FAST>(C, m, q0, p);
+        Q4<T> gs;  // y slopes stored as (h, zs, v, u)
+        gs.a = g.a; gs.b = g.b; gs.c = g.d; gs.d = g.c;
+        s.gy[gyi(x, y)] = gs;
     }
     if (t < 64) {
-        int j = t & 15, side = t >> 4;
-        const T* arr[4] = {s.h, s.zs, s.u, s.v};
+        const int j = t & 15, side = t >> 4;
         if (side < 2) {
-            int x = side == 0 ? -1 : BW;
-            int c = tix(x, j);
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const T* a = arr[k];
-                s.gx[k][gxi(x, j)] = A::div(minmod2(th, a[c - 1], a[c], a[c + 1]), dl);
-            }
+            const int x = side == 0 ? -1 : BW;
+            const int c = tix(x, j);
+            s.gx[gxi(x, j)] = slopes<T, FAST>(C, s.q[c - 1], s.q[c], s.q[c + 1]);
         } else {
-            int y = side == 2 ? -1 : BW;
-            int c = tix(j, y);
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const T* a = arr[k];
-                s.gy[k][gyi(j, y)] = A::div(minmod2(th, a[c - TP], a[c], a[c + TP]), dl);
-            }
+            const int y = side == 2 ? -1 : BW;
+            const int c = tix(j, y);
+            Q4<T> g = slopes<T, FAST>(C, s.q[c - TP], s.q[c], s.q[c + TP]);
+            Q4<T> gs;
+            gs.a = g.a; gs.b = g.b; gs.c = g.d; gs.d = g.c;
+            s.gy[gyi(j, y)] = gs;
         }
     }
 }
 
-template <class T, bool FAST>
+template <class T, bool FAST, bool WANT_DT>
 __device__ __forceinline__ bool xface(Tile<T>& s, const Cst<T>& C, int xf, int y, T& dt) {
-    int ci = tix(xf, y), cn = ci - 1;
-    int gi = gxi(xf, y), gn = gi - 1;
-    T Fh, Fq, S, Fv;
-    bool wet = kurg_face<T, FAST>(C, s.h[ci], s.h[cn], s.zs[ci], s.zs[cn], s.gx[0][gi], s.gx[0][gn], s.gx[1][gi],
-                                  s.gx[1][gn], s.u[ci], s.u[cn], s.gx[2][gi], s.gx[2][gn], s.v[ci], s.v[cn],
-                                  s.gx[3][gi], s.gx[3][gn], Fh, Fq, S, Fv, dt);
-    int f = fxi(xf, y);
-    s.fx[0][f] = Fh;
-    s.fx[1][f] = Fq;
-    s.fx[2][f] = S;
-    s.fx[3][f] = Fv;
+    const int ci = tix(xf, y);
+    const Q4<T> qi = s.q[ci], qn = s.q[ci - 1];
+    const int gi = gxi(xf, y);
+    Q4<T> F;
+    bool wet = kurg_face<T, FAST, WANT_DT>(C, qi.a, qn.a, qi.b, qn.b, qi.c, qn.c, qi.d, qn.d, s.gx[gi], s.gx[gi - 1],
+                                           F, dt);
+    s.fx[fxi(xf, y)] = F;
     return wet;
 }
 
-template <class T, bool FAST>
+template <class T, bool FAST, bool WANT_DT>
 __device__ __forceinline__ bool yface(Tile<T>& s, const Cst<T>& C, int x, int yf, T& dt) {
-    int ci = tix(x, yf), cn = ci - TP;
-    int gi = gyi(x, yf), gn = gi - 16;
-    T Fh, Fq, S, Fv;
-    bool wet = kurg_face<T, FAST>(C, s.h[ci], s.h[cn], s.zs[ci], s.zs[cn], s.gy[0][gi], s.gy[0][gn], s.gy[1][gi],
-                                  s.gy[1][gn], s.v[ci], s.v[cn], s.gy[3][gi], s.gy[3][gn], s.u[ci], s.u[cn],
-                                  s.gy[2][gi], s.gy[2][gn], Fh, Fq, S, Fv, dt);
-    int f = fyi(x, yf);
-    s.fy[0][f] = Fh;
-    s.fy[1][f] = Fq;
-    s.fy[2][f] = S;
-    s.fy[3][f] = Fv;
+    const int ci = tix(x, yf);
+    const Q4<T> qi = s.q[ci], qn = s.q[ci - TP];
+    const int gi = gyi(x, yf);
+    Q4<T> F;
+    bool wet = kurg_face<T, FAST, WANT_DT>(C, qi.a, qn.a, qi.b, qn.b, qi.d, qn.d, qi.c, qn.c, s.gy[gi], s.gy[gi - 16],
+                                           F, dt);
+    s.fy[fyi(x, yf)] = F;
     return wet;
 }
 
-// Fluxes of the tile state, the wall patch and the tendencies of the
-// thread's cell.  Returns the cell's dtmax (src/Kurganov.cu:102-105, 159).
-template <class T, bool FAST>
+// Fluxes of the tile state and the tendencies of the thread's cell, with the
+// wall patch (noslipbndQ, src/Boundary.cu:1887-1891) applied as operand
+// substitutions: at a left (bottom) wall the boundary face has no mass flux
+// and the first cell's right-face source S is replaced by its left-face Fq;
+// at a right (top) wall the boundary face has no mass flux and the last
+// cell's left-face Fq is replaced by its right-face S.  Returns the cell's
+// dtmax: its X face then its Y face, a dry face storing 1/epsi outright
+// (src/Kurganov.cu:102-105, 159, 1488-1491, 1537).
+template <class T, bool FAST, bool WANT_DT>
 __device__ __forceinline__ T tile_fluxes(Tile<T>& s, const Cst<T>& C, int t, bool wl, bool wr, bool wb, bool wt,
                                          T& dh, T& dhu, T& dhv) {
     tile_gradients<T, FAST>(s, C, t);
     __syncthreads();
     const int x = t & 15, y = t >> 4;
-    T dtx, dty, dtd;
-    bool wx = xface<T, FAST>(s, C, x, y, dtx);
-    bool wy = yface<T, FAST>(s, C, x, y, dty);
-    if (t < 16) xface<T, FAST>(s, C, BW, t, dtd);
-    else if (t < 32) yface<T, FAST>(s, C, t - 16, BW, dtd);
-    // X face then Y face write dtmax[i]; a dry face stores 1/epsi outright.
-    T dtc = wx ? dtx : C.inv_epsi;
-    if (wy) dtc = dty < dtc ? dty : dtc;
-    else dtc = C.inv_epsi;
+    T dtx = T(0.0), dty = T(0.0), dtd;
+    xface<T, FAST, WANT_DT>(s, C, x, y, dtx);
+    bool wy = yface<T, FAST, WANT_DT>(s, C, x, y, dty);
+    if (t < 16) xface<T, FAST, false>(s, C, BW, t, dtd);
+    else if (t < 32) yface<T, FAST, false>(s, C, t - 16, BW, dtd);
+    T dtc = T(0.0);
+    if (WANT_DT) dtc = wy ? (dty < dtx ? dty : dtx) : C.inv_epsi;
     __syncthreads();
-    // Wall boundaries (noslipbndQ): left/bottom F=0 and S(inside)=G(face);
-    // right/top F=0 and Fq(inside)=Ss(face).
-    if (t < 16) {
-        if (wl) { s.fx[0][fxi(0, t)] = T(0.0); s.fx[2][fxi(1, t)] = s.fx[1][fxi(0, t)]; }
-        if (wr) { s.fx[0][fxi(BW, t)] = T(0.0); s.fx[1][fxi(BW - 1, t)] = s.fx[2][fxi(BW, t)]; }
-    } else if (t < 32) {
-        int j = t - 16;
-        if (wb) { s.fy[0][fyi(j, 0)] = T(0.0); s.fy[2][fyi(j, 1)] = s.fy[1][fyi(j, 0)]; }
-        if (wt) { s.fy[0][fyi(j, BW)] = T(0.0); s.fy[1][fyi(j, BW - 1)] = s.fy[2][fyi(j, BW)]; }
-    }
-    __syncthreads();
+    const Q4<T> Fl = s.fx[fxi(x, y)], Fr = s.fx[fxi(x + 1, y)];
+    const Q4<T> Fb = s.fy[fyi(x, y)], Ft = s.fy[fyi(x, y + 1)];
+    const Q4<T> q = s.q[tix(x, y)];
+    T Fhl = Fl.a, Fhr = Fr.a, Fql = Fl.b, Sr = Fr.c;
+    T Fhb = Fb.a, Fht = Ft.a, Fqb = Fb.b, St = Ft.c;
+    if (wl && x == 0) { Fhl = T(0.0); Sr = Fql; }
+    if (wr && x == BW - 1) { Fhr = T(0.0); Fql = Sr; }
+    if (wb && y == 0) { Fhb = T(0.0); St = Fqb; }
+    if (wt && y == BW - 1) { Fht = T(0.0); Fqb = St; }
     // src/Advection.cu:236-250 (Cartesian: the metric terms vanish)
-    const int f = fxi(x, y), g = fyi(x, y);
-    const T hi = s.h[tix(x, y)];
-    dh = T(-1.0) * (s.fx[0][f + 1] - s.fx[0][f] + s.fy[0][g + 16] - s.fy[0][g]) * C.cmdinv;
-    dhu = (s.fx[1][f] + s.fy[3][g] - s.fx[2][f + 1] - s.fy[3][g + 16]) * C.cmdinv + C.fc * hi * s.v[tix(x, y)];
-    dhv = (s.fy[1][g] + s.fx[3][f] - s.fy[2][g + 16] - s.fx[3][f + 1]) * C.cmdinv - C.fc * hi * s.u[tix(x, y)];
+    dh = T(-1.0) * (Fhr - Fhl + Fht - Fhb) * C.cmdinv;
+    dhu = (Fql + Fb.d - Sr - Ft.d) * C.cmdinv + C.fc * q.a * q.d;
+    dhv = (Fqb + Fl.d - St - Fr.d) * C.cmdinv - C.fc * q.a * q.c;
     return dtc;
 }
 
@@ -322,25 +350,26 @@ __device__ __forceinline__ void advance(const Cst<T>& C, int fmc, T dt, T hold, 
     T edt;
     if (fmc || dhi >= T(0.0)) {
         edt = dt;
-    } else if (FAST && sizeof(T) == 4) {
-        edt = dmin(dt, A::div(dmax(hold, C.eps), fabs(dhi)));
+    } else if (FAST) {
+        edt = dmin(dt, dmax(hold, C.eps) * A::rcp(fabs(dhi)));
     } else {
         double lim = (hold > C.eps_d ? (double)hold : C.eps_d) / (double)fabs(dhi);
         edt = T((double)dt < lim ? (double)dt : lim);
     }
-    ho = hold + edt * dhi;
-    if (ho > C.eps) {
+    if (FAST) {
+        ho = fma(edt, dhi, hold);
         T r = A::rcp(ho);
-        if (FAST) {
-            uo = (hold * uold + edt * dhui) * r;
-            vo = (hold * vold + edt * dhvi) * r;
-        } else {
+        uo = ho > C.eps ? fma(hold, uold, edt * dhui) * r : T(0.0);
+        vo = ho > C.eps ? fma(hold, vold, edt * dhvi) * r : T(0.0);
+    } else {
+        ho = hold + edt * dhi;
+        if (ho > C.eps) {
             uo = (hold * uold + edt * dhui) / ho;
             vo = (hold * vold + edt * dhvi) / ho;
+        } else {
+            uo = T(0.0);
+            vo = T(0.0);
         }
-    } else {
-        uo = T(0.0);
-        vo = T(0.0);
     }
 }
 
@@ -357,56 +386,57 @@ template <class T> __device__ __forceinline__ T block_min(T v, T* red, int t) {
     return m;
 }
 
-// Blocks beyond the step's end time skip all work: the device loop state
-// carries a done flag.
-__device__ __forceinline__ bool step_done(const DevLoop* L) { return L->done != 0; }
+// Positive floating-point values order like their bit patterns.
+__device__ __forceinline__ void slot_min(DevLoop* L, int k, float v) {
+    atomicMin(&L->slot32[k & (NSLOT - 1)], __float_as_uint(v));
+}
+__device__ __forceinline__ void slot_min(DevLoop* L, int k, double v) {
+    atomicMin(&L->slot64[k & (NSLOT - 1)], (unsigned long long)__double_as_longlong(v));
+}
 
 template <class T, bool FAST>
-__global__ __launch_bounds__(256) void k_predictor(KParams P, StepBuffers<T> B, const DevLoop* L) {
-    if (step_done(L)) return;
+__global__ __launch_bounds__(256) void k_predictor(KParams P, StepBuffers<T> B, DevLoop* L) {
+    if (L->done) return;
     __shared__ Tile<T> s;
     const int t = threadIdx.x;
     const int ibl = blockIdx.x;
     const int ib = P.active[ibl];
     const int nL = P.LeftBot[ib], nR = P.RightBot[ib], nB = P.BotLeft[ib], nT = P.TopLeft[ib];
     const Cst<T> C = make_cst<T, FAST>(P);
+    const int i = memloc(t & 15, t >> 4, ib);
     {
-        int x = t & 15, y = t >> 4;
-        int src = memloc(x, y, ib);
-        int c = tix(x, y);
-        T hv = B.h[src];
-        s.h[c] = hv;
-        s.zs[c] = B.zb[src] + hv;
-        s.u[c] = B.u[src];
-        s.v[c] = B.v[src];
+        Q4<T> q;
+        q.a = B.h[i];
+        q.b = B.zb[i] + q.a;
+        q.c = B.u[i];
+        q.d = B.v[i];
+        s.q[tix(t & 15, t >> 4)] = q;
     }
     if (t < 128) {
-        int x, y;
-        strip_pos(t, x, y);
-        int src = src_cell(x, y, ib, nL, nR, nB, nT);
-        int c = tix(x, y);
-        T hv = B.h[src];
-        s.h[c] = hv;
-        s.zs[c] = B.zb[src] + hv;
-        s.u[c] = B.u[src];
-        s.v[c] = B.v[src];
+        int c, src;
+        strip_slot(t, ib, nL, nR, nB, nT, c, src);
+        Q4<T> q;
+        q.a = B.h[src];
+        q.b = B.zb[src] + q.a;
+        q.c = B.u[src];
+        q.d = B.v[src];
+        s.q[c] = q;
     }
     __syncthreads();
     const bool wall = P.aoibnd == 0;
     T dh, dhu, dhv;
-    T dtc = tile_fluxes<T, FAST>(s, C, t, wall && nL == ib, wall && nR == ib, wall && nB == ib, wall && nT == ib,
-                                 dh, dhu, dhv);
-    const int i = memloc(t & 15, t >> 4, ib);
+    T dtc = tile_fluxes<T, FAST, true>(s, C, t, wall && nL == ib, wall && nR == ib, wall && nB == ib,
+                                       wall && nT == ib, dh, dhu, dhv);
     B.dh[i] = dh;
     B.dhu[i] = dhu;
     B.dhv[i] = dhv;
     T m = block_min(dtc, s.red, t);
-    if (t == 0) B.blockmin[ibl] = m;
+    if (t == 0) slot_min(L, ibl, m);
 }
 
 template <class T, bool FAST>
 __global__ __launch_bounds__(256) void k_corrector(KParams P, StepBuffers<T> B, DevLoop* L) {
-    if (step_done(L)) return;
+    if (L->done) return;
     __shared__ Tile<T> s;
     const int t = threadIdx.x;
     const int ibl = blockIdx.x;
@@ -419,33 +449,26 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, StepBuffers<T> B, 
     // own cell: keep U^n for the full-step advance
     const int x0 = t & 15, y0 = t >> 4;
     const int i = memloc(x0, y0, ib);
-    T h0 = B.h[i], u0 = B.u[i], v0 = B.v[i], zb0 = B.zb[i];
+    const T h0 = B.h[i], u0 = B.u[i], v0 = B.v[i], zb0 = B.zb[i];
     {
-        T ho, uo, vo;
-        advance<T, FAST>(C, fmc, hdt, h0, u0, v0, B.dh[i], B.dhu[i], B.dhv[i], ho, uo, vo);
-        int c = tix(x0, y0);
-        s.h[c] = ho;
-        s.zs[c] = zb0 + ho;
-        s.u[c] = uo;
-        s.v[c] = vo;
+        Q4<T> q;
+        advance<T, FAST>(C, fmc, hdt, h0, u0, v0, B.dh[i], B.dhu[i], B.dhv[i], q.a, q.c, q.d);
+        q.b = zb0 + q.a;
+        s.q[tix(x0, y0)] = q;
     }
     if (t < 128) {
-        int x, y;
-        strip_pos(t, x, y);
-        int src = src_cell(x, y, ib, nL, nR, nB, nT);
-        T ho, uo, vo;
-        advance<T, FAST>(C, fmc, hdt, B.h[src], B.u[src], B.v[src], B.dh[src], B.dhu[src], B.dhv[src], ho, uo, vo);
-        int c = tix(x, y);
-        s.h[c] = ho;
-        s.zs[c] = B.zb[src] + ho;
-        s.u[c] = uo;
-        s.v[c] = vo;
+        int c, src;
+        strip_slot(t, ib, nL, nR, nB, nT, c, src);
+        Q4<T> q;
+        advance<T, FAST>(C, fmc, hdt, B.h[src], B.u[src], B.v[src], B.dh[src], B.dhu[src], B.dhv[src], q.a, q.c, q.d);
+        q.b = B.zb[src] + q.a;
+        s.q[c] = q;
     }
     __syncthreads();
     const bool wall = P.aoibnd == 0;
     T dh, dhu, dhv;
-    (void)tile_fluxes<T, FAST>(s, C, t, wall && nL == ib, wall && nR == ib, wall && nB == ib, wall && nT == ib,
-                               dh, dhu, dhv);
+    (void)tile_fluxes<T, FAST, false>(s, C, t, wall && nL == ib, wall && nR == ib, wall && nB == ib,
+                                      wall && nT == ib, dh, dhu, dhv);
     T h1, u1, v1;
     advance<T, FAST>(C, fmc, dt, h0, u0, v0, dh, dhu, dhv, h1, u1, v1);
     // bottom friction, src/Friction.cu:89-148
@@ -453,7 +476,7 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, StepBuffers<T> B, 
         typedef Ar<T, FAST> A;
         T normu = A::sqrt_(u1 * u1 + v1 * v1);
         T cfi;
-        T cfv = B.cf[i];
+        const T cfv = B.cf[i];
         if (P.frictionmodel == 0) {
             cfi = cfv;
         } else if (P.frictionmodel == 1) {
@@ -462,17 +485,24 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, StepBuffers<T> B, 
             else cfi = T(1.0) / (T(2.5) * (log(Hbar) - T(1.0) + T(1.359) / Hbar));
             cfi = cfi * cfi;
         } else {
-            cfi = C.g * cfv * cfv / cbrt(h1);
+            cfi = FAST ? C.g * cfv * cfv * (T)exp2f(-0.333333343f * log2f((float)h1)) : C.g * cfv * cfv / cbrt(h1);
         }
-        T tb = cfi * normu / h1 * dt;
-        u1 = u1 / (T(1.0) + tb);
-        v1 = v1 / (T(1.0) + tb);
+        if (FAST) {
+            T tb = cfi * normu * A::rcp(h1) * dt;
+            T r = A::rcp(T(1.0) + tb);
+            u1 = u1 * r;
+            v1 = v1 * r;
+        } else {
+            T tb = cfi * normu / h1 * dt;
+            u1 = u1 / (T(1.0) + tb);
+            v1 = v1 / (T(1.0) + tb);
+        }
     }
     // implicit uniform rain after cleanup, src/Updateforcing.cu:530-581
     if (P.rain_on) {
-        T act = zb0 < T(P.mask) ? T(1.0) : T(0.0);
+        T act = (double)zb0 < P.mask ? T(1.0) : T(0.0);
         T rainhh = dmax(C.rainrate / T(1000.0) / T(3600.0) * dt, T(0.0)) * act;
-        T qvol = h1 / (h1 + rainhh);
+        T qvol = FAST ? h1 * Ar<T, FAST>::rcp(h1 + rainhh) : h1 / (h1 + rainhh);
         bool wetold = (double)h1 > C.eps_d;
         h1 = h1 + rainhh;
         if (wetold) {
@@ -485,29 +515,31 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, StepBuffers<T> B, 
     B.vo[i] = v1;
 }
 
-// One workgroup: min of the block minima and 1/epsi, then the clamp and the
-// output-time landing of CalctimestepCPU in FP64.
+// One wave: min of the 64 block-minimum slots and 1/epsi (slots reset for
+// the next step), then the clamp and the output-time landing of
+// CalctimestepCPU in FP64.
 template <class T>
-__global__ __launch_bounds__(1024) void k_reduce_dt(const T* __restrict__ blockmin, int nblk, DevLoop* L,
-                                                    double endtime, int advance_time) {
+__global__ __launch_bounds__(64) void k_reduce_dt(DevLoop* L, double endtime, int advance_time) {
     if (L->done) return;
-    __shared__ T red[16];
     const int t = threadIdx.x;
     const T epsi = sizeof(T) == 8 ? T(2.220446049250313e-16) : T(1.1920928955078125e-07);
-    T m = T(1.0) / epsi;
-    for (int k = t; k < nblk; k += 1024) {
-        T b = blockmin[k];
-        m = b < m ? b : m;
+    T m;
+    if (sizeof(T) == 8) {
+        unsigned long long b = L->slot64[t];
+        m = (T)__longlong_as_double((long long)b);
+        L->slot64[t] = 0x7ff0000000000000ull;
+    } else {
+        unsigned int b = L->slot32[t];
+        m = (T)__uint_as_float(b);
+        L->slot32[t] = 0x7f800000u;
     }
     for (int off = 32; off > 0; off >>= 1) {
         T o = __shfl_xor(m, off, 64);
         m = o < m ? o : m;
     }
-    if ((t & 63) == 0) red[t >> 6] = m;
-    __syncthreads();
     if (t == 0) {
-        T dt = red[0];
-        for (int k = 1; k < 16; k++) dt = red[k] < dt ? red[k] : dt;
+        T dt = T(1.0) / epsi;
+        dt = m < dt ? m : dt;
         if ((double)dt > 1.5 * L->dtmax) dt = T(1.5 * L->dtmax);
         double rem = L->nextoutputtime - L->totaltime;
         if (ceil(rem / (double)dt) > 0.0) dt = T(rem / ceil(rem / (double)dt));
@@ -525,14 +557,14 @@ __global__ __launch_bounds__(1024) void k_reduce_dt(const T* __restrict__ blockm
 // Per-cell initial step delta/sqrt(g*max(h,eps)) and its block minimum,
 // src/Mainloop.cu:569-589.
 template <class T>
-__global__ __launch_bounds__(256) void k_init_dt(KParams P, const T* __restrict__ h, T* __restrict__ blockmin) {
+__global__ __launch_bounds__(256) void k_init_dt(KParams P, const T* __restrict__ h, DevLoop* L) {
     __shared__ T red[4];
     const int t = threadIdx.x;
     const int ib = P.active[blockIdx.x];
     const int i = memloc(t & 15, t >> 4, ib);
     T d = T(P.delta) / sqrt(T(P.g) * (h[i] > T(P.eps) ? h[i] : T(P.eps)));
     T m = block_min(d, red, t);
-    if (t == 0) blockmin[blockIdx.x] = m;
+    if (t == 0) slot_min(L, blockIdx.x, m);
 }
 
 // Block partial sums of h*delta^2 in FP64 (deterministic two-level sum).
@@ -583,7 +615,7 @@ void launch_step_t(const KParams& P, const StepBuffers<T>& B, DevLoop* L, double
     if (ev) hipEventRecord(ev[0], st);
     hipLaunchKernelGGL((k_predictor<T, FAST>), dim3(P.nblk), dim3(256), 0, st, P, B, L);
     if (ev) hipEventRecord(ev[1], st);
-    hipLaunchKernelGGL((k_reduce_dt<T>), dim3(1), dim3(1024), 0, st, B.blockmin, P.nblk, L, endtime, advance_time);
+    hipLaunchKernelGGL((k_reduce_dt<T>), dim3(1), dim3(64), 0, st, L, endtime, advance_time);
     if (ev) hipEventRecord(ev[2], st);
     hipLaunchKernelGGL((k_corrector<T, FAST>), dim3(P.nblk), dim3(256), 0, st, P, B, L);
     if (ev) hipEventRecord(ev[3], st);
@@ -599,13 +631,13 @@ void launch_step(int precision, const KParams& P, const void* buffers, DevLoop* 
         launch_step_t<float, true>(P, *(const StepBuffers<float>*)buffers, L, endtime, advance_time, st, ev);
 }
 
-void launch_init_dt(int precision, const KParams& P, const void* h, void* blockmin, DevLoop* L, hipStream_t st) {
+void launch_init_dt(int precision, const KParams& P, const void* h, DevLoop* L, hipStream_t st) {
     if (precision == 0) {
-        hipLaunchKernelGGL(k_init_dt<double>, dim3(P.nblk), dim3(256), 0, st, P, (const double*)h, (double*)blockmin);
-        hipLaunchKernelGGL(k_reduce_dt<double>, dim3(1), dim3(1024), 0, st, (const double*)blockmin, P.nblk, L, 0.0, 0);
+        hipLaunchKernelGGL(k_init_dt<double>, dim3(P.nblk), dim3(256), 0, st, P, (const double*)h, L);
+        hipLaunchKernelGGL(k_reduce_dt<double>, dim3(1), dim3(64), 0, st, L, 0.0, 0);
     } else {
-        hipLaunchKernelGGL(k_init_dt<float>, dim3(P.nblk), dim3(256), 0, st, P, (const float*)h, (float*)blockmin);
-        hipLaunchKernelGGL(k_reduce_dt<float>, dim3(1), dim3(1024), 0, st, (const float*)blockmin, P.nblk, L, 0.0, 0);
+        hipLaunchKernelGGL(k_init_dt<float>, dim3(P.nblk), dim3(256), 0, st, P, (const float*)h, L);
+        hipLaunchKernelGGL(k_reduce_dt<float>, dim3(1), dim3(64), 0, st, L, 0.0, 0);
     }
 }
 
