@@ -7,8 +7,10 @@
 // (evolv).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -46,7 +48,7 @@ struct bgf_handle_s {
     int nblk, nblkmem;
     size_t S;   // elements per array
     std::vector<int> active;
-    int *d_active = nullptr, *d_level = nullptr;
+    int *d_active = nullptr, *d_order = nullptr, *d_level = nullptr;
     int *d_nb[8] = {nullptr};  // LeftBot LeftTop RightBot RightTop TopLeft TopRight BotLeft BotRight
     void *d_h[2] = {nullptr, nullptr}, *d_u[2] = {nullptr, nullptr}, *d_v[2] = {nullptr, nullptr};
     void *d_zb = nullptr, *d_cf = nullptr, *d_dh = nullptr, *d_dhu = nullptr, *d_dhv = nullptr;
@@ -86,7 +88,7 @@ template <class T> double coriolis(double lat) {
 
 void free_handle(bgf_handle h) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
-    F(h->d_active); F(h->d_level);
+    F(h->d_active); F(h->d_order); F(h->d_level);
     for (int k = 0; k < 8; k++) F(h->d_nb[k]);
     for (int k = 0; k < 2; k++) { F(h->d_h[k]); F(h->d_u[k]); F(h->d_v[k]); }
     F(h->d_zb); F(h->d_cf); F(h->d_dh); F(h->d_dhu); F(h->d_dhv);
@@ -167,6 +169,37 @@ bgf_status settle(bgf_handle h, int steps_done) {
     return BGF_OK;
 }
 
+// Processing order of the persistent kernels: the active blocks grouped in
+// tiles of TILE_X x TILE_Y blocks (row-major tiles, row-major blocks inside a
+// tile), by their integer block coordinates.  Each XCD processes consecutive
+// positions of this order, so the blocks it has in flight at once form a
+// compact patch whose neighbours are mostly in its own L2.
+constexpr int TILE_X = 16, TILE_Y = 12;
+std::vector<int> processing_order(const bgf_blocks* b, int nblk, double dx) {
+    std::vector<int> order(b->active, b->active + nblk);
+    std::vector<long long> key(nblk);
+    long long maxbx = 1;
+    std::vector<long long> bx(nblk), by(nblk);
+    for (int k = 0; k < nblk; k++) {
+        const int ib = b->active[k];
+        const int lev = b->level[ib];
+        const double levdx = lev < 0 ? dx * (1 << -lev) : dx / (1 << lev);
+        bx[k] = (long long)std::floor(b->xo[ib] / (BW * levdx));
+        by[k] = (long long)std::floor(b->yo[ib] / (BW * levdx));
+        if (bx[k] + 1 > maxbx) maxbx = bx[k] + 1;
+    }
+    const long long ntx = (maxbx + TILE_X - 1) / TILE_X;
+    for (int k = 0; k < nblk; k++) {
+        const long long tx = bx[k] / TILE_X, ty = by[k] / TILE_Y;
+        key[k] = ((ty * ntx + tx) * TILE_Y + by[k] % TILE_Y) * TILE_X + bx[k] % TILE_X;
+    }
+    std::vector<int> idx(nblk);
+    for (int k = 0; k < nblk; k++) idx[k] = k;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int c) { return key[a] < key[c]; });
+    for (int k = 0; k < nblk; k++) order[k] = b->active[idx[k]];
+    return order;
+}
+
 }  // namespace
 
 extern "C" {
@@ -217,6 +250,7 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
     };
     bool ok = true;
     ok = ok && A((void**)&h->d_active, sizeof(int) * nblk);
+    ok = ok && A((void**)&h->d_order, sizeof(int) * nblk);
     ok = ok && A((void**)&h->d_level, sizeof(int) * nbm);
     for (int k = 0; k < 8; k++) ok = ok && A((void**)&h->d_nb[k], sizeof(int) * nbm);
     for (int k = 0; k < 2; k++) {
@@ -234,7 +268,9 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
         return fail(BGF_ERR_OOM, "device allocation failed");
     }
     const int* nbs[8] = {b->LeftBot, b->LeftTop, b->RightBot, b->RightTop, b->TopLeft, b->TopRight, b->BotLeft, b->BotRight};
+    std::vector<int> order = processing_order(b, nblk, p->dx);
     bool cp = hipMemcpy(h->d_active, b->active, sizeof(int) * nblk, hipMemcpyHostToDevice) == hipSuccess;
+    cp = cp && hipMemcpy(h->d_order, order.data(), sizeof(int) * nblk, hipMemcpyHostToDevice) == hipSuccess;
     cp = cp && hipMemcpy(h->d_level, b->level, sizeof(int) * nbm, hipMemcpyHostToDevice) == hipSuccess;
     for (int k = 0; k < 8; k++) cp = cp && hipMemcpy(h->d_nb[k], nbs[k], sizeof(int) * nbm, hipMemcpyHostToDevice) == hipSuccess;
     cp = cp && hipStreamCreate(&h->stream) == hipSuccess;
@@ -253,7 +289,18 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
     kp.ForceMassConserve = p->ForceMassConserve;
     kp.rain_on = p->rain_on;
     kp.nblk = nblk;
+    // BGF_WG_PER_CU="p,c" caps the persistent workgroups per CU of the
+    // predictor and the corrector (a tuning override; default: occupancy).
+    int wp = 0, wc = 0;
+    if (const char* e = std::getenv("BGF_WG_PER_CU")) std::sscanf(e, "%d,%d", &wp, &wc);
+    kp.grid_pred = bgf::persistent_grid(h->prec, 0, nblk, wp);
+    kp.grid_corr = bgf::persistent_grid(h->prec, 1, nblk, wc);
+    if (kp.grid_pred <= 0 || kp.grid_corr <= 0) {
+        free_handle(h);
+        return fail(BGF_ERR_HIP, "occupancy query failed");
+    }
     kp.active = h->d_active;
+    kp.order = h->d_order;
     kp.LeftBot = h->d_nb[0];
     kp.RightBot = h->d_nb[2];
     kp.TopLeft = h->d_nb[4];

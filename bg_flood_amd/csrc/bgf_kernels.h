@@ -25,7 +25,9 @@ struct DevLoop {
 struct KParams {
     double g, eps, CFL, theta, delta, fc, mask, rain_mmh;
     int frictionmodel, aoibnd, ForceMassConserve, rain_on, nblk;
-    const int* active;
+    int grid_pred, grid_corr;  // persistent workgroup counts (multiples of 8)
+    const int* active;  // active blocks in the reference order
+    const int* order;   // the same blocks in the persistent kernels' processing order
     const int* LeftBot;
     const int* RightBot;
     const int* BotLeft;
@@ -50,6 +52,8 @@ template <class T> struct StepBuffers {
 // the reduction and after the corrector.
 void launch_step(int precision, const KParams& P, const void* buffers, DevLoop* L, double endtime, int advance_time,
                  hipStream_t st, hipEvent_t* ev);
+// Workgroups of a persistent launch; wg_per_cu > 0 caps the resident workgroups per CU.
+int persistent_grid(int precision, int corrector, int nblk, int wg_per_cu);
 void launch_init_dt(int precision, const KParams& P, const void* h, DevLoop* L, hipStream_t st);
 void launch_volume(int precision, const KParams& P, const void* h, double* part, double* out, hipStream_t st);
 void launch_copy_state(int precision, const KParams& P, const void* const* src, void* const* dst, hipStream_t st);

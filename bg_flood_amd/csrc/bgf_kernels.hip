@@ -52,17 +52,27 @@ namespace bgf {
 constexpr int BW = 16;
 constexpr int MW = 18;
 constexpr int BS = MW * MW;
-constexpr int TP = 20;  // LDS tile pitch: x, y in [-2, 17]
+// LDS pitches are odd so that column accesses (halo strips, the 17th face)
+// fall into distinct banks.
+constexpr int TP = 21;  // tile: x, y in [-2, 17]
+constexpr int GXP = 19; // x slopes: x in [-1, 16], y in [0, 15]
+constexpr int GYP = 17; // y slopes: x in [0, 15], y in [-1, 16]
+constexpr int FP = 17;  // X faces xf in [0, 16]; Y faces x in [0, 15]
 
 __device__ __forceinline__ int memloc(int i, int j, int ib) { return (i + 1) + (j + 1) * MW + ib * BS; }
 __device__ __forceinline__ int tix(int x, int y) { return (y + 2) * TP + (x + 2); }
-__device__ __forceinline__ int gxi(int x, int y) { return y * 18 + (x + 1); }   // x in [-1,16], y in [0,15]
-__device__ __forceinline__ int gyi(int x, int y) { return (y + 1) * 16 + x; }   // x in [0,15], y in [-1,16]
-__device__ __forceinline__ int fxi(int xf, int y) { return y * 17 + xf; }       // xf in [0,16], y in [0,15]
-__device__ __forceinline__ int fyi(int x, int yf) { return yf * 16 + x; }       // x in [0,15], yf in [0,16]
+__device__ __forceinline__ int gxi(int x, int y) { return y * GXP + (x + 1); }
+__device__ __forceinline__ int gyi(int x, int y) { return (y + 1) * GYP + x; }
+__device__ __forceinline__ int fxi(int xf, int y) { return y * FP + xf; }
+__device__ __forceinline__ int fyi(int x, int yf) { return yf * FP + x; }
 
-template <class T> __device__ __forceinline__ T dmax(T a, T b) { return a > b ? a : b; }
-template <class T> __device__ __forceinline__ T dmin(T a, T b) { return a < b ? a : b; }
+// fmin/fmax (v_min/v_max, fused into v_min3/v_max3).  They differ from the
+// reference's std::min/std::max only for NaN operands and in the sign of a
+// zero result, neither of which changes any value of the step.
+__device__ __forceinline__ float dmax(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ float dmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ double dmax(double a, double b) { return fmax(a, b); }
+__device__ __forceinline__ double dmin(double a, double b) { return fmin(a, b); }
 
 // A cell's (h, zs, u, v), a cell's four gradients, or a face's four fluxes.
 template <class T> struct alignas(4 * sizeof(T)) Q4 {
@@ -79,26 +89,32 @@ template <> struct Ar<float, true> {
     __device__ static __forceinline__ float sqrt_(float a) { return __builtin_amdgcn_sqrtf(a); }
 };
 
-// src/Util_CPU.cu:243-263 without branches.  s0 < s1 < s2 holds exactly when
-// theta*(s1-s0) > 0 and theta*(s2-s1) > 0 (theta >= 1, IEEE subtraction is
-// zero only for equal operands), and the reference's two-step min (max) of
-// d1, d2, d3 is their three-way min (max).
-template <class T> __device__ __forceinline__ T minmod2(T theta, T s0, T s1, T s2) {
-    T d1 = theta * (s1 - s0);
-    T d2 = (s2 - s0) / T(2.0);
-    T d3 = theta * (s2 - s1);
+// src/Util_CPU.cu:243-263 without branches.  With d1 = theta*(s1-s0),
+// d2 = (s2-s0)/2, d3 = theta*(s2-s1): s0 < s1 < s2 holds exactly when
+// min(d1,d2,d3) > 0 (theta >= 1 and IEEE subtraction is zero only for equal
+// operands), and then the reference returns that minimum; s0 > s1 > s2 holds
+// exactly when max(d1,d2,d3) < 0 and then it returns that maximum; otherwise
+// 0.  At most one of max(mn,0) and min(mx,0) is non-zero, so their sum is
+// exact.  FAST kernels pass half the coefficients to get half-cell slopes.
+template <class T> __device__ __forceinline__ T minmod2c(T th, T half, T s0, T s1, T s2) {
+    T d1 = th * (s1 - s0);
+    T d2 = (s2 - s0) * half;
+    T d3 = th * (s2 - s1);
     T mn = dmin(dmin(d1, d2), d3);
     T mx = dmax(dmax(d1, d2), d3);
-    T r = (d1 < T(0.0) && d3 < T(0.0)) ? mx : T(0.0);
-    return (d1 > T(0.0) && d3 > T(0.0)) ? mn : r;
+    return dmax(mn, T(0.0)) + dmin(mx, T(0.0));
 }
 
+// Constants of the step in T, computed once on the host (make_cst) and
+// passed by value, so kernels read them from scalar registers.
 template <class T> struct Cst {
     T g, ga, CFL, theta, epsi, inv_epsi, eps_w, eps, delta, dxh, cmdinv, fc, rainrate, cfl_delta, half;
-    double eps_d;
+    T th_s, half_s;  // theta/2 and 1/4: FAST half-cell slope coefficients
+    T mask, rain_step_scale;
+    double eps_d, mask_d;
 };
 
-template <class T, bool FAST> __device__ __forceinline__ Cst<T> make_cst(const KParams& P) {
+template <class T, bool FAST> Cst<T> make_cst(const KParams& P) {
     Cst<T> c;
     c.g = T(P.g);
     c.ga = c.g * T(0.5);
@@ -116,6 +132,11 @@ template <class T, bool FAST> __device__ __forceinline__ Cst<T> make_cst(const K
     c.rainrate = T(P.rain_mmh);
     c.cfl_delta = c.CFL * c.delta;
     c.half = T(0.5);
+    c.th_s = c.theta * T(0.5);
+    c.half_s = T(0.25);
+    c.mask = T(P.mask);
+    c.mask_d = P.mask;
+    c.rain_step_scale = T(0.0);
     return c;
 }
 
@@ -201,49 +222,53 @@ __device__ __forceinline__ bool kurg_face(const Cst<T>& C, T hi, T hn, T zsi, T 
 }
 
 template <class T> struct Tile {
-    Q4<T> q[TP * TP];   // (h, zs, u, v) at x, y in [-2, 17], corners unused
-    Q4<T> gx[16 * 18];  // slopes along x of (h, zs, u, v) at x in [-1,16], y in [0,15]
-    Q4<T> gy[18 * 16];  // slopes along y of (h, zs, v, u) at x in [0,15], y in [-1,16]
-    Q4<T> fx[17 * 16];  // (Fhu, Fqux, Su, Fqvx) at X faces
-    Q4<T> fy[16 * 17];  // (Fhv, Fqvy, Sv, Fquy) at Y faces
+    Q4<T> q[20 * TP];   // (h, zs, u, v) at x, y in [-2, 17], corners unused
+    Q4<T> gx[16 * GXP]; // slopes along x of (h, zs, u, v) at x in [-1,16], y in [0,15]
+    Q4<T> gy[18 * GYP]; // slopes along y of (h, zs, v, u) at x in [0,15], y in [-1,16]
+    Q4<T> fx[16 * FP];  // (Fhu, Fqux, Su, Fqvx) at X faces
+    Q4<T> fy[17 * FP];  // (Fhv, Fqvy, Sv, Fquy) at Y faces
     T red[4];
 };
 
-// Tile position and source cell of strip slot k in [0, 128): waves 0 and 1
-// fetch the two-deep x strips (left, right) and y strips (bottom, top).  A
-// missing neighbour (the block itself) maps to the block's own edge cell.
-__device__ __forceinline__ void strip_slot(int k, int ib, int nL, int nR, int nB, int nT, int& c, int& src) {
-    const int j = k & 15;
-    const int s = (k >> 4) & 3;           // 0: -1, 1: -2, 2: 16, 3: 17
-    const int o = s < 2 ? -1 - s : 14 + s;
-    const bool lo = s < 2;
-    if (k < 64) {  // x strips
+// Strip slot k in [0, 128): waves 0 and 1 fetch the two-deep x strips (left,
+// right) and y strips (bottom, top) of the 20x20 tile.  strip_cell gives the
+// tile position, strip_src the source cell; a missing neighbour (the block
+// itself) maps to the block's own edge cell.
+__device__ __forceinline__ int strip_off(int k) {
+    const int s = (k >> 4) & 3;  // 0: -1, 1: -2, 2: 16, 3: 17
+    return s < 2 ? -1 - s : 14 + s;
+}
+__device__ __forceinline__ int strip_cell(int k) {
+    const int j = k & 15, o = strip_off(k);
+    return k < 64 ? tix(o, j) : tix(j, o);
+}
+__device__ __forceinline__ int strip_src(int k, int ib, int nL, int nR, int nB, int nT) {
+    const int j = k & 15, o = strip_off(k);
+    const bool lo = o < 0;
+    if (k < 64) {
         const int nb = lo ? nL : nR;
         const int sx = nb == ib ? (lo ? 0 : BW - 1) : (lo ? BW + o : o - BW);
-        c = tix(o, j);
-        src = memloc(sx, j, nb);
-    } else {       // y strips
-        const int nb = lo ? nB : nT;
-        const int sy = nb == ib ? (lo ? 0 : BW - 1) : (lo ? BW + o : o - BW);
-        c = tix(j, o);
-        src = memloc(j, sy, nb);
+        return memloc(sx, j, nb);
     }
+    const int nb = lo ? nB : nT;
+    const int sy = nb == ib ? (lo ? 0 : BW - 1) : (lo ? BW + o : o - BW);
+    return memloc(j, sy, nb);
 }
 
 template <class T, bool FAST>
 __device__ __forceinline__ Q4<T> slopes(const Cst<T>& C, const Q4<T>& m, const Q4<T>& c, const Q4<T>& p) {
     Q4<T> g;
-    const T th = C.theta;
     if (FAST) {
-        g.a = C.half * minmod2(th, m.a, c.a, p.a);
-        g.b = C.half * minmod2(th, m.b, c.b, p.b);
-        g.c = C.half * minmod2(th, m.c, c.c, p.c);
-        g.d = C.half * minmod2(th, m.d, c.d, p.d);
+        // half-cell increments 0.5*minmod2: coefficients theta/2 and 1/4
+        g.a = minmod2c(C.th_s, C.half_s, m.a, c.a, p.a);
+        g.b = minmod2c(C.th_s, C.half_s, m.b, c.b, p.b);
+        g.c = minmod2c(C.th_s, C.half_s, m.c, c.c, p.c);
+        g.d = minmod2c(C.th_s, C.half_s, m.d, c.d, p.d);
     } else {
-        g.a = minmod2(th, m.a, c.a, p.a) / C.delta;
-        g.b = minmod2(th, m.b, c.b, p.b) / C.delta;
-        g.c = minmod2(th, m.c, c.c, p.c) / C.delta;
-        g.d = minmod2(th, m.d, c.d, p.d) / C.delta;
+        g.a = minmod2c(C.theta, C.half, m.a, c.a, p.a) / C.delta;
+        g.b = minmod2c(C.theta, C.half, m.b, c.b, p.b) / C.delta;
+        g.c = minmod2c(C.theta, C.half, m.c, c.c, p.c) / C.delta;
+        g.d = minmod2c(C.theta, C.half, m.d, c.d, p.d) / C.delta;
     }
     return g;
 }
@@ -297,7 +322,7 @@ __device__ __forceinline__ bool yface(Tile<T>& s, const Cst<T>& C, int x, int yf
     const Q4<T> qi = s.q[ci], qn = s.q[ci - TP];
     const int gi = gyi(x, yf);
     Q4<T> F;
-    bool wet = kurg_face<T, FAST, WANT_DT>(C, qi.a, qn.a, qi.b, qn.b, qi.d, qn.d, qi.c, qn.c, s.gy[gi], s.gy[gi - 16],
+    bool wet = kurg_face<T, FAST, WANT_DT>(C, qi.a, qn.a, qi.b, qn.b, qi.d, qn.d, qi.c, qn.c, s.gy[gi], s.gy[gi - GYP],
                                            F, dt);
     s.fy[fyi(x, yf)] = F;
     return wet;
@@ -386,6 +411,11 @@ template <class T> __device__ __forceinline__ T block_min(T v, T* red, int t) {
     return m;
 }
 
+// Workgroups are dispatched round-robin over the eight XCDs, each with its
+// own L2: the persistent kernels give each XCD a contiguous range of blocks so
+// a block's neighbours are mostly fetched into the same L2.
+constexpr int NXCD = 8;
+
 // Positive floating-point values order like their bit patterns.
 __device__ __forceinline__ void slot_min(DevLoop* L, int k, float v) {
     atomicMin(&L->slot32[k & (NSLOT - 1)], __float_as_uint(v));
@@ -394,125 +424,204 @@ __device__ __forceinline__ void slot_min(DevLoop* L, int k, double v) {
     atomicMin(&L->slot64[k & (NSLOT - 1)], (unsigned long long)__double_as_longlong(v));
 }
 
-template <class T, bool FAST>
-__global__ __launch_bounds__(256) void k_predictor(KParams P, StepBuffers<T> B, DevLoop* L) {
-    if (L->done) return;
-    __shared__ Tile<T> s;
-    const int t = threadIdx.x;
-    const int ibl = blockIdx.x;
-    const int ib = P.active[ibl];
-    const int nL = P.LeftBot[ib], nR = P.RightBot[ib], nB = P.BotLeft[ib], nT = P.TopLeft[ib];
-    const Cst<T> C = make_cst<T, FAST>(P);
-    const int i = memloc(t & 15, t >> 4, ib);
-    {
-        Q4<T> q;
-        q.a = B.h[i];
-        q.b = B.zb[i] + q.a;
-        q.c = B.u[i];
-        q.d = B.v[i];
-        s.q[tix(t & 15, t >> 4)] = q;
-    }
-    if (t < 128) {
-        int c, src;
-        strip_slot(t, ib, nL, nR, nB, nT, c, src);
-        Q4<T> q;
-        q.a = B.h[src];
-        q.b = B.zb[src] + q.a;
-        q.c = B.u[src];
-        q.d = B.v[src];
-        s.q[c] = q;
-    }
-    __syncthreads();
-    const bool wall = P.aoibnd == 0;
-    T dh, dhu, dhv;
-    T dtc = tile_fluxes<T, FAST, true>(s, C, t, wall && nL == ib, wall && nR == ib, wall && nB == ib,
-                                       wall && nT == ib, dh, dhu, dhv);
-    B.dh[i] = dh;
-    B.dhu[i] = dhu;
-    B.dhv[i] = dhv;
-    T m = block_min(dtc, s.red, t);
-    if (t == 0) slot_min(L, ibl, m);
+// Block indices are workgroup-uniform: read them through the constant address
+// space so they become scalar loads, which do not wait behind the vector
+// loads of the prefetch.
+__device__ __forceinline__ int ldu(const int* p, int i) {
+    return ((const __attribute__((address_space(4))) int*)p)[i];
+}
+struct Nb {
+    int ib, nL, nR, nB, nT;
+};
+__device__ __forceinline__ Nb block_nb(const KParams& P, int bl) {
+    Nb n;
+    n.ib = ldu(P.order, bl);
+    n.nL = ldu(P.LeftBot, n.ib);
+    n.nR = ldu(P.RightBot, n.ib);
+    n.nB = ldu(P.BotLeft, n.ib);
+    n.nT = ldu(P.TopLeft, n.ib);
+    return n;
 }
 
+// Block sequence of persistent workgroup w: the contiguous active-list range
+// of its XCD, strided by the number of workgroups on that XCD (nwg is a
+// multiple of NXCD, so every XCD has nwg / NXCD workgroups).
+struct Sched {
+    int first, stride, end;
+};
+__device__ __forceinline__ Sched schedule(int w, int nwg, int n) {
+    const int xcd = w % NXCD, k = w / NXCD;
+    const int per = n / NXCD, rem = n % NXCD;
+    const int start = xcd * per + (xcd < rem ? xcd : rem);
+    Sched S;
+    S.first = start + k;
+    S.stride = nwg / NXCD;
+    S.end = start + per + (xcd < rem ? 1 : 0);
+    return S;
+}
+
+// Persistent predictor: each workgroup walks its block sequence, prefetching
+// the next block's cells into registers while the current block is computed
+// from the LDS tile, so the gathers of one block overlap the arithmetic of the
+// previous one.  One atomic min per workgroup.
 template <class T, bool FAST>
-__global__ __launch_bounds__(256) void k_corrector(KParams P, StepBuffers<T> B, DevLoop* L) {
+__global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuffers<T> B, DevLoop* L) {
     if (L->done) return;
     __shared__ Tile<T> s;
     const int t = threadIdx.x;
-    const int ibl = blockIdx.x;
-    const int ib = P.active[ibl];
-    const int nL = P.LeftBot[ib], nR = P.RightBot[ib], nB = P.BotLeft[ib], nT = P.TopLeft[ib];
-    const Cst<T> C = make_cst<T, FAST>(P);
+    const int x = t & 15, y = t >> 4;
+    const Sched S = schedule(blockIdx.x, gridDim.x, P.nblk);
+    const bool wall = P.aoibnd == 0;
+    const int cs = strip_cell(t);
+    T bmin = C.inv_epsi;
+    T ph = T(0), pz = T(0), pu = T(0), pv = T(0), sh = T(0), sz = T(0), su = T(0), sv = T(0);
+    auto fetch = [&](const Nb& n) {
+        const int i = memloc(x, y, n.ib);
+        ph = B.h[i]; pz = B.zb[i]; pu = B.u[i]; pv = B.v[i];
+        if (t < 128) {
+            const int src = strip_src(t, n.ib, n.nL, n.nR, n.nB, n.nT);
+            sh = B.h[src]; sz = B.zb[src]; su = B.u[src]; sv = B.v[src];
+        }
+    };
+    Nb cur;
+    if (S.first < S.end) {
+        cur = block_nb(P, S.first);
+        fetch(cur);
+    }
+    for (int ibl = S.first; ibl < S.end; ibl += S.stride) {
+        {
+            Q4<T> q;
+            q.a = ph; q.b = pz + ph; q.c = pu; q.d = pv;
+            s.q[tix(x, y)] = q;
+            if (t < 128) {
+                Q4<T> r;
+                r.a = sh; r.b = sz + sh; r.c = su; r.d = sv;
+                s.q[cs] = r;
+            }
+        }
+        __syncthreads();
+        const Nb n = cur;
+        if (ibl + S.stride < S.end) {
+            cur = block_nb(P, ibl + S.stride);
+            fetch(cur);
+        }
+        T dh, dhu, dhv;
+        T dtc = tile_fluxes<T, FAST, true>(s, C, t, wall && n.nL == n.ib, wall && n.nR == n.ib, wall && n.nB == n.ib,
+                                           wall && n.nT == n.ib, dh, dhu, dhv);
+        const int i = memloc(x, y, n.ib);
+        B.dh[i] = dh;
+        B.dhu[i] = dhu;
+        B.dhv[i] = dhv;
+        bmin = dtc < bmin ? dtc : bmin;
+    }
+    T m = block_min(bmin, s.red, t);
+    if (t == 0) slot_min(L, blockIdx.x, m);
+}
+
+// Persistent corrector with the same schedule and prefetch.
+template <class T, bool FAST>
+__global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuffers<T> B, DevLoop* L) {
+    if (L->done) return;
+    __shared__ Tile<T> s;
+    const int t = threadIdx.x;
+    const int x = t & 15, y = t >> 4;
+    const Sched S = schedule(blockIdx.x, gridDim.x, P.nblk);
+    const bool wall = P.aoibnd == 0;
+    const int cs = strip_cell(t);
     const T dt = T(L->dt);
     const T hdt = dt * T(0.5);
     const int fmc = P.ForceMassConserve;
-    // own cell: keep U^n for the full-step advance
-    const int x0 = t & 15, y0 = t >> 4;
-    const int i = memloc(x0, y0, ib);
-    const T h0 = B.h[i], u0 = B.u[i], v0 = B.v[i], zb0 = B.zb[i];
-    {
-        Q4<T> q;
-        advance<T, FAST>(C, fmc, hdt, h0, u0, v0, B.dh[i], B.dhu[i], B.dhv[i], q.a, q.c, q.d);
-        q.b = zb0 + q.a;
-        s.q[tix(x0, y0)] = q;
-    }
-    if (t < 128) {
-        int c, src;
-        strip_slot(t, ib, nL, nR, nB, nT, c, src);
-        Q4<T> q;
-        advance<T, FAST>(C, fmc, hdt, B.h[src], B.u[src], B.v[src], B.dh[src], B.dhu[src], B.dhv[src], q.a, q.c, q.d);
-        q.b = B.zb[src] + q.a;
-        s.q[c] = q;
-    }
-    __syncthreads();
-    const bool wall = P.aoibnd == 0;
-    T dh, dhu, dhv;
-    (void)tile_fluxes<T, FAST, false>(s, C, t, wall && nL == ib, wall && nR == ib, wall && nB == ib,
-                                      wall && nT == ib, dh, dhu, dhv);
-    T h1, u1, v1;
-    advance<T, FAST>(C, fmc, dt, h0, u0, v0, dh, dhu, dhv, h1, u1, v1);
-    // bottom friction, src/Friction.cu:89-148
-    if (h1 > C.eps) {
-        typedef Ar<T, FAST> A;
-        T normu = A::sqrt_(u1 * u1 + v1 * v1);
-        T cfi;
-        const T cfv = B.cf[i];
-        if (P.frictionmodel == 0) {
-            cfi = cfv;
-        } else if (P.frictionmodel == 1) {
-            T Hbar = h1 / cfv;
-            if (Hbar <= T(2.71828182845905)) cfi = T(1.0) / (T(0.46) * Hbar);
-            else cfi = T(1.0) / (T(2.5) * (log(Hbar) - T(1.0) + T(1.359) / Hbar));
-            cfi = cfi * cfi;
-        } else {
-            cfi = FAST ? C.g * cfv * cfv * (T)exp2f(-0.333333343f * log2f((float)h1)) : C.g * cfv * cfv / cbrt(h1);
+    T ph = T(0), pu = T(0), pv = T(0), pdh = T(0), pdhu = T(0), pdhv = T(0), pz = T(0), pcf = T(0);
+    T sh = T(0), su = T(0), sv = T(0), sdh = T(0), sdhu = T(0), sdhv = T(0), sz = T(0);
+    auto fetch = [&](const Nb& n) {
+        const int i = memloc(x, y, n.ib);
+        ph = B.h[i]; pu = B.u[i]; pv = B.v[i]; pdh = B.dh[i]; pdhu = B.dhu[i]; pdhv = B.dhv[i];
+        pz = B.zb[i]; pcf = B.cf[i];
+        if (t < 128) {
+            const int src = strip_src(t, n.ib, n.nL, n.nR, n.nB, n.nT);
+            sh = B.h[src]; su = B.u[src]; sv = B.v[src]; sdh = B.dh[src]; sdhu = B.dhu[src]; sdhv = B.dhv[src];
+            sz = B.zb[src];
         }
-        if (FAST) {
-            T tb = cfi * normu * A::rcp(h1) * dt;
-            T r = A::rcp(T(1.0) + tb);
-            u1 = u1 * r;
-            v1 = v1 * r;
-        } else {
-            T tb = cfi * normu / h1 * dt;
-            u1 = u1 / (T(1.0) + tb);
-            v1 = v1 / (T(1.0) + tb);
-        }
+    };
+    Nb cur;
+    if (S.first < S.end) {
+        cur = block_nb(P, S.first);
+        fetch(cur);
     }
-    // implicit uniform rain after cleanup, src/Updateforcing.cu:530-581
-    if (P.rain_on) {
-        T act = (double)zb0 < P.mask ? T(1.0) : T(0.0);
-        T rainhh = dmax(C.rainrate / T(1000.0) / T(3600.0) * dt, T(0.0)) * act;
-        T qvol = FAST ? h1 * Ar<T, FAST>::rcp(h1 + rainhh) : h1 / (h1 + rainhh);
-        bool wetold = (double)h1 > C.eps_d;
-        h1 = h1 + rainhh;
-        if (wetold) {
-            u1 = u1 * qvol;
-            v1 = v1 * qvol;
+    for (int ibl = S.first; ibl < S.end; ibl += S.stride) {
+        // this block's U^n (kept for the full-step advance) and its U* tile
+        const T h0 = ph, u0 = pu, v0 = pv, zb0 = pz, cf0 = pcf;
+        // Consume the prefetched values here, where the loads are waited for
+        // anyway, so that no wait inside the block's arithmetic can stall on
+        // the next block's prefetch.
+        asm volatile("" ::"v"(h0), "v"(u0), "v"(v0), "v"(zb0), "v"(cf0));
+        {
+            Q4<T> q;
+            advance<T, FAST>(C, fmc, hdt, h0, u0, v0, pdh, pdhu, pdhv, q.a, q.c, q.d);
+            q.b = zb0 + q.a;
+            s.q[tix(x, y)] = q;
+            if (t < 128) {
+                Q4<T> r;
+                advance<T, FAST>(C, fmc, hdt, sh, su, sv, sdh, sdhu, sdhv, r.a, r.c, r.d);
+                r.b = sz + r.a;
+                s.q[cs] = r;
+            }
         }
+        __syncthreads();
+        const Nb n = cur;
+        if (ibl + S.stride < S.end) {
+            cur = block_nb(P, ibl + S.stride);
+            fetch(cur);
+        }
+        const int ib = n.ib, nL = n.nL, nR = n.nR, nB = n.nB, nT = n.nT;
+        T dh, dhu, dhv;
+        (void)tile_fluxes<T, FAST, false>(s, C, t, wall && nL == ib, wall && nR == ib, wall && nB == ib,
+                                          wall && nT == ib, dh, dhu, dhv);
+        T h1, u1, v1;
+        advance<T, FAST>(C, fmc, dt, h0, u0, v0, dh, dhu, dhv, h1, u1, v1);
+        // bottom friction, src/Friction.cu:89-148
+        if (h1 > C.eps) {
+            typedef Ar<T, FAST> A;
+            T normu = A::sqrt_(u1 * u1 + v1 * v1);
+            T cfi;
+            if (P.frictionmodel == 0) {
+                cfi = cf0;
+            } else if (P.frictionmodel == 1) {
+                T Hbar = h1 / cf0;
+                if (Hbar <= T(2.71828182845905)) cfi = T(1.0) / (T(0.46) * Hbar);
+                else cfi = T(1.0) / (T(2.5) * (log(Hbar) - T(1.0) + T(1.359) / Hbar));
+                cfi = cfi * cfi;
+            } else {
+                cfi = FAST ? C.g * cf0 * cf0 * (T)exp2f(-0.333333343f * log2f((float)h1)) : C.g * cf0 * cf0 / cbrt(h1);
+            }
+            if (FAST) {
+                T tb = cfi * normu * A::rcp(h1) * dt;
+                T r = A::rcp(T(1.0) + tb);
+                u1 = u1 * r;
+                v1 = v1 * r;
+            } else {
+                T tb = cfi * normu / h1 * dt;
+                u1 = u1 / (T(1.0) + tb);
+                v1 = v1 / (T(1.0) + tb);
+            }
+        }
+        // implicit uniform rain after cleanup, src/Updateforcing.cu:530-581
+        if (P.rain_on) {
+            T act = (double)zb0 < C.mask_d ? T(1.0) : T(0.0);
+            T rainhh = dmax(C.rainrate / T(1000.0) / T(3600.0) * dt, T(0.0)) * act;
+            T qvol = FAST ? h1 * Ar<T, FAST>::rcp(h1 + rainhh) : h1 / (h1 + rainhh);
+            bool wetold = (double)h1 > C.eps_d;
+            h1 = h1 + rainhh;
+            if (wetold) {
+                u1 = u1 * qvol;
+                v1 = v1 * qvol;
+            }
+        }
+        const int i = memloc(x, y, ib);
+        B.ho[i] = h1;
+        B.uo[i] = u1;
+        B.vo[i] = v1;
     }
-    B.ho[i] = h1;
-    B.uo[i] = u1;
-    B.vo[i] = v1;
 }
 
 // One wave: min of the 64 block-minimum slots and 1/epsi (slots reset for
@@ -613,11 +722,12 @@ template <class T, bool FAST>
 void launch_step_t(const KParams& P, const StepBuffers<T>& B, DevLoop* L, double endtime, int advance_time,
                    hipStream_t st, hipEvent_t* ev) {
     if (ev) hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL((k_predictor<T, FAST>), dim3(P.nblk), dim3(256), 0, st, P, B, L);
+    const Cst<T> C = make_cst<T, FAST>(P);
+    hipLaunchKernelGGL((k_predictor<T, FAST>), dim3(P.grid_pred), dim3(256), 0, st, P, C, B, L);
     if (ev) hipEventRecord(ev[1], st);
     hipLaunchKernelGGL((k_reduce_dt<T>), dim3(1), dim3(64), 0, st, L, endtime, advance_time);
     if (ev) hipEventRecord(ev[2], st);
-    hipLaunchKernelGGL((k_corrector<T, FAST>), dim3(P.nblk), dim3(256), 0, st, P, B, L);
+    hipLaunchKernelGGL((k_corrector<T, FAST>), dim3(P.grid_corr), dim3(256), 0, st, P, C, B, L);
     if (ev) hipEventRecord(ev[3], st);
 }
 
@@ -629,6 +739,31 @@ void launch_step(int precision, const KParams& P, const void* buffers, DevLoop* 
         launch_step_t<float, false>(P, *(const StepBuffers<float>*)buffers, L, endtime, advance_time, st, ev);
     else
         launch_step_t<float, true>(P, *(const StepBuffers<float>*)buffers, L, endtime, advance_time, st, ev);
+}
+
+// Persistent grid: all workgroups resident at once (CUs x occupancy), a
+// multiple of NXCD, and no more than the blocks need.
+template <class K> int grid_for(K kernel, int nblk, int wg_per_cu) {
+    int dev = 0, ncu = 0, occ = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, 256, 0) != hipSuccess) return 0;
+    if (wg_per_cu > 0 && wg_per_cu < occ) occ = wg_per_cu;
+    if (occ < 1) occ = 1;
+    long nwg = (long)ncu * occ;
+    long need = ((long)nblk + NXCD - 1) / NXCD * NXCD;
+    if (nwg > need) nwg = need;
+    nwg = (nwg + NXCD - 1) / NXCD * NXCD;
+    return (int)nwg;
+}
+
+int persistent_grid(int precision, int corrector, int nblk, int wg_per_cu) {
+    const int w = wg_per_cu;
+    if (precision == 0)
+        return corrector ? grid_for(k_corrector<double, false>, nblk, w) : grid_for(k_predictor<double, false>, nblk, w);
+    if (precision == 1)
+        return corrector ? grid_for(k_corrector<float, false>, nblk, w) : grid_for(k_predictor<float, false>, nblk, w);
+    return corrector ? grid_for(k_corrector<float, true>, nblk, w) : grid_for(k_predictor<float, true>, nblk, w);
 }
 
 void launch_init_dt(int precision, const KParams& P, const void* h, DevLoop* L, hipStream_t st) {
