@@ -98,9 +98,11 @@ def lib():
         L.bgf_kernel_times.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), dp, ip]
         L.bgf_mesh_uniform.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                        ip, ip, dp, dp] + [ip] * 8
+        L.bgf_mesh_from_leaves.argtypes = [ctypes.c_int, ip, ip, ip, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_double, ip, ip, dp, dp] + [ip] * 8
         for name in ("bgf_create", "bgf_destroy", "bgf_set_state", "bgf_get_state", "bgf_init_dt", "bgf_step",
                      "bgf_run", "bgf_run_async", "bgf_sync", "bgf_volume", "bgf_device_array", "bgf_set_timing",
-                     "bgf_kernel_times", "bgf_mesh_uniform"):
+                     "bgf_kernel_times", "bgf_mesh_uniform", "bgf_mesh_from_leaves"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -129,6 +131,22 @@ def mesh_uniform(nbx, nby, dx, level=0, dbl=True):
                                   _dptr(out["xo"]), _dptr(out["yo"]), _iptr(out["LeftBot"]), _iptr(out["LeftTop"]),
                                   _iptr(out["RightBot"]), _iptr(out["RightTop"]), _iptr(out["TopLeft"]),
                                   _iptr(out["TopRight"]), _iptr(out["BotLeft"]), _iptr(out["BotRight"])))
+    return out
+
+
+def mesh_from_leaves(leaves, nbx, nby, dx, base=0):
+    """Block list of a quadtree mesh: leaves is a sequence of (level, ix, iy); leaf k is block k."""
+    lv = np.ascontiguousarray(np.asarray(leaves, np.int32).reshape(-1, 3))
+    n = len(lv)
+    lev, ix, iy = [np.ascontiguousarray(lv[:, k]) for k in range(3)]
+    out = {k: np.zeros(n, np.int32) for k in BLOCK_KEYS}
+    out["xo"] = np.zeros(n)
+    out["yo"] = np.zeros(n)
+    _check(lib().bgf_mesh_from_leaves(n, _iptr(lev), _iptr(ix), _iptr(iy), nbx, nby, base, dx, _iptr(out["active"]),
+                                      _iptr(out["level"]), _dptr(out["xo"]), _dptr(out["yo"]), _iptr(out["LeftBot"]),
+                                      _iptr(out["LeftTop"]), _iptr(out["RightBot"]), _iptr(out["RightTop"]),
+                                      _iptr(out["TopLeft"]), _iptr(out["TopRight"]), _iptr(out["BotLeft"]),
+                                      _iptr(out["BotRight"])))
     return out
 
 

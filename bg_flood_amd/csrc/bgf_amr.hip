@@ -1,0 +1,783 @@
+// bgf_amr.hip -- the Kurganov step on multi-level (quadtree) block meshes.
+//
+// On a mesh whose blocks differ in level the halo of a block is no longer a
+// copy of its neighbours' edge cells: it is a restriction (2x2 mean of a finer
+// neighbour), a barycentric prolongation from a coarser neighbour, a linear
+// reconstruction from the coarse gradients, and the coarse face fluxes are the
+// means of the fine ones.  These kernels keep the reference's Model<T> layout
+// (18x18 tiles; gradients and face fluxes stored per cell) because the
+// coarse/fine exchanges read other blocks' halos and faces, which the one-level
+// kernels never store.  Per stage:
+//
+//   k_amr_halo      halo of h, u, v (src/Halo.cu:2083-4609) and the wet/dry
+//                   restriction of h (src/ConserveElevation.cu:218-378,676-726);
+//                   zs = zb + h everywhere (src/Halo.cu:72-107) is formed where
+//                   it is read, never stored.
+//   k_amr_grad      theta-minmod gradients of h, zs, u, v (src/Gradients.cu:921-957)
+//                   and their halo values (src/Gradients.cu:2584-3294).
+//   k_amr_refine    linear prolongation of h, u, v into the halo of blocks
+//                   beside a coarser one (src/Halo.cu:1044-1405).
+//   k_amr_prolong   wet/dry prolongation (src/ConserveElevation.cu:80-205,
+//                   451-482), one launch per level, finest last.
+//   k_amr_grad      again.
+//   k_amr_flux      17x16 X and 16x17 Y Kurganov faces (src/Kurganov.cu:474-641,
+//                   1404-1546) with the coarse-face source substitution, the
+//                   per-cell dtmax and the block minimum of the step.
+//   k_amr_fluxfix   coarse face flux = mean of the fine faces (src/Halo.cu:
+//                   1441-1862) and the wall patch (src/Boundary.cu:587-760).
+//   k_amr_advance   tendencies (src/Advection.cu:160-254), advance
+//                   (src/Advection.cu:332-385), and in the corrector bottom
+//                   friction, cleanup and implicit rain.
+//
+// Threads map to halo cells by (side, j): side 0 left, 1 right, 2 bottom,
+// 3 top, j the position along the side.  Every formula keeps the reference
+// CPU step's operation order and precision (this file is built with
+// -ffp-contract=off), including the per-side differences of the reference:
+// the order of the four terms of the gradient-halo means, the argument order
+// of the bilinear interpolation, and refine_linear's double-precision
+// ilevdx on the left and top sides.
+//
+// Sequential order.  The reference visits blocks in active-list order, and in
+// the wet/dry prolongation a block's test reads the halo of its coarser
+// neighbour, which that neighbour's own prolongation may already have
+// rewritten.  Levels are processed coarsest first, and a block reads its
+// coarse neighbour's halo from the current array if that neighbour comes
+// earlier in the active list, else from the copy taken before the
+// prolongation (hsnap).  Every other stage only reads cells that the same
+// launch does not write, so blocks run in parallel.
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "bgf_common.h"
+
+namespace bgf {
+namespace {
+
+enum { SL = 0, SR = 1, SB = 2, ST = 3 };
+
+template <class T> __device__ __forceinline__ T epsi_of() {
+    return sizeof(T) == 8 ? T(2.220446049250313e-16) : T(1.1920928955078125e-07);
+}
+template <class T> __device__ __forceinline__ T tmax_of() { return sizeof(T) == 8 ? T(DBL_MAX) : T(FLT_MAX); }
+
+// std::max / std::min with their NaN and signed-zero behaviour.
+template <class T> __device__ __forceinline__ T smax(T a, T b) { return (a < b) ? b : a; }
+template <class T> __device__ __forceinline__ T smin(T a, T b) { return (b < a) ? b : a; }
+
+// src/Util_CPU.cu:308-311
+template <class T> __device__ __forceinline__ int ftoi(T v) {
+    return v >= T(0) ? (int)(v + T(0.5)) : (int)(v - T(0.5));
+}
+
+// Neighbours covering the first half (bottom or left) and second half (top or
+// right) of a side; a block is its own neighbour at the domain edge.
+__device__ __forceinline__ int nb_first(const KParams& P, int s, int b) {
+    return s == SL ? P.LeftBot[b] : s == SR ? P.RightBot[b] : s == SB ? P.BotLeft[b] : P.TopLeft[b];
+}
+__device__ __forceinline__ int nb_second(const KParams& P, int s, int b) {
+    return s == SL ? P.LeftTop[b] : s == SR ? P.RightTop[b] : s == SB ? P.BotRight[b] : P.TopRight[b];
+}
+// For a coarser neighbour C on side s of a block: C's first-half neighbour on
+// the shared side is the block itself when the block covers that half.
+__device__ __forceinline__ int nb_back(const KParams& P, int s, int C) { return nb_first(P, s ^ 1, C); }
+// Neighbours of a coarser block C next to the ends of the shared side: before
+// its first cell (j == 0) and after its last (j == 15), on the half facing us.
+__device__ __forceinline__ int corner_lo(const KParams& P, int s, int C) {
+    const int side = s < 2 ? SB : SL;
+    return (s == SL || s == SB) ? nb_second(P, side, C) : nb_first(P, side, C);
+}
+__device__ __forceinline__ int corner_hi(const KParams& P, int s, int C) {
+    const int side = s < 2 ? ST : SR;
+    return (s == SL || s == SB) ? nb_second(P, side, C) : nb_first(P, side, C);
+}
+// Cell at depth d from side s of block b, at position k along the side
+// (d = -1 is the halo cell).
+__device__ __forceinline__ int edge_cell(int s, int d, int k, int b) {
+    return s == SL ? memloc(d, k, b) : s == SR ? memloc(BW - 1 - d, k, b) : s == SB ? memloc(k, d, b)
+                                                                                   : memloc(k, BW - 1 - d, b);
+}
+
+// A field value: the array itself, or zs = zb + h.
+template <class T, bool ZS> struct Fld {
+    const T* a;
+    const T* zb;
+    __device__ __forceinline__ T operator()(int c) const { return ZS ? zb[c] + a[c] : a[c]; }
+};
+
+// ------------------------------------------------------------- halo fill
+// 2x2 mean of a finer neighbour's cells next to the shared edge.
+template <class T> __device__ __forceinline__ T mean_fine(int os, int n, int jj, const T* z) {
+    return T(0.25) * (z[edge_cell(os, 0, jj, n)] + z[edge_cell(os, 1, jj, n)] + z[edge_cell(os, 0, jj + 1, n)] +
+                      z[edge_cell(os, 1, jj + 1, n)]);
+}
+
+// Halo value of side s, position j (src/Halo.cu:2083-2286 and its three mirrors).
+template <class T> __device__ T fill_value(const KParams& P, int s, int j, int ib, const T* z) {
+    const int os = s ^ 1;
+    const int n1 = nb_first(P, s, ib), n2 = nb_second(P, s, ib);
+    const int lev = P.level[ib];
+    if (n1 == ib) {
+        if (j < BW / 2 || n2 == ib) return z[edge_cell(s, 0, j, ib)];
+        return mean_fine(os, n2, (j - BW / 2) * 2, z);
+    }
+    const int l1 = P.level[n1];
+    if (l1 == lev) return z[edge_cell(os, 0, j, n1)];
+    if (l1 > lev) {
+        if (j < BW / 2) return mean_fine(os, n1, j * 2, z);
+        if (n2 == ib) return z[edge_cell(s, 0, j, ib)];
+        return mean_fine(os, n2, (j - BW / 2) * 2, z);
+    }
+    // coarser neighbour: 1/3 own edge cell + {1/6, 1/2} of the two nearest coarse cells
+    const int C = n1;
+    const bool first = nb_back(P, s, C) == ib;
+    const T cj = ceil(T(j) * T(0.5));
+    const int jjc = first ? ftoi(cj) : ftoi(cj + T(BW / 2));
+    const bool up = cj * T(2) > T(j);
+    T w1 = T(1.0 / 3.0);
+    T w2 = up ? T(1.0 / 6.0) : T(0.5);
+    T w3 = up ? T(0.5) : T(1.0 / 6.0);
+    const int ii = edge_cell(s, 0, j, ib);
+    int ir = edge_cell(os, 0, jjc, C);
+    int it = edge_cell(os, 0, jjc - 1, C);
+    if (first) {
+        if (j == 0) {
+            const int n = corner_lo(P, s, C);
+            if (n == C) {
+                // T(0.5*(1.0-w1)) and T(0.5)*(T(1.0)-w1) round to the same float
+                w3 = T(0.5 * (1.0 - (double)w1));
+                w2 = w3;
+                it = ir;
+            } else {
+                it = edge_cell(os, 0, BW - 1, n);
+                if (P.level[n] < P.level[C]) { w1 = T(4.0 / 10.0); w2 = T(5.0 / 10.0); w3 = T(1.0 / 10.0); }
+                else if (P.level[n] > P.level[C]) { w1 = T(1.0 / 4.0); w2 = T(1.0 / 2.0); w3 = T(1.0 / 4.0); }
+            }
+        }
+    } else if (j == BW - 1) {
+        const int C2 = nb_second(P, s, ib);
+        const int n = corner_hi(P, s, C2);
+        if (n == C2) {
+            w3 = T(0.5 * (1.0 - (double)w1));
+            w2 = w3;
+            ir = it;
+        } else {
+            ir = edge_cell(os, 0, 0, n);
+            if (P.level[n] < P.level[C2]) { w1 = T(4.0 / 10.0); w2 = T(1.0 / 10.0); w3 = T(5.0 / 10.0); }
+            else if (P.level[n] > P.level[C2]) { w1 = T(1.0 / 4.0); w2 = T(1.0 / 2.0); w3 = T(1.0 / 4.0); }
+        }
+    }
+    return w1 * z[ii] + w2 * z[ir] + w3 * z[it];
+}
+
+// Wet/dry restriction of one coarse halo cell beside a finer block
+// (src/ConserveElevation.cu:676-726): the 2x2 fine patch with lower-left
+// cell (i0, j0).  Returns false when the value is left as filled.
+template <class T>
+__device__ bool wetdry_restrict(T eps, int n, int i0, int j0, const T* h, const T* zb, T zbw, T& out) {
+    const int ii = memloc(i0, j0, n), ir = memloc(i0 + 1, j0, n), it = memloc(i0, j0 + 1, n),
+              itr = memloc(i0 + 1, j0 + 1, n);
+    const T hii = h[ii], hir = h[ir], hit = h[it], hitr = h[itr];
+    const T iiwet = hii > eps ? T(1.0) : T(0.0);
+    const T irwet = hir > eps ? T(1.0) : T(0.0);
+    const T itwet = hit > eps ? T(1.0) : T(0.0);
+    const T itrwet = hitr > eps ? T(1.0) : T(0.0);
+    const T cwet = iiwet + irwet + itwet + itrwet;
+    T hwet = iiwet * hii + irwet * hir + itwet * hit + itrwet * hitr;
+    T zswet = (iiwet * hii) * (zb[ii] + h[ii]) + (irwet * hir) * (zb[ir] + h[ir]) + (itwet * hit) * (zb[it] + h[it]) +
+              (itrwet * hitr) * (zb[itr] + h[itr]);
+    if (cwet > T(0.0) && cwet < T(4.0)) {
+        zswet = zswet / hwet;
+        out = smax(T(0.0), zswet - zbw);
+        return true;
+    }
+    return false;
+}
+
+// Four blocks per workgroup, 64 threads per block: (side, j) = (t/16 % 4, t % 16).
+__device__ __forceinline__ int quad_block(const KParams& P, int t) {
+    const int ibl = blockIdx.x * 4 + (t >> 6);
+    return ibl < P.nblk ? P.active[ibl] : -1;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_amr_halo(KParams P, T* h, T* u, T* v, const T* zb, DevLoop* L, int corr) {
+    if (L && (corr ? !L->go : L->done)) return;
+    const int t = threadIdx.x;
+    const int ib = quad_block(P, t);
+    if (ib < 0) return;
+    const int s = (t >> 4) & 3, j = t & 15;
+    const int w = edge_cell(s, -1, j, ib);
+    T hv = fill_value(P, s, j, ib, h);
+    const T uv = fill_value(P, s, j, ib, u);
+    const T vv = fill_value(P, s, j, ib, v);
+    if (P.wetdryfix) {
+        const int n = j < BW / 2 ? nb_first(P, s, ib) : nb_second(P, s, ib);
+        if (P.level[ib] < P.level[n]) {
+            const int jr = (j & (BW / 2 - 1)) * 2;
+            const int i0 = s == SL ? BW - 2 : s == SR ? 0 : jr;
+            const int j0 = s == SB ? BW - 2 : s == ST ? 0 : jr;
+            T r;
+            if (wetdry_restrict(T(P.eps), n, i0, j0, h, zb, zb[w], r)) hv = r;
+        }
+    }
+    h[w] = hv;
+    u[w] = uv;
+    v[w] = vv;
+}
+
+// Halo of one static field (the bed at set-up, src/InitialConditions.cu:135-147).
+template <class T> __global__ __launch_bounds__(256) void k_amr_fill(KParams P, T* z) {
+    const int t = threadIdx.x;
+    const int ib = quad_block(P, t);
+    if (ib < 0) return;
+    const int s = (t >> 4) & 3, j = t & 15;
+    z[edge_cell(s, -1, j, ib)] = fill_value(P, s, j, ib, z);
+}
+
+// ------------------------------------------------------------- gradients
+// src/Util_CPU.cu:243-263
+template <class T> __device__ __forceinline__ T minmod2(T theta, T s0, T s1, T s2) {
+    if (s0 < s1 && s1 < s2) {
+        T d1 = theta * (s1 - s0);
+        const T d2 = (s2 - s0) / T(2.0);
+        const T d3 = theta * (s2 - s1);
+        if (d2 < d1) d1 = d2;
+        return smin(d1, d3);
+    }
+    if (s0 > s1 && s1 > s2) {
+        T d1 = theta * (s1 - s0);
+        const T d2 = (s2 - s0) / T(2.0);
+        const T d3 = theta * (s2 - s1);
+        if (d2 > d1) d1 = d2;
+        return smax(d1, d3);
+    }
+    return T(0.0);
+}
+
+// src/Util_CPU.cu:129-144 on the unit square.
+template <class T> __device__ __forceinline__ T bilinear(T q11, T q12, T q21, T q22, T x, T y) {
+    const T x1 = T(0.0), x2 = T(1.0), y1 = T(0.0), y2 = T(1.0);
+    const T x2x1 = x2 - x1, y2y1 = y2 - y1, x2x = x2 - x, y2y = y2 - y, yy1 = y - y1, xx1 = x - x1;
+    return T(1.0) / (x2x1 * y2y1) * (q11 * x2x * y2y + q21 * xx1 * y2y + q12 * x2x * yy1 + q22 * xx1 * yy1);
+}
+
+// Value two cells beyond the edge on side s, the outer point of the limited
+// gradient at the halo cell (src/Gradients.cu:2713-3294).
+template <class T, class F> __device__ T grad_far(const KParams& P, int s, int j, int ib, const F& a) {
+    const int os = s ^ 1;
+    const int n1 = nb_first(P, s, ib), n2 = nb_second(P, s, ib);
+    const int lev = P.level[ib];
+    // finer neighbour: mean of its depth-2 and depth-3 cells; the left and
+    // bottom sides add the depth-2 cell first, the right and top sides the depth-3 cell
+    const int da = (s == SL || s == SB) ? 2 : 3, db = 5 - da;
+    auto mean2 = [&](int n, int jj) {
+        return T(0.25) * (a(edge_cell(os, da, jj, n)) + a(edge_cell(os, db, jj, n)) + a(edge_cell(os, da, jj + 1, n)) +
+                          a(edge_cell(os, db, jj + 1, n)));
+    };
+    if (n1 == ib) {
+        if (j < BW / 2 || n2 == ib) return a(edge_cell(s, 0, j, ib));
+        return mean2(n2, (j - BW / 2) * 2);
+    }
+    const int l1 = P.level[n1];
+    if (l1 == lev) return a(edge_cell(os, 1, j, n1));
+    if (l1 > lev) {
+        if (j < BW / 2) return mean2(n1, j * 2);
+        if (n2 == ib) return a(edge_cell(s, 0, j, ib));
+        return mean2(n2, (j - BW / 2) * 2);
+    }
+    const int C = n1;
+    const T cj = ceil(T(j) * T(0.5));
+    const int jj = nb_back(P, s, C) == ib ? ftoi(cj) : ftoi(cj + T(BW / 2));
+    const T jr = cj * T(2) > T(j) ? T(0.25) : T(0.75);
+    const T p0 = a(edge_cell(os, 0, jj, C)), p1 = a(edge_cell(os, 1, jj, C));
+    const T q0 = a(edge_cell(os, 0, jj - 1, C)), q1 = a(edge_cell(os, 1, jj - 1, C));
+    switch (s) {
+        case SL: return bilinear(q1, p1, q0, p0, T(0.75), jr);
+        case SR: return bilinear(q0, p0, q1, p1, T(0.25), jr);
+        case SB: return bilinear(q1, q0, p1, p0, jr, T(0.75));
+        default: return bilinear(q0, q1, p0, p1, jr, T(0.25));
+    }
+}
+
+template <class T, class F>
+__device__ __forceinline__ void grad_cell(const KParams& P, T theta, T delta, int ib, int x, int y, const F& a,
+                                          T* dadx, T* dady) {
+    const int i = memloc(x, y, ib);
+    const T ai = a(i);
+    dadx[i] = minmod2(theta, a(memloc(x - 1, y, ib)), ai, a(memloc(x + 1, y, ib))) / delta;
+    dady[i] = minmod2(theta, a(memloc(x, y - 1, ib)), ai, a(memloc(x, y + 1, ib))) / delta;
+}
+
+template <class T, class F>
+__device__ __forceinline__ void grad_halo(const KParams& P, T theta, T delta, int ib, int s, int j, const F& a,
+                                          T* dadx, T* dady) {
+    const int i = edge_cell(s, -1, j, ib);
+    const T own = a(edge_cell(s, 0, j, ib));
+    const T far = grad_far<T>(P, s, j, ib, a);
+    const T g = (s == SL || s == SB) ? minmod2(theta, far, a(i), own) : minmod2(theta, own, a(i), far);
+    (s < 2 ? dadx : dady)[i] = g / delta;
+}
+
+template <class T> struct GradOut {
+    T *dhdx, *dhdy, *dzsdx, *dzsdy, *dudx, *dudy, *dvdx, *dvdy;
+};
+
+template <class T>
+__global__ __launch_bounds__(256) void k_amr_grad(KParams P, const T* h, const T* u, const T* v, const T* zb,
+                                                  GradOut<T> G, DevLoop* L, int corr) {
+    if (corr ? !L->go : L->done) return;
+    const int t = threadIdx.x;
+    const int ib = P.active[blockIdx.x];
+    const T theta = T(P.theta);
+    const T delta = calcres(T(P.delta0), P.level[ib]);
+    const Fld<T, false> fh{h, zb}, fu{u, zb}, fv{v, zb};
+    const Fld<T, true> fzs{h, zb};
+    const int x = t & 15, y = t >> 4;
+    grad_cell(P, theta, delta, ib, x, y, fh, G.dhdx, G.dhdy);
+    grad_cell(P, theta, delta, ib, x, y, fzs, G.dzsdx, G.dzsdy);
+    grad_cell(P, theta, delta, ib, x, y, fu, G.dudx, G.dudy);
+    grad_cell(P, theta, delta, ib, x, y, fv, G.dvdx, G.dvdy);
+    if (t < 64) {
+        const int s = t >> 4, j = t & 15;
+        grad_halo(P, theta, delta, ib, s, j, fh, G.dhdx, G.dhdy);
+        grad_halo(P, theta, delta, ib, s, j, fzs, G.dzsdx, G.dzsdy);
+        grad_halo(P, theta, delta, ib, s, j, fu, G.dudx, G.dudy);
+        grad_halo(P, theta, delta, ib, s, j, fv, G.dvdx, G.dvdy);
+    }
+}
+
+// ----------------------------------------------------------- prolongation
+// Linear reconstruction z_c + (+-dzdx_c +- dzdy_c) * dx_f / 2 of one halo
+// cell beside a coarser block (src/Halo.cu:1044-1066 and mirrors).
+template <class T>
+__device__ __forceinline__ void refine_one(const KParams& P, int s, int j, int ib, int lev, T* z, const T* dzdx,
+                                           const T* dzdy) {
+    const int C = nb_first(P, s, ib);
+    const T fj = floor(T(j) * T(0.5));
+    const int jj = nb_back(P, s, C) == ib ? ftoi(fj) : ftoi(fj + T(BW / 2));
+    const int il = edge_cell(s ^ 1, 0, jj, C);
+    const T along = fj * T(2.0) < (T(j) - T(0.01)) ? T(1.0) : T(-1.0);
+    const T nrm = (s == SL || s == SB) ? T(1.0) : T(-1.0);
+    const T faclr = s < 2 ? nrm : along, facbt = s < 2 ? along : nrm;
+    const int w = edge_cell(s, -1, j, ib);
+    const T slope = faclr * dzdx[il] + facbt * dzdy[il];
+    if (s == SL || s == ST) {
+        const double ilevdx = calcres(P.delta0, lev) * (double)T(0.5);
+        z[w] = T((double)z[il] + (double)slope * ilevdx);
+    } else {
+        const T ilevdx = calcres(T(P.delta0), lev) * T(0.5);
+        z[w] = z[il] + slope * ilevdx;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_amr_refine(KParams P, T* h, T* u, T* v, GradOut<T> G, T* hsnap, DevLoop* L,
+                                                    int corr) {
+    if (corr ? !L->go : L->done) return;
+    const int t = threadIdx.x;
+    const int ib = quad_block(P, t);
+    if (ib < 0) return;
+    const int s = (t >> 4) & 3, j = t & 15;
+    const int lev = P.level[ib];
+    if (P.level[nb_first(P, s, ib)] < lev) {
+        refine_one(P, s, j, ib, lev, h, G.dhdx, G.dhdy);
+        refine_one(P, s, j, ib, lev, u, G.dudx, G.dudy);
+        refine_one(P, s, j, ib, lev, v, G.dvdx, G.dvdy);
+    }
+    if (hsnap) {
+        const int w = edge_cell(s, -1, j, ib);
+        hsnap[w] = h[w];
+    }
+}
+
+// Wet/dry prolongation of the blocks of one level (src/ConserveElevation.cu:80-205):
+// where the coarse cell or one of its four neighbours is dry, the halo takes
+// the coarse cell's h and zb.
+template <class T>
+__global__ __launch_bounds__(256) void k_amr_prolong(KParams P, T* h, T* zb, const T* hsnap, int level, DevLoop* L,
+                                                     int corr) {
+    if (corr ? !L->go : L->done) return;
+    const int t = threadIdx.x;
+    const int ib = quad_block(P, t);
+    if (ib < 0) return;
+    const int lev = P.level[ib];
+    if (lev != level) return;
+    const int s = (t >> 4) & 3, j = t & 15;
+    const int C = nb_first(P, s, ib);
+    if (!(lev > P.level[C])) return;
+    const T eps = T(P.eps);
+    const T fj = floor(T(j) * T(0.5));
+    const int jp = nb_back(P, s, C) == ib ? ftoi(fj) : ftoi(fj + T(BW / 2));
+    const int cx = s == SL ? BW - 1 : s == SR ? 0 : jp;
+    const int cy = s == SB ? BW - 1 : s == ST ? 0 : jp;
+    // C's halo as the sequential reference sees it when it reaches this block
+    const T* hc = P.inv[C] < P.inv[ib] ? h : hsnap;
+    auto hv = [&](int x, int y) {
+        const int c = memloc(x, y, C);
+        return (x < 0 || x >= BW || y < 0 || y >= BW) ? hc[c] : h[c];
+    };
+    const int ii = memloc(cx, cy, C);
+    if (!(h[ii] > eps && hv(cx - 1, cy) > eps && hv(cx + 1, cy) > eps && hv(cx, cy + 1) > eps &&
+          hv(cx, cy - 1) > eps)) {
+        const int w = edge_cell(s, -1, j, ib);
+        h[w] = h[ii];
+        zb[w] = zb[ii];
+    }
+}
+
+// ----------------------------------------------------------------- fluxes
+// src/Kurganov.cu:1865-1911
+template <class T>
+__device__ __forceinline__ T kurg_solver(T g, T delta, T epsi, T CFL, T cm, T fm, T hp, T hm, T up, T um, T& fh,
+                                         T& fu) {
+    const T cp = sqrt(g * hp);
+    const T cmo = sqrt(g * hm);
+    const T ap = smax(smax(up + cp, um + cmo), T(0.0));
+    const T am = smin(smin(up - cp, um - cmo), T(0.0));
+    const T ad = T(1.0) / (ap - am);
+    const T qm = hm * um;
+    const T qp = hp * up;
+    const T hm2 = hm * hm;
+    const T hp2 = hp * hp;
+    const T a = smax(ap, -am);
+    const T ga = g * T(0.5);
+    const T apm = ap * am;
+    const T dlt = delta * cm / fm;
+    if (a > epsi) {
+        fh = (ap * qm - am * qp + apm * (hp - hm)) * ad;
+        fu = (ap * (qm * um + ga * hm2) - am * (qp * up + ga * hp2) + apm * (qp - qm)) * ad;
+        return CFL * dlt / a;
+    }
+    fh = T(0.0);
+    fu = T(0.0);
+    return T(1.0) / epsi;
+}
+
+template <class T> struct StageIn {
+    const T *h, *u, *v;
+};
+template <class T> struct FluxOut {
+    T *Fhu, *Fqux, *Su, *Fqvx, *Fhv, *Fqvy, *Sv, *Fquy;
+};
+
+// One face between cell n (left or below) and cell i.  Y faces swap the roles
+// of u and v.  hs/zs_s: h and zb substituted into the source terms at a face
+// shared with a coarser block (src/Kurganov.cu:132-145).  Returns whether the
+// face is wet; dt receives the face's CFL step.
+template <class T>
+__device__ __forceinline__ bool kface(const KParams& P, T delta, const T* h, const T* zb, const T* un_, const T* ut_,
+                                      const T* dhd, const T* dzsd, const T* dund, const T* dutd, int i, int n,
+                                      int subi, int subn, T* F0, T* F1, T* F2, T* F3, T& dt) {
+    const T g = T(P.g), CFL = T(P.CFL);
+    const T epsi = epsi_of<T>();
+    const T eps = T(P.eps) + epsi;
+    T hi = h[i], hn = h[n];
+    if (!(hi > eps || hn > eps)) {
+        F0[i] = T(0.0);
+        F1[i] = T(0.0);
+        F2[i] = T(0.0);
+        F3[i] = T(0.0);
+        dt = T(1.0) / epsi;
+        return false;
+    }
+    const T dhdi = dhd[i], dhdn = dhd[n];
+    const T dx = delta * T(0.5);
+    T zi = (zb[i] + hi) - hi;
+    const T zl = zi - dx * (dzsd[i] - dhdi);
+    T zn = (zb[n] + hn) - hn;
+    const T zr = zn + dx * (dzsd[n] - dhdn);
+    const T zlr = smax(zl, zr);
+    const T hl = hi - dx * dhdi;
+    const T up = un_[i] - dx * dund[i];
+    const T hp = smax(T(0.0), hl + zl - zlr);
+    const T hr = hn + dx * dhdn;
+    const T um = un_[n] + dx * dund[n];
+    const T hm = smax(T(0.0), hr + zr - zlr);
+    const T ga = g * T(0.5);
+    T fh, fu, fv;
+    dt = kurg_solver(g, delta, epsi, CFL, T(1.0), T(1.0), hp, hm, up, um, fh, fu);
+    if (fh > T(0.0)) fv = (ut_[n] + dx * dutd[n]) * fh;
+    else fv = (ut_[i] - dx * dutd[i]) * fh;
+    if (subi >= 0) {
+        hi = h[subi];
+        zi = zb[subi];
+    }
+    if (subn >= 0) {
+        hn = h[subn];
+        zn = zb[subn];
+    }
+    const T sl = ga * (hp * hp - hl * hl + (hl + hi) * (zi - zl));
+    const T sr = ga * (hm * hm - hr * hr + (hr + hn) * (zn - zr));
+    const T fmu = T(1.0);
+    F0[i] = fmu * fh;
+    F1[i] = fmu * (fu - sl);
+    F2[i] = fmu * (fu - sr);
+    F3[i] = fmu * fv;
+    return true;
+}
+
+// X face ix (0..16) of row iy: the source substitution uses the coarse cell
+// beside the face when the block's right (ix = 16) or left (ix = 0) neighbour
+// is coarser.
+template <class T>
+__device__ __forceinline__ bool xface(const KParams& P, T delta, int ib, int lev, int ix, int iy, const StageIn<T>& E,
+                                      const T* zb, const GradOut<T>& G, const FluxOut<T>& F, T& dt) {
+    int subi = -1, subn = -1;
+    if (ix == BW) {
+        const int RB = P.RightBot[ib];
+        if (P.level[RB] < lev) {
+            const T f = floor(T(iy) * T(0.5));
+            const int jj = P.LeftBot[RB] == ib ? ftoi(f) : ftoi(f + T(BW / 2));
+            subi = memloc(0, jj, RB);
+        }
+    }
+    if (ix == 0) {
+        const int LB = P.LeftBot[ib];
+        if (P.level[LB] < lev) {
+            const T f = floor(T(iy) * T(0.5));
+            const int jj = P.RightBot[LB] == ib ? ftoi(f) : ftoi(f + T(BW / 2));
+            subn = memloc(BW - 1, jj, LB);
+        }
+    }
+    return kface(P, delta, E.h, zb, E.u, E.v, G.dhdx, G.dzsdx, G.dudx, G.dvdx, memloc(ix, iy, ib),
+                 memloc(ix - 1, iy, ib), subi, subn, F.Fhu, F.Fqux, F.Su, F.Fqvx, dt);
+}
+
+template <class T>
+__device__ __forceinline__ bool yface(const KParams& P, T delta, int ib, int lev, int ix, int iy, const StageIn<T>& E,
+                                      const T* zb, const GradOut<T>& G, const FluxOut<T>& F, T& dt) {
+    int subi = -1, subn = -1;
+    if (iy == BW) {
+        const int TL = P.TopLeft[ib];
+        if (P.level[TL] < lev) {
+            const T f = floor(T(ix) * T(0.5));
+            const int jj = P.BotLeft[TL] == ib ? ftoi(f) : ftoi(f + T(BW / 2));
+            subi = memloc(jj, 0, TL);
+        }
+    }
+    if (iy == 0) {
+        const int BL = P.BotLeft[ib];
+        if (P.level[BL] < lev) {
+            const T f = floor(T(ix) * T(0.5));
+            const int jj = P.TopLeft[BL] == ib ? ftoi(f) : ftoi(f + T(BW / 2));
+            subn = memloc(jj, BW - 1, BL);
+        }
+    }
+    return kface(P, delta, E.h, zb, E.v, E.u, G.dhdy, G.dzsdy, G.dvdy, G.dudy, memloc(ix, iy, ib),
+                 memloc(ix, iy - 1, ib), subi, subn, F.Fhv, F.Fqvy, F.Sv, F.Fquy, dt);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_amr_flux(KParams P, StageIn<T> E, const T* zb, GradOut<T> G, FluxOut<T> F,
+                                                  DevLoop* L, int corr) {
+    if (corr ? !L->go : L->done) return;
+    __shared__ T red[4];
+    const int t = threadIdx.x;
+    const int ib = P.active[blockIdx.x];
+    const int lev = P.level[ib];
+    const T delta = calcres(T(P.delta0), lev);
+    const int x = t & 15, y = t >> 4;
+    const T inv_epsi = T(1.0) / epsi_of<T>();
+    T d = tmax_of<T>(), dt;
+    // dtmax of the cell: its X face (left), then its Y face (bottom), each
+    // keeping the smaller step, a dry face resetting it to 1/epsi
+    if (xface(P, delta, ib, lev, x, y, E, zb, G, F, dt)) d = dt < d ? dt : d;
+    else d = inv_epsi;
+    if (yface(P, delta, ib, lev, x, y, E, zb, G, F, dt)) d = dt < d ? dt : d;
+    else d = inv_epsi;
+    if (x == BW - 1) (void)xface(P, delta, ib, lev, BW, y, E, zb, G, F, dt);
+    if (y == BW - 1) (void)yface(P, delta, ib, lev, x, BW, E, zb, G, F, dt);
+    if (!corr) {
+        const T m = block_min(d, red, t);
+        if (t == 0) slot_min(L, blockIdx.x, m);
+    }
+}
+
+// Face position of side s at j: left and bottom faces are stored at the
+// block's first cells, right and top faces at its halo.
+__device__ __forceinline__ int face_at(int s, int j, int b) {
+    return s == SL ? memloc(0, j, b) : s == SR ? memloc(BW, j, b) : s == SB ? memloc(j, 0, b) : memloc(j, BW, b);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_amr_fluxfix(KParams P, FluxOut<T> F, DevLoop* L, int corr) {
+    if (corr ? !L->go : L->done) return;
+    const int t = threadIdx.x;
+    const int ib = quad_block(P, t);
+    if (ib < 0) return;
+    const int s = (t >> 4) & 3, j = t & 15;
+    const int lev = P.level[ib];
+    T* const A[4] = {s < 2 ? F.Fhu : F.Fhv, s < 2 ? F.Fqux : F.Fqvy, s < 2 ? F.Su : F.Sv, s < 2 ? F.Fqvx : F.Fquy};
+    // coarse face = mean of the two fine faces (src/Halo.cu:1441-1529, 1766-1862)
+    const int n = j < BW / 2 ? nb_first(P, s, ib) : nb_second(P, s, ib);
+    if (P.level[n] > lev) {
+        const int jj = (j & (BW / 2 - 1)) * 2;
+        const int i = face_at(s, j, ib), il = face_at(s ^ 1, jj, n), itl = face_at(s ^ 1, jj + 1, n);
+#pragma unroll
+        for (int k = 0; k < 4; k++) A[k][i] = T(0.5) * (A[k][il] + A[k][itl]);
+    }
+    // wall boundary (noslipbndQ, src/Boundary.cu:1887-1891): no mass flux through
+    // the boundary face and no normal-momentum flux difference in the edge cell
+    if (P.aoibnd == 0 && (nb_first(P, s, ib) == ib || nb_second(P, s, ib) == ib)) {
+        const int i = face_at(s, j, ib);
+        A[0][i] = T(0.0);
+        if (s == SL) F.Su[memloc(1, j, ib)] = F.Fqux[i];
+        else if (s == SR) F.Fqux[memloc(BW - 1, j, ib)] = F.Su[i];
+        else if (s == SB) F.Sv[memloc(j, 1, ib)] = F.Fqvy[i];
+        else F.Fqvy[memloc(j, BW - 1, ib)] = F.Sv[i];
+    }
+}
+
+// --------------------------------------------------------------- advance
+template <class T> struct AdvArgs {
+    const T *h, *u, *v;        // stage state E of the tendencies
+    const T *hold, *uold, *vold;  // state U^n the step starts from
+    T *ho, *uo, *vo;           // output
+    const T* zb;
+    const T* cf;
+};
+
+template <class T>
+__global__ __launch_bounds__(256) void k_amr_advance(KParams P, AdvArgs<T> A, FluxOut<T> F, DevLoop* L, int corr) {
+    if (!L->go) return;
+    const int t = threadIdx.x;
+    const int ib = P.active[blockIdx.x];
+    const int x = t & 15, y = t >> 4;
+    const int i = memloc(x, y, ib), iright = memloc(x + 1, y, ib), itop = memloc(x, y + 1, ib);
+    const T delta = calcres(T(P.delta0), P.level[ib]);
+    const T g = T(P.g);
+    const T fc = T(P.fc);
+    // src/Advection.cu:160-254 (Cartesian metric terms)
+    const T cm = T(1.0), fmu = T(1.0), fmv = T(1.0), fmup = T(1.0), fmvp = T(1.0);
+    const T hi = A.h[i], uui = A.u[i], vvi = A.v[i];
+    const T cmdinv = T(1.0) / (cm * delta);
+    const T ga = T(0.5) * g;
+    const T dhi = T(-1.0) * (F.Fhu[iright] - F.Fhu[i] + F.Fhv[itop] - F.Fhv[i]) * cmdinv;
+    const T dmdl = (fmup - fmu) / (cm * delta);
+    const T dmdt = (fmvp - fmv) / (cm * delta);
+    const T fG = vvi * dmdl - uui * dmdt;
+    T dhu = (F.Fqux[i] + F.Fquy[i] - F.Su[iright] - F.Fquy[itop]) * cmdinv + fc * hi * vvi;
+    T dhv = (F.Fqvy[i] + F.Fqvx[i] - F.Sv[itop] - F.Fqvx[iright]) * cmdinv - fc * hi * uui;
+    dhu += hi * (ga * hi * dmdl + fG * vvi);
+    dhv += hi * (ga * hi * dmdt - fG * uui);
+    // src/Advection.cu:332-385; the drying limiter is evaluated in double
+    const T dt = corr ? T(L->dt) : T(L->dt) * T(0.5);
+    const T eps = T(P.eps);
+    const T hold = A.hold[i];
+    T edt;
+    if (P.ForceMassConserve || dhi >= T(0.0)) {
+        edt = dt;
+    } else {
+        const double lim = smax((double)hold, P.eps) / (double)fabs(dhi);
+        edt = T(smin((double)dt, lim));
+    }
+    T ho = hold + edt * dhi, uo, vo;
+    if (ho > eps) {
+        uo = (hold * A.uold[i] + edt * dhu) / ho;
+        vo = (hold * A.vold[i] + edt * dhv) / ho;
+    } else {
+        uo = T(0.0);
+        vo = T(0.0);
+    }
+    if (corr) {
+        // bottom friction, src/Friction.cu:89-148
+        if (ho > eps) {
+            const T normu = sqrt(uo * uo + vo * vo);
+            const T cfv = A.cf[i];
+            T cfi = T(0);
+            if (P.frictionmodel == 0) {
+                cfi = cfv;
+            } else if (P.frictionmodel == 1) {
+                const T ee = T(2.71828182845905);
+                const T Hbar = ho / cfv;
+                if (Hbar <= ee) cfi = T(1.0) / (T(0.46) * Hbar);
+                else cfi = T(1.0) / (T(2.5) * (log(Hbar) - T(1.0) + T(1.359) / Hbar));
+                cfi = cfi * cfi;
+            } else if (P.frictionmodel == -1) {
+                cfi = g * cfv * cfv / cbrt(ho);
+            }
+            const T tb = cfi * normu / ho * dt;
+            uo = uo / (T(1.0) + tb);
+            vo = vo / (T(1.0) + tb);
+        }
+        // cleanup then implicit uniform rain, src/Updateforcing.cu:530-581
+        if (P.rain_on) {
+            const T act = (double)A.zb[i] < P.mask ? T(1.0) : T(0.0);
+            T rainhh = T(P.rain_mmh);
+            rainhh = smax(rainhh / T(1000.0) / T(3600.0) * T(L->dt), T(0.0)) * act;
+            const T qvol = ho / (ho + rainhh);
+            const bool wet = (double)ho > P.eps;
+            ho = ho + rainhh;
+            if (wet) {
+                uo = uo * qvol;
+                vo = vo * qvol;
+            }
+        }
+    }
+    A.ho[i] = ho;
+    A.uo[i] = uo;
+    A.vo[i] = vo;
+}
+
+template <class T>
+void stage(const KParams& P, const AmrBuffers<T>& B, DevLoop* L, int corr, hipStream_t st) {
+    const T* h = corr ? B.ho : B.h;
+    const T* u = corr ? B.uo : B.u;
+    const T* v = corr ? B.vo : B.v;
+    T* hw = const_cast<T*>(h);
+    T* uw = const_cast<T*>(u);
+    T* vw = const_cast<T*>(v);
+    const GradOut<T> G{B.dhdx, B.dhdy, B.dzsdx, B.dzsdy, B.dudx, B.dudy, B.dvdx, B.dvdy};
+    const FluxOut<T> F{B.Fhu, B.Fqux, B.Su, B.Fqvx, B.Fhv, B.Fqvy, B.Sv, B.Fquy};
+    const dim3 quad((P.nblk + 3) / 4), blk(P.nblk), tpb(256);
+    hipLaunchKernelGGL(k_amr_halo<T>, quad, tpb, 0, st, P, hw, uw, vw, (const T*)B.zb, L, corr);
+    hipLaunchKernelGGL(k_amr_grad<T>, blk, tpb, 0, st, P, h, u, v, (const T*)B.zb, G, L, corr);
+    const bool prolong = P.wetdryfix && P.maxlevel > P.minlevel;
+    hipLaunchKernelGGL(k_amr_refine<T>, quad, tpb, 0, st, P, hw, uw, vw, G, prolong ? B.hsnap : (T*)nullptr, L, corr);
+    if (prolong)
+        for (int lev = P.minlevel + 1; lev <= P.maxlevel; lev++)
+            hipLaunchKernelGGL(k_amr_prolong<T>, quad, tpb, 0, st, P, hw, B.zb, (const T*)B.hsnap, lev, L, corr);
+    hipLaunchKernelGGL(k_amr_grad<T>, blk, tpb, 0, st, P, h, u, v, (const T*)B.zb, G, L, corr);
+    const StageIn<T> E{h, u, v};
+    hipLaunchKernelGGL(k_amr_flux<T>, blk, tpb, 0, st, P, E, (const T*)B.zb, G, F, L, corr);
+    hipLaunchKernelGGL(k_amr_fluxfix<T>, quad, tpb, 0, st, P, F, L, corr);
+}
+
+template <class T>
+void amr_step(const KParams& P, const AmrBuffers<T>& B, DevLoop* L, double endtime, int advance_time, hipStream_t st,
+              hipEvent_t* ev) {
+    const dim3 blk(P.nblk), tpb(256);
+    if (ev) (void)hipEventRecord(ev[0], st);
+    stage(P, B, L, 0, st);
+    if (ev) (void)hipEventRecord(ev[1], st);
+    launch_reduce_dt(sizeof(T) == 8 ? 0 : 1, L, endtime, advance_time, st);
+    if (ev) (void)hipEventRecord(ev[2], st);
+    // predictor: evolv_o = evolv + dt/2 L(evolv)
+    AdvArgs<T> a0{B.h, B.u, B.v, B.h, B.u, B.v, B.ho, B.uo, B.vo, B.zb, B.cf};
+    hipLaunchKernelGGL(k_amr_advance<T>, blk, tpb, 0, st, P, a0, FluxOut<T>{B.Fhu, B.Fqux, B.Su, B.Fqvx, B.Fhv, B.Fqvy,
+                                                                            B.Sv, B.Fquy}, L, 0);
+    // corrector: evolv = friction(evolv + dt L(evolv_o)) + rain
+    stage(P, B, L, 1, st);
+    AdvArgs<T> a1{B.ho, B.uo, B.vo, B.h, B.u, B.v, B.h, B.u, B.v, B.zb, B.cf};
+    hipLaunchKernelGGL(k_amr_advance<T>, blk, tpb, 0, st, P, a1, FluxOut<T>{B.Fhu, B.Fqux, B.Su, B.Fqvx, B.Fhv, B.Fqvy,
+                                                                            B.Sv, B.Fquy}, L, 1);
+    if (ev) (void)hipEventRecord(ev[3], st);
+}
+
+}  // namespace
+
+void launch_amr_step(int precision, const KParams& P, const void* buffers, DevLoop* L, double endtime,
+                     int advance_time, hipStream_t st, hipEvent_t* ev) {
+    if (precision == 0) amr_step<double>(P, *(const AmrBuffers<double>*)buffers, L, endtime, advance_time, st, ev);
+    else amr_step<float>(P, *(const AmrBuffers<float>*)buffers, L, endtime, advance_time, st, ev);
+}
+
+void launch_amr_fill(int precision, const KParams& P, void* z, hipStream_t st) {
+    const dim3 quad((P.nblk + 3) / 4), tpb(256);
+    if (precision == 0) hipLaunchKernelGGL(k_amr_fill<double>, quad, tpb, 0, st, P, (double*)z);
+    else hipLaunchKernelGGL(k_amr_fill<float>, quad, tpb, 0, st, P, (float*)z);
+}
+
+}  // namespace bgf

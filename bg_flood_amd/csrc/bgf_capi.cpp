@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/bgflood_amd.h"
@@ -63,6 +64,11 @@ struct bgf_handle_s {
     int pending_steps = 0;  // steps enqueued by bgf_run_async
     bool state_set = false;
     bgf::DevLoop loop_host;  // staging copy of the loop state for uploads
+    // multi-level meshes: the step runs the bgf_amr kernels on the reference
+    // layout (state stays in buffer 0, buffer 1 is evolv_o)
+    bool amr = false;
+    int* d_inv = nullptr;
+    void* d_amr[17] = {nullptr};  // 8 gradients, 8 face fluxes, hsnap
 };
 
 namespace {
@@ -93,6 +99,8 @@ void free_handle(bgf_handle h) {
     for (int k = 0; k < 2; k++) { F(h->d_h[k]); F(h->d_u[k]); F(h->d_v[k]); }
     F(h->d_zb); F(h->d_cf); F(h->d_dh); F(h->d_dhu); F(h->d_dhv);
     F(h->d_volpart); F(h->d_volout); F(h->d_loop);
+    F(h->d_inv);
+    for (int k = 0; k < 17; k++) F(h->d_amr[k]);
     for (int k = 0; k < 4; k++) if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -138,12 +146,34 @@ void fill_buffers(bgf_handle h, int in, void* out_storage) {
     }
 }
 
+template <class T> void fill_amr_t(bgf_handle h, bgf::AmrBuffers<T>* B) {
+    B->h = (T*)h->d_h[0]; B->u = (T*)h->d_u[0]; B->v = (T*)h->d_v[0];
+    B->ho = (T*)h->d_h[1]; B->uo = (T*)h->d_u[1]; B->vo = (T*)h->d_v[1];
+    B->zb = (T*)h->d_zb; B->cf = (const T*)h->d_cf;
+    T** g[8] = {&B->dhdx, &B->dhdy, &B->dzsdx, &B->dzsdy, &B->dudx, &B->dudy, &B->dvdx, &B->dvdy};
+    for (int k = 0; k < 8; k++) *g[k] = (T*)h->d_amr[k];
+    T** f[8] = {&B->Fhu, &B->Fqux, &B->Su, &B->Fqvx, &B->Fhv, &B->Fqvy, &B->Sv, &B->Fquy};
+    for (int k = 0; k < 8; k++) *f[k] = (T*)h->d_amr[8 + k];
+    B->hsnap = (T*)h->d_amr[16];
+}
+
+void fill_amr_buffers(bgf_handle h, void* storage) {
+    if (h->prec == 0) fill_amr_t<double>(h, (bgf::AmrBuffers<double>*)storage);
+    else fill_amr_t<float>(h, (bgf::AmrBuffers<float>*)storage);
+}
+
 bgf_status enqueue_steps(bgf_handle h, int nsteps, double endtime, int advance_time) {
-    alignas(16) unsigned char storage[sizeof(bgf::StepBuffers<double>)];
+    alignas(16) unsigned char storage[sizeof(bgf::AmrBuffers<double>)];
+    if (h->amr) fill_amr_buffers(h, storage);
     for (int k = 0; k < nsteps; k++) {
-        fill_buffers(h, k & 1, storage);
-        bgf::launch_step(h->prec, h->kp, storage, h->d_loop, endtime, advance_time, h->stream,
-                         h->timing ? h->ev : nullptr);
+        if (h->amr) {
+            bgf::launch_amr_step(h->prec == 0 ? 0 : 1, h->kp, storage, h->d_loop, endtime, advance_time, h->stream,
+                                 h->timing ? h->ev : nullptr);
+        } else {
+            fill_buffers(h, k & 1, storage);
+            bgf::launch_step(h->prec, h->kp, storage, h->d_loop, endtime, advance_time, h->stream,
+                             h->timing ? h->ev : nullptr);
+        }
         if (h->timing) {
             HIPCHK(hipEventSynchronize(h->ev[3]));
             float a = 0, b = 0, c = 0;
@@ -159,7 +189,7 @@ bgf_status enqueue_steps(bgf_handle h, int nsteps, double endtime, int advance_t
 
 // After a run, bring the state back to buffer 0 if it ended in buffer 1.
 bgf_status settle(bgf_handle h, int steps_done) {
-    if (steps_done & 1) {
+    if (!h->amr && (steps_done & 1)) {
         const void* src[3] = {h->d_h[1], h->d_u[1], h->d_v[1]};
         void* dst[3] = {h->d_h[0], h->d_u[0], h->d_v[0]};
         bgf::launch_copy_state(h->prec, h->kp, src, dst, h->stream);
@@ -222,20 +252,35 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(BGF_ERR_NO_DEVICE, "no HIP device");
     const int nblk = p->nblk, nbm = p->nblkmem;
-    // The one-level kernels need each side to be a boundary or one same-level neighbour.
+    // Mesh validation.  One-level meshes (every side a boundary or one
+    // same-level neighbour) run the fused kernels; others the multi-level ones,
+    // which need neighbour levels within one of each other (2:1 balance).
+    bool multilevel = false;
+    int minlev = 1 << 30, maxlev = -(1 << 30);
+    std::vector<int> inv(nbm, -1);
     for (int ibl = 0; ibl < nblk; ibl++) {
         int ib = b->active[ibl];
         if (ib < 0 || ib >= nbm) return fail(BGF_ERR_INVALID_ARG, "active block index out of range");
+        if (inv[ib] >= 0) return fail(BGF_ERR_INVALID_ARG, "block listed twice in active");
+        inv[ib] = ibl;
+        minlev = std::min(minlev, b->level[ib]);
+        maxlev = std::max(maxlev, b->level[ib]);
+    }
+    for (int ibl = 0; ibl < nblk; ibl++) {
+        int ib = b->active[ibl];
         const int* pairs[4][2] = {{b->LeftBot, b->LeftTop}, {b->RightBot, b->RightTop},
                                   {b->TopLeft, b->TopRight}, {b->BotLeft, b->BotRight}};
         for (auto& pr : pairs) {
             int n0 = pr[0][ib], n1 = pr[1][ib];
             if (n0 < 0 || n0 >= nbm || n1 < 0 || n1 >= nbm) return fail(BGF_ERR_INVALID_ARG, "neighbour out of range");
-            if (n0 != n1 || (n0 != ib && b->level[n0] != b->level[ib]))
-                return fail(BGF_ERR_UNSUPPORTED, "multi-level block lists need the AMR kernels");
+            if (inv[n0] < 0 || inv[n1] < 0) return fail(BGF_ERR_INVALID_ARG, "neighbour is not an active block");
+            for (int n : {n0, n1})
+                if (std::abs(b->level[n] - b->level[ib]) > 1)
+                    return fail(BGF_ERR_INVALID_ARG, "neighbour levels differ by more than one");
+            if (n0 != n1 || (n0 != ib && b->level[n0] != b->level[ib])) multilevel = true;
         }
-        if (b->level[ib] != b->level[b->active[0]]) return fail(BGF_ERR_UNSUPPORTED, "blocks of different levels");
     }
+    if (minlev != maxlev) multilevel = true;
     auto* h = new bgf_handle_s();
     h->p = *p;
     h->prec = p->precision;
@@ -262,6 +307,11 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
     ok = ok && A(&h->d_dh, h->es * h->S) && A(&h->d_dhu, h->es * h->S) && A(&h->d_dhv, h->es * h->S);
     ok = ok && A((void**)&h->d_volpart, sizeof(double) * nblk) && A((void**)&h->d_volout, sizeof(double));
     ok = ok && A((void**)&h->d_loop, sizeof(bgf::DevLoop));
+    h->amr = multilevel;
+    if (multilevel) {
+        ok = ok && A((void**)&h->d_inv, sizeof(int) * nbm);
+        for (int k = 0; k < 17; k++) ok = ok && A(&h->d_amr[k], h->es * h->S);
+    }
     if (!ok) {
         free_handle(h);
         (void)hipGetLastError();
@@ -273,6 +323,9 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
     cp = cp && hipMemcpy(h->d_order, order.data(), sizeof(int) * nblk, hipMemcpyHostToDevice) == hipSuccess;
     cp = cp && hipMemcpy(h->d_level, b->level, sizeof(int) * nbm, hipMemcpyHostToDevice) == hipSuccess;
     for (int k = 0; k < 8; k++) cp = cp && hipMemcpy(h->d_nb[k], nbs[k], sizeof(int) * nbm, hipMemcpyHostToDevice) == hipSuccess;
+    if (multilevel) {
+        cp = cp && hipMemcpy(h->d_inv, inv.data(), sizeof(int) * nbm, hipMemcpyHostToDevice) == hipSuccess;
+    }
     cp = cp && hipStreamCreate(&h->stream) == hipSuccess;
     for (int k = 0; k < 4; k++) cp = cp && hipEventCreate(&h->ev[k]) == hipSuccess;
     if (!cp) {
@@ -280,7 +333,9 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
         return fail(BGF_ERR_HIP, "device upload failed");
     }
     bgf::KParams& kp = h->kp;
-    kp.g = p->g; kp.eps = p->eps; kp.CFL = p->CFL; kp.theta = p->theta; kp.delta = p->delta;
+    kp.g = p->g; kp.eps = p->eps; kp.CFL = p->CFL; kp.theta = p->theta;
+    kp.delta0 = p->delta;
+    kp.delta = minlev < 0 ? p->delta * (1 << -minlev) : p->delta / (1 << minlev);  // one-level kernels
     kp.fc = p->precision == 0 ? coriolis<double>(p->lat) : coriolis<float>(p->lat);
     kp.mask = p->mask;
     kp.rain_mmh = (double)(float)p->rain;  // Rain.nowvalue is a float (src/Forcing.h)
@@ -289,6 +344,9 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
     kp.ForceMassConserve = p->ForceMassConserve;
     kp.rain_on = p->rain_on;
     kp.nblk = nblk;
+    kp.wetdryfix = p->wetdryfix;
+    kp.minlevel = minlev;
+    kp.maxlevel = maxlev;
     // BGF_WG_PER_CU="p,c" caps the persistent workgroups per CU of the
     // predictor and the corrector (a tuning override; default: occupancy).
     int wp = 0, wc = 0;
@@ -305,6 +363,12 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
     kp.RightBot = h->d_nb[2];
     kp.TopLeft = h->d_nb[4];
     kp.BotLeft = h->d_nb[6];
+    kp.level = h->d_level;
+    kp.LeftTop = h->d_nb[1];
+    kp.RightTop = h->d_nb[3];
+    kp.TopRight = h->d_nb[5];
+    kp.BotRight = h->d_nb[7];
+    kp.inv = h->d_inv;
     *out = h;
     g_err.clear();
     return BGF_OK;
@@ -325,6 +389,11 @@ bgf_status bgf_set_state(bgf_handle h, const void* zb, const void* hh, const voi
     HIPCHK(hipMemcpyAsync(h->d_u[0], u, bytes, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(h->d_v[0], v, bytes, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(h->d_cf, cf, bytes, hipMemcpyHostToDevice, h->stream));
+    if (h->amr) {
+        // halo of the bed, as InitzbHalo (src/InitialConditions.cu:135-147)
+        bgf::launch_amr_fill(h->prec == 0 ? 0 : 1, h->kp, h->d_zb, h->stream);
+        HIPCHK(hipGetLastError());
+    }
     HIPCHK(hipStreamSynchronize(h->stream));
     h->state_set = true;
     return BGF_OK;
@@ -519,6 +588,77 @@ bgf_status bgf_mesh_uniform(int nbx, int nby, double dx, int level, int dbl, int
             BotLeft[ib] = BotRight[ib] = bt;
             TopLeft[ib] = TopRight[ib] = t;
         }
+    return BGF_OK;
+}
+
+// Block list of a quadtree mesh given by its leaves.  Leaf k is block k, at
+// level lev[k] with integer position (ix[k], iy[k]) among the blocks of that
+// level; the domain is nbx x nby blocks of level `base`.  Neighbours follow
+// the convention of the reference's adaptation (src/Adaptation.cu:1017-1795):
+// a side shared with one block of the same or a coarser level points both
+// halves at it, a side shared with two finer blocks points the first half
+// (bottom or left) and the second half at them, and a side on the domain
+// edge points at the block itself.  The leaves must tile the domain exactly
+// and neighbouring levels may differ by one at most.
+bgf_status bgf_mesh_from_leaves(int nleaf, const int* lev, const int* ix, const int* iy, int nbx, int nby, int base,
+                                double dx, int* active, int* level, double* xo, double* yo, int* LeftBot,
+                                int* LeftTop, int* RightBot, int* RightTop, int* TopLeft, int* TopRight,
+                                int* BotLeft, int* BotRight) {
+    if (nleaf <= 0 || nbx <= 0 || nby <= 0 || !lev || !ix || !iy || !active || !level || !xo || !yo)
+        return fail(BGF_ERR_INVALID_ARG, "bad mesh arguments");
+    auto key = [base](int l, long long x, long long y) {
+        return ((unsigned long long)(l - base) << 58) | ((unsigned long long)x << 29) | (unsigned long long)y;
+    };
+    std::unordered_map<unsigned long long, int> at;
+    at.reserve((size_t)nleaf * 2);
+    double area = 0.0;
+    for (int k = 0; k < nleaf; k++) {
+        const int d = lev[k] - base;
+        if (d < 0 || d > 28) return fail(BGF_ERR_INVALID_ARG, "leaf level out of range");
+        const long long w = (long long)nbx << d, hgt = (long long)nby << d;
+        if (ix[k] < 0 || iy[k] < 0 || ix[k] >= w || iy[k] >= hgt) return fail(BGF_ERR_INVALID_ARG, "leaf outside the domain");
+        if (!at.emplace(key(lev[k], ix[k], iy[k]), k).second) return fail(BGF_ERR_INVALID_ARG, "duplicate leaf");
+        area += std::ldexp(1.0, -2 * d);
+    }
+    for (int k = 0; k < nleaf; k++)  // no leaf may contain another
+        for (int l = lev[k] - 1, x = ix[k] >> 1, y = iy[k] >> 1; l >= base; l--, x >>= 1, y >>= 1)
+            if (at.count(key(l, x, y))) return fail(BGF_ERR_INVALID_ARG, "overlapping leaves");
+    if (area != (double)nbx * (double)nby) return fail(BGF_ERR_INVALID_ARG, "leaves do not cover the domain");
+    auto find = [&](int l, long long x, long long y) -> int {
+        auto it = at.find(key(l, x, y));
+        return it == at.end() ? -1 : it->second;
+    };
+    // neighbour(s) of block k across the side with offset (ox, oy); (f0, f1)
+    // are the positions of the two finer blocks relative to 2*(x+ox, y+oy)
+    auto side = [&](int k, int ox, int oy, int fx0, int fy0, int fx1, int fy1, int* first, int* second) -> bool {
+        const int l = lev[k], d = l - base;
+        const long long x = (long long)ix[k] + ox, y = (long long)iy[k] + oy;
+        if (x < 0 || y < 0 || x >= ((long long)nbx << d) || y >= ((long long)nby << d)) {
+            first[k] = second[k] = k;
+            return true;
+        }
+        int n = find(l, x, y);
+        if (n < 0 && l > base) n = find(l - 1, x >> 1, y >> 1);
+        if (n >= 0) {
+            first[k] = second[k] = n;
+            return true;
+        }
+        const int a = find(l + 1, 2 * x + fx0, 2 * y + fy0), c = find(l + 1, 2 * x + fx1, 2 * y + fy1);
+        if (a < 0 || c < 0) return false;
+        first[k] = a;
+        second[k] = c;
+        return true;
+    };
+    for (int k = 0; k < nleaf; k++) {
+        active[k] = k;
+        level[k] = lev[k];
+        const double ldx = lev[k] < 0 ? dx * (1 << -lev[k]) : dx / (1 << lev[k]);
+        xo[k] = ix[k] * (double)BW * ldx + 0.5 * ldx;
+        yo[k] = iy[k] * (double)BW * ldx + 0.5 * ldx;
+        bool ok = side(k, -1, 0, 1, 0, 1, 1, LeftBot, LeftTop) && side(k, 1, 0, 0, 0, 0, 1, RightBot, RightTop) &&
+                  side(k, 0, -1, 0, 1, 1, 1, BotLeft, BotRight) && side(k, 0, 1, 0, 0, 1, 0, TopLeft, TopRight);
+        if (!ok) return fail(BGF_ERR_INVALID_ARG, "mesh is not 2:1 balanced");
+    }
     return BGF_OK;
 }
 

@@ -16,15 +16,18 @@ struct DevLoop {
     int done;        // totaltime reached the end time: later steps are no-ops
     int bad;         // a non-finite or non-positive dt was produced
     int steps_done;  // steps executed by the current run
-    int pad;
+    int go;          // set by the dt reduction: the corrector half of this step runs
     unsigned int slot32[NSLOT];
     unsigned long long slot64[NSLOT];
 };
 
 // Scalar parameters and block neighbour arrays of one domain.
 struct KParams {
-    double g, eps, CFL, theta, delta, fc, mask, rain_mmh;
-    int frictionmodel, aoibnd, ForceMassConserve, rain_on, nblk;
+    double g, eps, CFL, theta;
+    double delta;   // cell size of the one-level kernels' level
+    double delta0;  // Param.delta: cell size of level 0
+    double fc, mask, rain_mmh;
+    int frictionmodel, aoibnd, ForceMassConserve, rain_on, nblk, wetdryfix;
     int grid_pred, grid_corr;  // persistent workgroup counts (multiples of 8)
     const int* active;  // active blocks in the reference order
     const int* order;   // the same blocks in the persistent kernels' processing order
@@ -32,6 +35,14 @@ struct KParams {
     const int* RightBot;
     const int* BotLeft;
     const int* TopLeft;
+    // multi-level meshes
+    const int* level;     // block level (every mesh)
+    const int* LeftTop;
+    const int* RightTop;
+    const int* TopRight;
+    const int* BotRight;
+    const int* inv;       // position of each block in the active list (-1 if inactive)
+    int minlevel, maxlevel;  // levels present among the active blocks
 };
 
 template <class T> struct StepBuffers {
@@ -57,5 +68,24 @@ int persistent_grid(int precision, int corrector, int nblk, int wg_per_cu);
 void launch_init_dt(int precision, const KParams& P, const void* h, DevLoop* L, hipStream_t st);
 void launch_volume(int precision, const KParams& P, const void* h, double* part, double* out, hipStream_t st);
 void launch_copy_state(int precision, const KParams& P, const void* const* src, void* const* dst, hipStream_t st);
+
+// dt reduction of the block-minimum slots, clamp and output landing (one wave).
+void launch_reduce_dt(int precision, DevLoop* L, double endtime, int advance_time, hipStream_t st);
+
+// Arrays of the multi-level step (the reference's Model<T> layout).
+template <class T> struct AmrBuffers {
+    T *h, *u, *v;     // evolv
+    T *ho, *uo, *vo;  // evolv_o
+    T* zb;            // its halo is written by the halo fill and the wet/dry prolongation
+    const T* cf;
+    T *dhdx, *dhdy, *dzsdx, *dzsdy, *dudx, *dudy, *dvdx, *dvdy;
+    T *Fhu, *Fqux, *Su, *Fqvx, *Fhv, *Fqvy, *Sv, *Fquy;
+    T* hsnap;         // h before the wet/dry prolongation
+};
+// One step on a multi-level mesh (FP64 or FP32, reference arithmetic).
+void launch_amr_step(int precision, const KParams& P, const void* buffers, DevLoop* L, double endtime,
+                     int advance_time, hipStream_t st, hipEvent_t* ev);
+// Halo of a static field (the bed) on a multi-level mesh, src/InitialConditions.cu:135-147.
+void launch_amr_fill(int precision, const KParams& P, void* z, hipStream_t st);
 
 }  // namespace bgf

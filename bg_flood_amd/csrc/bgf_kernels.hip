@@ -45,13 +45,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "bgf_kernels.h"
+#include "bgf_common.h"
 
 namespace bgf {
 
-constexpr int BW = 16;
-constexpr int MW = 18;
-constexpr int BS = MW * MW;
 // LDS pitches are odd so that column accesses (halo strips, the 17th face)
 // fall into distinct banks.
 constexpr int TP = 21;  // tile: x, y in [-2, 17]
@@ -59,7 +56,6 @@ constexpr int GXP = 19; // x slopes: x in [-1, 16], y in [0, 15]
 constexpr int GYP = 17; // y slopes: x in [0, 15], y in [-1, 16]
 constexpr int FP = 17;  // X faces xf in [0, 16]; Y faces x in [0, 15]
 
-__device__ __forceinline__ int memloc(int i, int j, int ib) { return (i + 1) + (j + 1) * MW + ib * BS; }
 __device__ __forceinline__ int tix(int x, int y) { return (y + 2) * TP + (x + 2); }
 __device__ __forceinline__ int gxi(int x, int y) { return y * GXP + (x + 1); }
 __device__ __forceinline__ int gyi(int x, int y) { return (y + 1) * GYP + x; }
@@ -398,31 +394,11 @@ __device__ __forceinline__ void advance(const Cst<T>& C, int fmc, T dt, T hold, 
     }
 }
 
-template <class T> __device__ __forceinline__ T block_min(T v, T* red, int t) {
-    for (int off = 32; off > 0; off >>= 1) {
-        T o = __shfl_xor(v, off, 64);
-        v = o < v ? o : v;
-    }
-    if ((t & 63) == 0) red[t >> 6] = v;
-    __syncthreads();
-    T m = red[0];
-#pragma unroll
-    for (int k = 1; k < 4; k++) m = red[k] < m ? red[k] : m;
-    return m;
-}
 
 // Workgroups are dispatched round-robin over the eight XCDs, each with its
 // own L2: the persistent kernels give each XCD a contiguous range of blocks so
 // a block's neighbours are mostly fetched into the same L2.
 constexpr int NXCD = 8;
-
-// Positive floating-point values order like their bit patterns.
-__device__ __forceinline__ void slot_min(DevLoop* L, int k, float v) {
-    atomicMin(&L->slot32[k & (NSLOT - 1)], __float_as_uint(v));
-}
-__device__ __forceinline__ void slot_min(DevLoop* L, int k, double v) {
-    atomicMin(&L->slot64[k & (NSLOT - 1)], (unsigned long long)__double_as_longlong(v));
-}
 
 // Block indices are workgroup-uniform: read them through the constant address
 // space so they become scalar loads, which do not wait behind the vector
@@ -521,7 +497,7 @@ __global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuff
 // Persistent corrector with the same schedule and prefetch.
 template <class T, bool FAST>
 __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuffers<T> B, DevLoop* L) {
-    if (L->done) return;
+    if (!L->go) return;
     __shared__ Tile<T> s;
     const int t = threadIdx.x;
     const int x = t & 15, y = t >> 4;
@@ -629,8 +605,11 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
 // CalctimestepCPU in FP64.
 template <class T>
 __global__ __launch_bounds__(64) void k_reduce_dt(DevLoop* L, double endtime, int advance_time) {
-    if (L->done) return;
     const int t = threadIdx.x;
+    if (L->done) {
+        if (t == 0) L->go = 0;
+        return;
+    }
     const T epsi = sizeof(T) == 8 ? T(2.220446049250313e-16) : T(1.1920928955078125e-07);
     T m;
     if (sizeof(T) == 8) {
@@ -654,6 +633,7 @@ __global__ __launch_bounds__(64) void k_reduce_dt(DevLoop* L, double endtime, in
         if (ceil(rem / (double)dt) > 0.0) dt = T(rem / ceil(rem / (double)dt));
         L->dt = (double)dt;
         L->dtmax = (double)dt;
+        L->go = 1;
         if (!isfinite((double)dt) || dt <= T(0)) L->bad = 1;
         if (advance_time) {
             L->totaltime = L->totaltime + L->dt;
@@ -671,7 +651,7 @@ __global__ __launch_bounds__(256) void k_init_dt(KParams P, const T* __restrict_
     const int t = threadIdx.x;
     const int ib = P.active[blockIdx.x];
     const int i = memloc(t & 15, t >> 4, ib);
-    T d = T(P.delta) / sqrt(T(P.g) * (h[i] > T(P.eps) ? h[i] : T(P.eps)));
+    T d = calcres(T(P.delta0), P.level[ib]) / sqrt(T(P.g) * (h[i] > T(P.eps) ? h[i] : T(P.eps)));
     T m = block_min(d, red, t);
     if (t == 0) slot_min(L, blockIdx.x, m);
 }
@@ -682,7 +662,8 @@ __global__ __launch_bounds__(256) void k_volume(KParams P, const T* __restrict__
     __shared__ double red[4];
     const int t = threadIdx.x;
     const int ib = P.active[blockIdx.x];
-    double v = (double)h[memloc(t & 15, t >> 4, ib)] * P.delta * P.delta;
+    const double d = calcres(P.delta0, P.level[ib]);
+    double v = (double)h[memloc(t & 15, t >> 4, ib)] * d * d;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if ((t & 63) == 0) red[t >> 6] = v;
     __syncthreads();
@@ -774,6 +755,11 @@ void launch_init_dt(int precision, const KParams& P, const void* h, DevLoop* L, 
         hipLaunchKernelGGL(k_init_dt<float>, dim3(P.nblk), dim3(256), 0, st, P, (const float*)h, L);
         hipLaunchKernelGGL(k_reduce_dt<float>, dim3(1), dim3(64), 0, st, L, 0.0, 0);
     }
+}
+
+void launch_reduce_dt(int precision, DevLoop* L, double endtime, int advance_time, hipStream_t st) {
+    if (precision == 0) hipLaunchKernelGGL(k_reduce_dt<double>, dim3(1), dim3(64), 0, st, L, endtime, advance_time);
+    else hipLaunchKernelGGL(k_reduce_dt<float>, dim3(1), dim3(64), 0, st, L, endtime, advance_time);
 }
 
 void launch_volume(int precision, const KParams& P, const void* h, double* part, double* out, hipStream_t st) {

@@ -48,6 +48,78 @@ def cell_coords(blocks, nblkmem, xo, yo, delta, dtype):
     return X.reshape(-1), Y.reshape(-1)
 
 
+def cell_coords_levels(blocks, nblkmem, xo, yo, dx, dtype):
+    """Cell centres of a multi-level block list: x = xo + blk.xo + ix*calcres(dx, level), in T."""
+    T = dtype
+    nb = len(blocks["active"])
+    lev = np.asarray(blocks["level"][:nb])
+    d = np.where(lev < 0, dx * np.exp2(-lev.astype(np.float64)), dx / np.exp2(lev.astype(np.float64))).astype(T)
+    ix = np.arange(BW, dtype=T)
+    X = np.zeros((nblkmem, MW, MW), T)
+    Y = np.zeros((nblkmem, MW, MW), T)
+    bxo = np.asarray(blocks["xo"][:nb]).astype(T)
+    byo = np.asarray(blocks["yo"][:nb]).astype(T)
+    X[:nb, 1:MW - 1, 1:MW - 1] = (T(xo) + bxo[:, None, None]) + ix[None, None, :] * d[:, None, None]
+    Y[:nb, 1:MW - 1, 1:MW - 1] = (T(yo) + byo[:, None, None]) + ix[None, :, None] * d[:, None, None]
+    return X.reshape(-1), Y.reshape(-1)
+
+
+def quadtree_leaves(nbx, nby, maxlevel, refine):
+    """Leaves of a 2:1-balanced quadtree over nbx x nby level-0 blocks.
+
+    refine(level, ix, iy) says whether block (ix, iy) of that level is split; blocks are split
+    level by level up to maxlevel, and coarse leaves next to leaves two levels finer are split
+    until neighbouring levels differ by one at most.  Returns (level, ix, iy) tuples in Z order
+    (level-0 blocks in raster order, children bottom-left, bottom-right, top-left, top-right)."""
+    leaves = {(0, x, y) for y in range(nby) for x in range(nbx)}
+
+    def split(b):
+        l, x, y = b
+        leaves.discard(b)
+        leaves.update({(l + 1, 2 * x + i, 2 * y + j) for j in (0, 1) for i in (0, 1)})
+
+    def cover(l, x, y):
+        for lv in range(l, -1, -1):
+            b = (lv, x >> (l - lv), y >> (l - lv))
+            if b in leaves:
+                return b
+        return None
+
+    def balance():
+        changed = True
+        while changed:
+            changed = False
+            for (l, x, y) in sorted(leaves, reverse=True):
+                if l < 2 or (l, x, y) not in leaves:
+                    continue
+                for ox, oy in ((-1, 0), (1, 0), (0, -1), (0, 1)):
+                    nx_, ny_ = x + ox, y + oy
+                    if nx_ < 0 or ny_ < 0 or nx_ >= (nbx << l) or ny_ >= (nby << l):
+                        continue
+                    c = cover(l, nx_, ny_)
+                    if c is not None and c[0] < l - 1:
+                        split(c)
+                        changed = True
+
+    for l in range(maxlevel):
+        for b in sorted(leaves):
+            if b[0] == l and refine(*b):
+                split(b)
+        balance()
+
+    def zkey(b):
+        l, x, y = b
+        sh = maxlevel - l
+        X, Y = x << sh, y << sh
+        root = (Y >> maxlevel) * nbx + (X >> maxlevel)
+        code = 0
+        for k in range(maxlevel):
+            code |= ((X >> k) & 1) << (2 * k) | ((Y >> k) & 1) << (2 * k + 1)
+        return root, code
+
+    return sorted(leaves, key=zkey)
+
+
 def uniform(nx, ny, dx, precision="fp64", xo=0.0, yo=0.0, **kw):
     """Param + block list of a uniform one-level nx x ny grid (multiples of 16)."""
     assert nx % BW == 0 and ny % BW == 0
@@ -166,3 +238,46 @@ def lake_at_rest(n=64, dx=1.0, level=0.5, amp=1.0, seed=3, precision="fp64", aoi
     h[~inter.reshape(-1)] = 0
     z = np.zeros(S, T)
     return dict(param=p, blocks=blocks, nbx=nbx, nby=nby, zb=zb, h=h, u=z.copy(), v=z.copy(), cf=z.copy())
+
+
+def amr_basin(nbx=4, nby=4, dx=10.0, maxlevel=2, depth=10.0, amp=1.0, radius=None, precision="fp64", aoibnd=0,
+              island=False, rain=0.0, frictionmodel=0, cf=0.0, zsinit=0.0, refine=None):
+    """A quadtree mesh over nbx x nby level-0 blocks refined towards the centre (maxlevel levels).
+
+    Flat bed zb = -depth; optionally a conical island (peak 2 m above the still level) straddling
+    the refinement boundary so wet/dry fronts cross coarse/fine interfaces.  zs = zsinit + a
+    Gaussian hump of amplitude amp at the centre where the bed is below zsinit (cold start:
+    h = max(zs - zb, 0), u = v = 0)."""
+    from . import mesh_from_leaves
+    Lx, Ly = nbx * BW * dx, nby * BW * dx
+    cx, cy = 0.5 * Lx, 0.5 * Ly
+    if refine is None:
+        def refine(l, x, y):
+            s = BW * dx / (1 << l)
+            bx, by = (x + 0.5) * s, (y + 0.5) * s
+            return math.hypot(bx - cx, by - cy) < (0.3 if l == 0 else 0.18) * min(Lx, Ly)
+    leaves = quadtree_leaves(nbx, nby, maxlevel, refine)
+    blocks = mesh_from_leaves(leaves, nbx, nby, dx)
+    nblk = len(leaves)
+    levels = [b[0] for b in leaves]
+    p = default_param(dx=dx, delta=dx, nblk=nblk, nblkmem=nblkmem_for(nblk), precision=precision,
+                      minlevel=min(levels), maxlevel=max(levels), aoibnd=aoibnd, frictionmodel=frictionmodel,
+                      cf=cf, rain_on=1 if rain else 0, rain=rain)
+    T = np.float64 if precision == "fp64" else np.float32
+    S = p["nblkmem"] * BS
+    X, Y = cell_coords_levels(blocks, p["nblkmem"], 0.0, 0.0, dx, T)
+    zb = np.full(S, T(-depth), T)
+    if island:
+        ix_, iy_ = T(0.32 * Lx), T(0.5 * Ly)
+        r = np.sqrt((X - ix_) * (X - ix_) + (Y - iy_) * (Y - iy_))
+        zb = np.maximum(zb, (T(2.0) - r * T((depth + 2.0) / (0.2 * Lx))).astype(T)).astype(T)
+    rad = T(radius if radius is not None else 0.12 * Lx)
+    hump = T(amp) * np.exp(T(-1.0) * ((X - T(cx)) * (X - T(cx)) + (Y - T(cy)) * (Y - T(cy))) / (T(2.0) * rad * rad))
+    zs = np.maximum(T(zsinit), zb) + np.where(zb < T(zsinit), hump, T(0.0))
+    h = np.maximum(zs - zb, T(0.0)).astype(T)
+    inter = np.zeros((p["nblkmem"], MW, MW), bool)
+    inter[:nblk, 1:MW - 1, 1:MW - 1] = True
+    h[~inter.reshape(-1)] = 0
+    z = np.zeros(S, T)
+    cfa = np.full(S, T(cf), T)
+    return dict(param=p, blocks=blocks, nbx=nbx, nby=nby, zb=zb, h=h, u=z.copy(), v=z.copy(), cf=cfa, leaves=leaves)

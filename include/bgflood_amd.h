@@ -156,6 +156,19 @@ bgf_status bgf_mesh_uniform(int nbx, int nby, double dx, int level, int dbl, int
                             double* xo, double* yo, int* LeftBot, int* LeftTop, int* RightBot,
                             int* RightTop, int* TopLeft, int* TopRight, int* BotLeft, int* BotRight);
 
+/* Block list of a quadtree (multi-level) mesh from its leaves, with the
+ * neighbour convention of the reference's adaptation
+ * (src/Adaptation.cu:1017-1795; see bgf_blocks).  Leaf k becomes block k and
+ * active[k] = k: level lev[k], integer block position (ix[k], iy[k]) among
+ * the blocks of that level.  The domain is nbx x nby blocks at level `base`,
+ * dx is the level-0 cell size.  The leaves must tile the domain and
+ * neighbouring leaves may differ by one level at most (BGF_ERR_INVALID_ARG
+ * otherwise).  Output arrays have nleaf entries. */
+bgf_status bgf_mesh_from_leaves(int nleaf, const int* lev, const int* ix, const int* iy, int nbx, int nby, int base,
+                                double dx, int* active, int* level, double* xo, double* yo, int* LeftBot,
+                                int* LeftTop, int* RightBot, int* RightTop, int* TopLeft, int* TopRight,
+                                int* BotLeft, int* BotRight);
+
 #ifdef __cplusplus
 }
 #endif

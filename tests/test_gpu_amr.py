@@ -1,0 +1,126 @@
+"""Multi-level meshes on the GPU (bgf_amr kernels) against the oracle: FP64 and FP32 runs are
+bit-identical to the reference CPU step's restatement, including wet/dry fronts crossing
+coarse/fine interfaces, rain, friction and both boundary types."""
+import numpy as np
+import pytest
+
+import bg_flood_amd as bgf
+from bg_flood_amd import scenarios
+from oracle import pyoracle as po
+from tests._util import domain_for, interior, oracle_for
+
+pytestmark = pytest.mark.gpu
+
+
+def run_both(sc, nsteps, prec):
+    print("case: %s nblk=%d levels=%d..%d steps=%d" % (prec, sc["param"]["nblk"], sc["param"]["minlevel"],
+                                                        sc["param"]["maxlevel"], nsteps), flush=True)
+    d = domain_for(sc, precision=prec)
+    o = oracle_for(sc, precision="fp64" if prec == "fp64" else "fp32")
+    lg = bgf.Loop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+    lo = po.OrcLoop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+    d.init_dt(lg)
+    o.init_loop(lo)
+    assert lg.dtmax == lo.dtmax
+    for k in range(nsteps):
+        d.step(lg)
+        o.step(lo)
+        assert lg.dt == lo.dt, "step %d" % k
+        lg.totaltime += lg.dt
+        lo.totaltime += lo.dt
+    return d, o
+
+
+def assert_same(sc, d, o):
+    m = interior(sc)
+    st = d.get_state()
+    for k in ("h", "u", "v"):
+        a, b = st[k][m], o.get(k)[m]
+        assert np.array_equal(a, b), "%s: max diff %g" % (k, np.abs(a - b).max())
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("aoibnd", [0, 1])
+def test_amr_hump_exact(prec, aoibnd):
+    sc = scenarios.amr_basin(precision=prec, aoibnd=aoibnd)
+    d, o = run_both(sc, 30, prec)
+    assert_same(sc, d, o)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_amr_island_wet_dry_rain_friction_exact(prec):
+    sc = scenarios.amr_basin(precision=prec, island=True, amp=1.5, rain=80.0, frictionmodel=0, cf=0.002)
+    d, o = run_both(sc, 40, prec)
+    assert_same(sc, d, o)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_amr_three_levels_shoreline_exact(prec):
+    """Four levels with the shoreline crossing several of them: the wet/dry prolongation runs
+    one launch per level and must see the same halos as the sequential reference."""
+    L = 6 * 16 * 8.0
+
+    def refine(l, x, y):
+        s = 16 * 8.0 / (1 << l)
+        bx, by = (x + 0.5) * s, (y + 0.5) * s
+        return abs(bx - 0.45 * L) + 0.5 * abs(by - 0.5 * L) < (0.35, 0.22, 0.12)[l] * L
+
+    sc = scenarios.amr_basin(nbx=6, nby=6, dx=8.0, maxlevel=3, precision=prec, island=True, amp=2.0,
+                             refine=refine)
+    assert sc["param"]["maxlevel"] == 3
+    d, o = run_both(sc, 40, prec)
+    assert_same(sc, d, o)
+
+
+def test_amr_manning_close():
+    sc = scenarios.amr_basin(precision="fp64", island=True, rain=50.0, frictionmodel=-1, cf=0.03)
+    d, o = run_both(sc, 25, "fp64")
+    m = interior(sc)
+    st = d.get_state()
+    for k in ("h", "u", "v"):
+        np.testing.assert_allclose(st[k][m], o.get(k)[m], rtol=1e-11, atol=1e-13)
+
+
+def test_amr_run_equals_steps_and_volume():
+    sc = scenarios.amr_basin(precision="fp64", island=True)
+    a = domain_for(sc)
+    b = domain_for(sc)
+    la = bgf.Loop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+    a.init_dt(la)
+    lb = bgf.Loop(nextoutputtime=1e9, dt=0, dtmax=la.dtmax, totaltime=0)
+    for _ in range(9):
+        a.step(la)
+        la.totaltime += la.dt
+    b.run(lb, 9)
+    assert lb.totaltime == la.totaltime
+    sa, sb = a.get_state(), b.get_state()
+    for k in ("h", "u", "v"):
+        assert np.array_equal(sa[k], sb[k]), k
+    o = oracle_for(sc)
+    lo = po.OrcLoop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+    o.init_loop(lo)
+    for _ in range(9):
+        o.step(lo)
+        lo.totaltime += lo.dt
+    assert a.volume() == pytest.approx(o.volume(), rel=1e-14)
+
+
+@pytest.mark.parametrize("amr", [False, True])
+def test_run_to_endtime_runs_the_last_corrector(amr):
+    """The step that reaches endtime is a whole step: its corrector runs."""
+    sc = scenarios.amr_basin(precision="fp64") if amr else scenarios.gaussian_hump(n=48, dx=10.0, radius=50.0)
+    a = domain_for(sc)
+    b = domain_for(sc)
+    la = bgf.Loop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+    a.init_dt(la)
+    lb = bgf.Loop(nextoutputtime=1e9, dt=0, dtmax=la.dtmax, totaltime=0)
+    b.run(lb, 1000, endtime=2.0)
+    n = 0
+    while la.totaltime < 2.0:
+        a.step(la)
+        la.totaltime += la.dt
+        n += 1
+    assert lb.totaltime == la.totaltime and n > 3
+    sa, sb = a.get_state(), b.get_state()
+    for k in ("h", "u", "v"):
+        assert np.array_equal(sa[k], sb[k]), k
