@@ -100,9 +100,28 @@ def lib():
                                        ip, ip, dp, dp] + [ip] * 8
         L.bgf_mesh_from_leaves.argtypes = [ctypes.c_int, ip, ip, ip, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_double, ip, ip, dp, dp] + [ip] * 8
+        L.bgf_batch_create.argtypes = [ctypes.POINTER(BgfParam), ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+        L.bgf_batch_destroy.argtypes = [vp]
+        L.bgf_batch_add.argtypes = [vp, ctypes.POINTER(BgfParam), ctypes.POINTER(BgfBlocks)] + [vp] * 5 + [
+            ctypes.c_double, ctypes.c_double, ip]
+        L.bgf_batch_remove.argtypes = [vp, ctypes.c_int]
+        L.bgf_batch_run_async.argtypes = [vp, ctypes.c_int]
+        L.bgf_batch_sync.argtypes = [vp]
+        L.bgf_batch_loop.argtypes = [vp, ctypes.c_int, ctypes.POINTER(Loop), ip]
+        L.bgf_batch_get_state.argtypes = [vp, ctypes.c_int] + [vp] * 4
+        L.bgf_batch_volume.argtypes = [vp, ctypes.c_int, dp]
+        L.bgf_batch_update_maxdepth.argtypes = [vp]
+        L.bgf_batch_get_maxdepth.argtypes = [vp, ctypes.c_int, vp]
+        L.bgf_batch_set_gauges.argtypes = [vp, ctypes.c_int, ctypes.c_int, ip]
+        L.bgf_batch_sample_gauges.argtypes = [vp, vp, vp]
+        L.bgf_batch_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ip, ip]
         for name in ("bgf_create", "bgf_destroy", "bgf_set_state", "bgf_get_state", "bgf_init_dt", "bgf_step",
                      "bgf_run", "bgf_run_async", "bgf_sync", "bgf_volume", "bgf_device_array", "bgf_set_timing",
-                     "bgf_kernel_times", "bgf_mesh_uniform", "bgf_mesh_from_leaves"):
+                     "bgf_kernel_times", "bgf_mesh_uniform", "bgf_mesh_from_leaves", "bgf_batch_create",
+                     "bgf_batch_destroy", "bgf_batch_add", "bgf_batch_remove", "bgf_batch_run_async",
+                     "bgf_batch_sync", "bgf_batch_loop", "bgf_batch_get_state", "bgf_batch_volume",
+                     "bgf_batch_update_maxdepth", "bgf_batch_get_maxdepth", "bgf_batch_set_gauges",
+                     "bgf_batch_sample_gauges", "bgf_batch_stats"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -162,40 +181,54 @@ def default_param(**kw):
     return p
 
 
+def _precision_code(prec):
+    return PRECISIONS[prec] if isinstance(prec, str) else int(prec)
+
+
+def _param_struct(param):
+    P = BgfParam()
+    for name, _t in BgfParam._fields_:
+        if name == "precision":
+            P.precision = _precision_code(param["precision"])
+        else:
+            setattr(P, name, param[name])
+    return P
+
+
+def _blocks_struct(blocks, nbm, keep):
+    """BgfBlocks over nbm-long copies of the block arrays (kept alive in keep); unused slots
+    point at themselves."""
+    B = BgfBlocks()
+    for k in BLOCK_KEYS:
+        a = np.zeros(nbm, np.int32)
+        src = np.asarray(blocks[k], np.int32)
+        a[:len(src)] = src
+        if k != "active":
+            a[len(src):] = np.arange(len(src), nbm) if k != "level" else 0
+        keep[k] = a
+        setattr(B, k, _iptr(a))
+    for k in ("xo", "yo"):
+        a = np.zeros(nbm)
+        a[:len(blocks[k])] = blocks[k]
+        keep[k] = a
+        setattr(B, k, _dptr(a))
+    return B
+
+
 class Domain:
     """One BUQ domain resident on the GPU (a bgflood_amd handle)."""
 
     def __init__(self, param, blocks):
-        prec = param["precision"]
-        self.precision = PRECISIONS[prec] if isinstance(prec, str) else int(prec)
+        self.precision = _precision_code(param["precision"])
         self.dtype = np.float64 if self.precision == FP64 else np.float32
         self.param = dict(param)
         nblk = int(param["nblk"])
         nbm = int(param["nblkmem"])
         self.nblk, self.nblkmem = nblk, nbm
         self.size = nbm * BS
-        P = BgfParam()
-        for name, _t in BgfParam._fields_:
-            if name == "precision":
-                P.precision = self.precision
-            else:
-                setattr(P, name, param[name])
+        P = _param_struct(param)
         self._keep = {}
-        B = BgfBlocks()
-        for k in BLOCK_KEYS:
-            a = np.zeros(nbm, np.int32)
-            src = np.asarray(blocks[k], np.int32)
-            a[:len(src)] = src
-            if k != "active":
-                # unused blocks point at themselves
-                a[len(src):] = np.arange(len(src), nbm) if k != "level" else 0
-            self._keep[k] = a
-            setattr(B, k, _iptr(a))
-        for k in ("xo", "yo"):
-            a = np.zeros(nbm)
-            a[:len(blocks[k])] = blocks[k]
-            self._keep[k] = a
-            setattr(B, k, _dptr(a))
+        B = _blocks_struct(blocks, nbm, self._keep)
         self.active = np.asarray(blocks["active"][:nblk], np.int64)
         h = ctypes.c_void_p()
         _check(lib().bgf_create(ctypes.byref(P), ctypes.byref(B), ctypes.byref(h)))
@@ -263,6 +296,108 @@ class Domain:
         n = ctypes.c_int()
         _check(lib().bgf_kernel_times(self.h, 8, names, ms, ctypes.byref(n)))
         return {names[i].decode(): ms[i] for i in range(n.value)}
+
+
+class Batch:
+    """Many one-level domains advanced together on the GPU (a bgf_batch): members are added with
+    their own block list, state, rain rate and end time, and each takes steps of its own dt."""
+
+    NGAUGE = 16
+
+    def __init__(self, common, max_blocks, max_members):
+        self.precision = _precision_code(common["precision"])
+        self.dtype = np.float64 if self.precision == FP64 else np.float32
+        self.common = dict(common)
+        self.max_members = int(max_members)
+        P = _param_struct(dict(common, nblk=1, nblkmem=1))
+        h = ctypes.c_void_p()
+        _check(lib().bgf_batch_create(ctypes.byref(P), int(max_blocks), int(max_members), ctypes.byref(h)))
+        self.h = h
+        self.members = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bgf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, param, blocks, zb, h, u, v, cf, endtime, dtinit=-1.0):
+        """Adds a member; returns its id."""
+        nbm = int(param["nblkmem"])
+        keep = {}
+        P = _param_struct(param)
+        B = _blocks_struct(blocks, nbm, keep)
+        arrs = [np.ascontiguousarray(x, dtype=self.dtype) for x in (zb, h, u, v, cf)]
+        for a in arrs:
+            if a.size != nbm * BS:
+                raise BgfError("member array has %d values, expected %d" % (a.size, nbm * BS))
+        m = ctypes.c_int()
+        _check(lib().bgf_batch_add(self.h, ctypes.byref(P), ctypes.byref(B), *[a.ctypes.data for a in arrs],
+                                   float(dtinit), float(endtime), ctypes.byref(m)))
+        self.members[m.value] = dict(nblkmem=nbm, active=np.asarray(blocks["active"][:param["nblk"]]))
+        return m.value
+
+    def remove(self, m):
+        _check(lib().bgf_batch_remove(self.h, int(m)))
+        self.members.pop(m, None)
+
+    def run_async(self, nsteps):
+        _check(lib().bgf_batch_run_async(self.h, int(nsteps)))
+
+    def sync(self):
+        _check(lib().bgf_batch_sync(self.h))
+
+    def run(self, nsteps):
+        self.run_async(nsteps)
+        self.sync()
+
+    def loop(self, m):
+        L = Loop()
+        done = ctypes.c_int()
+        _check(lib().bgf_batch_loop(self.h, int(m), ctypes.byref(L), ctypes.byref(done)))
+        return L, bool(done.value)
+
+    def get_state(self, m):
+        n = self.members[m]["nblkmem"] * BS
+        out = {k: np.zeros(n, self.dtype) for k in ("zs", "h", "u", "v")}
+        _check(lib().bgf_batch_get_state(self.h, int(m), out["zs"].ctypes.data, out["h"].ctypes.data,
+                                         out["u"].ctypes.data, out["v"].ctypes.data))
+        return out
+
+    def volume(self, m):
+        v = ctypes.c_double()
+        _check(lib().bgf_batch_volume(self.h, int(m), ctypes.byref(v)))
+        return v.value
+
+    def update_maxdepth(self):
+        _check(lib().bgf_batch_update_maxdepth(self.h))
+
+    def maxdepth(self, m):
+        out = np.zeros(self.members[m]["nblkmem"] * BS, np.float32)
+        _check(lib().bgf_batch_get_maxdepth(self.h, int(m), out.ctypes.data))
+        return out
+
+    def set_gauges(self, m, cells):
+        c = np.ascontiguousarray(np.asarray(cells, np.int32))
+        _check(lib().bgf_batch_set_gauges(self.h, int(m), len(c), _iptr(c)))
+
+    def sample_gauges(self):
+        vals = np.zeros((self.max_members, self.NGAUGE), np.float32)
+        times = np.zeros(self.max_members)
+        _check(lib().bgf_batch_sample_gauges(self.h, vals.ctypes.data, times.ctypes.data))
+        return vals, times
+
+    def stats(self):
+        cu = ctypes.c_longlong()
+        nm = ctypes.c_int()
+        nb = ctypes.c_int()
+        _check(lib().bgf_batch_stats(self.h, ctypes.byref(cu), ctypes.byref(nm), ctypes.byref(nb)))
+        return dict(cell_updates=cu.value, members=nm.value, blocks=nb.value)
 
 
 def interior_mask(nblkmem, active):

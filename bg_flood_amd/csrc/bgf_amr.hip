@@ -705,7 +705,7 @@ __global__ __launch_bounds__(256) void k_amr_advance(KParams P, AdvArgs<T> A, Fl
         // cleanup then implicit uniform rain, src/Updateforcing.cu:530-581
         if (P.rain_on) {
             const T act = (double)A.zb[i] < P.mask ? T(1.0) : T(0.0);
-            T rainhh = T(P.rain_mmh);
+            T rainhh = T(L->rain);
             rainhh = smax(rainhh / T(1000.0) / T(3600.0) * T(L->dt), T(0.0)) * act;
             const T qvol = ho / (ho + rainhh);
             const bool wet = (double)ho > P.eps;
@@ -746,13 +746,12 @@ void stage(const KParams& P, const AmrBuffers<T>& B, DevLoop* L, int corr, hipSt
 }
 
 template <class T>
-void amr_step(const KParams& P, const AmrBuffers<T>& B, DevLoop* L, double endtime, int advance_time, hipStream_t st,
-              hipEvent_t* ev) {
+void amr_step(const KParams& P, const AmrBuffers<T>& B, DevLoop* L, int advance_time, hipStream_t st, hipEvent_t* ev) {
     const dim3 blk(P.nblk), tpb(256);
     if (ev) (void)hipEventRecord(ev[0], st);
     stage(P, B, L, 0, st);
     if (ev) (void)hipEventRecord(ev[1], st);
-    launch_reduce_dt(sizeof(T) == 8 ? 0 : 1, L, endtime, advance_time, st);
+    launch_reduce_dt(sizeof(T) == 8 ? 0 : 1, L, 1, advance_time, st);
     if (ev) (void)hipEventRecord(ev[2], st);
     // predictor: evolv_o = evolv + dt/2 L(evolv)
     AdvArgs<T> a0{B.h, B.u, B.v, B.h, B.u, B.v, B.ho, B.uo, B.vo, B.zb, B.cf};
@@ -768,10 +767,10 @@ void amr_step(const KParams& P, const AmrBuffers<T>& B, DevLoop* L, double endti
 
 }  // namespace
 
-void launch_amr_step(int precision, const KParams& P, const void* buffers, DevLoop* L, double endtime,
-                     int advance_time, hipStream_t st, hipEvent_t* ev) {
-    if (precision == 0) amr_step<double>(P, *(const AmrBuffers<double>*)buffers, L, endtime, advance_time, st, ev);
-    else amr_step<float>(P, *(const AmrBuffers<float>*)buffers, L, endtime, advance_time, st, ev);
+void launch_amr_step(int precision, const KParams& P, const void* buffers, DevLoop* L, int advance_time,
+                     hipStream_t st, hipEvent_t* ev) {
+    if (precision == 0) amr_step<double>(P, *(const AmrBuffers<double>*)buffers, L, advance_time, st, ev);
+    else amr_step<float>(P, *(const AmrBuffers<float>*)buffers, L, advance_time, st, ev);
 }
 
 void launch_amr_fill(int precision, const KParams& P, void* z, hipStream_t st) {

@@ -1,0 +1,514 @@
+// bgf_batch.cpp -- batches of independent one-level domains (ensembles and
+// continuous batching of solve requests) advanced by one set of launches.
+//
+// The members' blocks are packed into one block list of fixed capacity: a
+// member owns a contiguous range of block slots (its own nblkmem blocks, with
+// neighbour indices offset into the range), so one predictor / reduction /
+// corrector launch advances every member.  Each member keeps its own loop
+// state (time step, time, end time, rain rate) in one DevLoop: the predictor
+// merges block minima into the member's slots, the reduction runs one wave per
+// member, and the corrector reads the member's dt.  Members that reached their
+// end time are skipped, members can be added and removed between runs, and
+// their states stay resident in HBM.  The reference has no counterpart: it
+// runs one domain per process (src/Mainloop.cu:17-91).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "bgf_host.h"
+
+using namespace bgf_host;
+
+namespace {
+
+struct Member {
+    bool used = false;
+    int off = 0;      // first block slot of the member's range
+    int nblk = 0;     // active blocks
+    int nblkmem = 0;  // block slots
+    std::vector<int> active;  // global block ids, in the member's own active order
+    std::vector<int> order;   // global block ids, tiled processing order
+    double rain = 0.0;
+    std::vector<int> gauges;  // global cell indices
+};
+
+}  // namespace
+
+struct bgf_batch_s {
+    bgf_param common;
+    int prec = 0;
+    size_t es = 4;
+    int cap_blocks = 0, cap_members = 0;
+    size_t S = 0;
+    std::vector<Member> members;
+    std::vector<int> slot_owner;  // member of each block slot, -1 if free
+    std::vector<bgf::DevLoop> loops;  // host mirror of the device loop states
+    int *d_active = nullptr, *d_order = nullptr, *d_sid = nullptr, *d_level = nullptr;
+    int* d_nb[8] = {nullptr};
+    void *d_h[2] = {nullptr, nullptr}, *d_u[2] = {nullptr, nullptr}, *d_v[2] = {nullptr, nullptr};
+    void *d_zb = nullptr, *d_cf = nullptr, *d_dh = nullptr, *d_dhu = nullptr, *d_dhv = nullptr, *d_hmax = nullptr;
+    double *d_volpart = nullptr, *d_volout = nullptr;
+    int* d_gcell = nullptr;    // gauge cells, 16 per member slot
+    float* d_gval = nullptr;   // sampled depths
+    bgf::DevLoop* d_loop = nullptr;
+    bgf::KParams kp;
+    hipStream_t stream = nullptr;
+    int pending = 0;
+    long long cell_updates = 0;
+    int nactive_blocks = 0;
+    int wg_pred = 0, wg_corr = 0;
+};
+
+namespace {
+
+constexpr int NGAUGE = 16;
+
+void free_batch(bgf_batch b) {
+    auto F = [](void* p) { if (p) (void)hipFree(p); };
+    F(b->d_active); F(b->d_order); F(b->d_sid); F(b->d_level);
+    for (int k = 0; k < 8; k++) F(b->d_nb[k]);
+    for (int k = 0; k < 2; k++) { F(b->d_h[k]); F(b->d_u[k]); F(b->d_v[k]); }
+    F(b->d_zb); F(b->d_cf); F(b->d_dh); F(b->d_dhu); F(b->d_dhv); F(b->d_hmax);
+    F(b->d_volpart); F(b->d_volout); F(b->d_gcell); F(b->d_gval); F(b->d_loop);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+bgf::DevLoop empty_loop() {
+    bgf::DevLoop L;
+    std::memset(&L, 0, sizeof(L));
+    L.done = 1;
+    for (int k = 0; k < bgf::NSLOT; k++) {
+        L.slot32[k] = 0x7f800000u;
+        L.slot64[k] = 0x7ff0000000000000ull;
+    }
+    return L;
+}
+
+// Rebuilds the packed active list, processing order and member ids after a
+// membership change, and the launch parameters that depend on them.
+bgf_status rebuild(bgf_batch b) {
+    std::vector<int> active, order, sid(b->cap_blocks, 0);
+    for (int m = 0; m < b->cap_members; m++) {
+        const Member& M = b->members[m];
+        if (!M.used) continue;
+        active.insert(active.end(), M.active.begin(), M.active.end());
+        order.insert(order.end(), M.order.begin(), M.order.end());
+        for (int k = 0; k < M.nblkmem; k++) sid[M.off + k] = m;
+    }
+    b->nactive_blocks = (int)active.size();
+    if (!active.empty()) {
+        HIPCHK(hipMemcpy(b->d_active, active.data(), sizeof(int) * active.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b->d_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(b->d_sid, sid.data(), sizeof(int) * sid.size(), hipMemcpyHostToDevice));
+    b->kp.nblk = b->nactive_blocks;
+    if (b->nactive_blocks > 0) {
+        b->kp.grid_pred = bgf::persistent_grid(b->prec, 0, b->nactive_blocks, b->wg_pred);
+        b->kp.grid_corr = bgf::persistent_grid(b->prec, 1, b->nactive_blocks, b->wg_corr);
+        if (b->kp.grid_pred <= 0 || b->kp.grid_corr <= 0) return fail(BGF_ERR_HIP, "occupancy query failed");
+    }
+    return BGF_OK;
+}
+
+// Launch parameters restricted to one member's blocks.
+bgf::KParams member_view(bgf_batch b, int m, int aoff) {
+    bgf::KParams P = b->kp;
+    P.active = b->d_active + aoff;
+    P.nblk = b->members[m].nblk;
+    return P;
+}
+
+int active_offset(bgf_batch b, int m) {
+    int a = 0;
+    for (int k = 0; k < m; k++)
+        if (b->members[k].used) a += b->members[k].nblk;
+    return a;
+}
+
+bgf_status check_member(bgf_batch b, int m) {
+    if (!b) return fail(BGF_ERR_INVALID_ARG, "null batch");
+    if (m < 0 || m >= b->cap_members || !b->members[m].used) return fail(BGF_ERR_INVALID_ARG, "no such member");
+    if (b->pending) return fail(BGF_ERR_STATE, "bgf_batch_run_async pending: call bgf_batch_sync first");
+    return BGF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+bgf_status bgf_batch_create(const bgf_param* common, int max_blocks, int max_members, bgf_batch* out) {
+    if (!out || max_blocks <= 0 || max_members <= 0) return fail(BGF_ERR_INVALID_ARG, "bad batch arguments");
+    *out = nullptr;
+    bgf_param p = *common;
+    p.nblk = 1;
+    p.nblkmem = 1;
+    bgf_status s = check_param(&p);
+    if (s != BGF_OK) return s;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(BGF_ERR_NO_DEVICE, "no HIP device");
+    auto* b = new bgf_batch_s();
+    b->common = *common;
+    b->prec = common->precision;
+    b->es = b->prec == 0 ? sizeof(double) : sizeof(float);
+    b->cap_blocks = max_blocks;
+    b->cap_members = max_members;
+    b->S = (size_t)max_blocks * BS;
+    b->members.assign(max_members, Member());
+    b->slot_owner.assign(max_blocks, -1);
+    b->loops.assign(max_members, empty_loop());
+    auto A = [&](void** ptr, size_t bytes) -> bool {
+        if (hipMalloc(ptr, bytes) != hipSuccess) return false;
+        return hipMemset(*ptr, 0, bytes) == hipSuccess;
+    };
+    bool ok = A((void**)&b->d_active, sizeof(int) * max_blocks) && A((void**)&b->d_order, sizeof(int) * max_blocks) &&
+              A((void**)&b->d_sid, sizeof(int) * max_blocks) && A((void**)&b->d_level, sizeof(int) * max_blocks);
+    for (int k = 0; k < 8; k++) ok = ok && A((void**)&b->d_nb[k], sizeof(int) * max_blocks);
+    for (int k = 0; k < 2; k++)
+        ok = ok && A(&b->d_h[k], b->es * b->S) && A(&b->d_u[k], b->es * b->S) && A(&b->d_v[k], b->es * b->S);
+    ok = ok && A(&b->d_zb, b->es * b->S) && A(&b->d_cf, b->es * b->S) && A(&b->d_dh, b->es * b->S) &&
+         A(&b->d_dhu, b->es * b->S) && A(&b->d_dhv, b->es * b->S) && A(&b->d_hmax, sizeof(float) * b->S);
+    ok = ok && A((void**)&b->d_volpart, sizeof(double) * max_blocks) && A((void**)&b->d_volout, sizeof(double));
+    ok = ok && A((void**)&b->d_gcell, sizeof(int) * NGAUGE * max_members) &&
+         A((void**)&b->d_gval, sizeof(float) * NGAUGE * max_members);
+    ok = ok && A((void**)&b->d_loop, sizeof(bgf::DevLoop) * max_members);
+    ok = ok && hipMemset(b->d_gcell, 0xff, sizeof(int) * NGAUGE * max_members) == hipSuccess;  // no gauges: -1
+    if (!ok) {
+        free_batch(b);
+        (void)hipGetLastError();
+        return fail(BGF_ERR_OOM, "device allocation failed");
+    }
+    if (hipStreamCreate(&b->stream) != hipSuccess) {
+        free_batch(b);
+        return fail(BGF_ERR_HIP, "stream creation failed");
+    }
+    bgf::KParams& kp = b->kp;
+    kp.g = p.g; kp.eps = p.eps; kp.CFL = p.CFL; kp.theta = p.theta;
+    kp.delta0 = p.delta;
+    kp.delta = p.delta;  // members are one-level grids of level 0 (checked at add)
+    kp.fc = p.precision == 0 ? coriolis<double>(p.lat) : coriolis<float>(p.lat);
+    kp.mask = p.mask;
+    kp.frictionmodel = p.frictionmodel;
+    kp.aoibnd = p.aoibnd;
+    kp.ForceMassConserve = p.ForceMassConserve;
+    kp.rain_on = p.rain_on;
+    kp.wetdryfix = p.wetdryfix;
+    kp.active = b->d_active;
+    kp.order = b->d_order;
+    kp.LeftBot = b->d_nb[0]; kp.LeftTop = b->d_nb[1]; kp.RightBot = b->d_nb[2]; kp.RightTop = b->d_nb[3];
+    kp.TopLeft = b->d_nb[4]; kp.TopRight = b->d_nb[5]; kp.BotLeft = b->d_nb[6]; kp.BotRight = b->d_nb[7];
+    kp.level = b->d_level;
+    kp.sid = b->d_sid;
+    kp.nmember = max_members;
+    if (const char* e = std::getenv("BGF_WG_PER_CU")) std::sscanf(e, "%d,%d", &b->wg_pred, &b->wg_corr);
+    std::vector<bgf::DevLoop> L(max_members, empty_loop());
+    if (hipMemcpy(b->d_loop, L.data(), sizeof(bgf::DevLoop) * max_members, hipMemcpyHostToDevice) != hipSuccess) {
+        free_batch(b);
+        return fail(BGF_ERR_HIP, "loop upload failed");
+    }
+    *out = b;
+    g_err.clear();
+    return BGF_OK;
+}
+
+bgf_status bgf_batch_destroy(bgf_batch b) {
+    if (!b) return fail(BGF_ERR_INVALID_ARG, "null batch");
+    (void)hipStreamSynchronize(b->stream);
+    free_batch(b);
+    return BGF_OK;
+}
+
+bgf_status bgf_batch_add(bgf_batch b, const bgf_param* p, const bgf_blocks* blk, const void* zb, const void* hh,
+                         const void* u, const void* v, const void* cf, double dtinit, double endtime, int* member) {
+    if (!b || !p || !blk || !zb || !hh || !u || !v || !cf || !member) return fail(BGF_ERR_INVALID_ARG, "null argument");
+    if (b->pending) return fail(BGF_ERR_STATE, "bgf_batch_run_async pending: call bgf_batch_sync first");
+    bgf_status s = check_param(p);
+    if (s != BGF_OK) return s;
+    const bgf_param& c = b->common;
+    if (p->precision != c.precision || p->g != c.g || p->eps != c.eps || p->CFL != c.CFL || p->theta != c.theta ||
+        p->delta != c.delta || p->frictionmodel != c.frictionmodel || p->aoibnd != c.aoibnd ||
+        p->ForceMassConserve != c.ForceMassConserve || p->lat != c.lat || p->mask != c.mask || p->rain_on != c.rain_on)
+        return fail(BGF_ERR_INVALID_ARG, "member parameters must match the batch's except nblk, nblkmem and rain");
+    const int nblk = p->nblk, nbm = p->nblkmem;
+    // one-level, level-0 members only (the fused kernels)
+    std::vector<int> seen(nbm, 0);
+    for (int k = 0; k < nblk; k++) {
+        const int ib = blk->active[k];
+        if (ib < 0 || ib >= nbm || seen[ib]++) return fail(BGF_ERR_INVALID_ARG, "bad active list");
+    }
+    for (int k = 0; k < nblk; k++) {
+        const int ib = blk->active[k];
+        if (blk->level[ib] != 0) return fail(BGF_ERR_UNSUPPORTED, "batch members must be one-level grids at level 0");
+        const int* pairs[4][2] = {{blk->LeftBot, blk->LeftTop}, {blk->RightBot, blk->RightTop},
+                                  {blk->TopLeft, blk->TopRight}, {blk->BotLeft, blk->BotRight}};
+        for (auto& pr : pairs) {
+            const int n0 = pr[0][ib], n1 = pr[1][ib];
+            if (n0 < 0 || n0 >= nbm || !seen[n0] || n0 != n1 || blk->level[n0] != 0)
+                return fail(BGF_ERR_UNSUPPORTED, "batch members must be one-level grids at level 0");
+        }
+    }
+    int m = -1;
+    for (int k = 0; k < b->cap_members; k++)
+        if (!b->members[k].used) { m = k; break; }
+    if (m < 0) return fail(BGF_ERR_OOM, "batch member capacity exhausted");
+    int off = -1;  // first fit
+    for (int k = 0, run = 0; k < b->cap_blocks; k++) {
+        run = b->slot_owner[k] < 0 ? run + 1 : 0;
+        if (run == nbm) { off = k - nbm + 1; break; }
+    }
+    if (off < 0) return fail(BGF_ERR_OOM, "batch block capacity exhausted");
+    Member& M = b->members[m];
+    M = Member();
+    M.used = true;
+    M.off = off;
+    M.nblk = nblk;
+    M.nblkmem = nbm;
+    M.rain = (double)(float)p->rain;
+    for (int k = 0; k < nblk; k++) M.active.push_back(off + blk->active[k]);
+    for (int ib : processing_order(blk, nblk, p->dx)) M.order.push_back(off + ib);
+    for (int k = 0; k < nbm; k++) b->slot_owner[off + k] = m;
+    auto undo = [&](bgf_status e) {
+        for (int k = 0; k < nbm; k++) b->slot_owner[off + k] = -1;
+        b->members[m] = Member();
+        return e;
+    };
+    // block description, offset into the range
+    const int* nbs[8] = {blk->LeftBot, blk->LeftTop, blk->RightBot, blk->RightTop,
+                         blk->TopLeft, blk->TopRight, blk->BotLeft, blk->BotRight};
+    std::vector<int> tmp(nbm);
+    for (int k = 0; k < 8; k++) {
+        for (int i = 0; i < nbm; i++) tmp[i] = off + (seen[i] ? nbs[k][i] : i);
+        if (hipMemcpy(b->d_nb[k] + off, tmp.data(), sizeof(int) * nbm, hipMemcpyHostToDevice) != hipSuccess)
+            return undo(fail(BGF_ERR_HIP, "neighbour upload failed"));
+    }
+    std::fill(tmp.begin(), tmp.end(), 0);
+    if (hipMemcpy(b->d_level + off, tmp.data(), sizeof(int) * nbm, hipMemcpyHostToDevice) != hipSuccess)
+        return undo(fail(BGF_ERR_HIP, "level upload failed"));
+    const size_t bytes = b->es * (size_t)nbm * BS, o = b->es * (size_t)off * BS;
+    auto up = [&](void* dst, const void* src) {
+        return hipMemcpy((char*)dst + o, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    if (!(up(b->d_zb, zb) && up(b->d_h[0], hh) && up(b->d_u[0], u) && up(b->d_v[0], v) && up(b->d_cf, cf)))
+        return undo(fail(BGF_ERR_HIP, "state upload failed"));
+    if (hipMemset((char*)b->d_hmax + sizeof(float) * (size_t)off * BS, 0, sizeof(float) * (size_t)nbm * BS) !=
+        hipSuccess)
+        return undo(fail(BGF_ERR_HIP, "max-depth reset failed"));
+    s = rebuild(b);
+    if (s != BGF_OK) return undo(s);
+    // initial time step of the member (src/Mainloop.cu:515-554)
+    bgf::DevLoop L = empty_loop();
+    L.done = 0;
+    L.nextoutputtime = endtime;
+    L.endtime = endtime;
+    L.rain = M.rain;
+    L.dtmax = dtinit > 0 ? dtinit / 1.5 : p->delta / std::sqrt(p->g * 10.0);
+    HIPCHK(hipMemcpy(b->d_loop + m, &L, sizeof(L), hipMemcpyHostToDevice));
+    const bgf::KParams P = member_view(b, m, active_offset(b, m));
+    bgf::launch_init_dt_member(b->prec, P, b->d_h[0], b->d_loop, m, b->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&L, b->d_loop + m, sizeof(L), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    L.dtmax = L.dt;
+    L.dt = 0.0;
+    L.steps_done = 0;
+    L.done = L.totaltime >= L.endtime ? 1 : 0;
+    b->loops[m] = L;
+    *member = m;
+    return BGF_OK;
+}
+
+bgf_status bgf_batch_remove(bgf_batch b, int m) {
+    bgf_status s = check_member(b, m);
+    if (s != BGF_OK) return s;
+    Member& M = b->members[m];
+    for (int k = 0; k < M.nblkmem; k++) b->slot_owner[M.off + k] = -1;
+    M = Member();
+    b->loops[m] = empty_loop();
+    return rebuild(b);
+}
+
+bgf_status bgf_batch_run_async(bgf_batch b, int nsteps) {
+    if (!b || nsteps < 0) return fail(BGF_ERR_INVALID_ARG, "bad argument");
+    if (b->pending) return fail(BGF_ERR_STATE, "bgf_batch_run_async pending: call bgf_batch_sync first");
+    for (auto& L : b->loops) {
+        L.steps_done = 0;
+        L.go = 0;
+        for (int k = 0; k < bgf::NSLOT; k++) {
+            L.slot32[k] = 0x7f800000u;
+            L.slot64[k] = 0x7ff0000000000000ull;
+        }
+    }
+    HIPCHK(hipMemcpyAsync(b->d_loop, b->loops.data(), sizeof(bgf::DevLoop) * b->cap_members, hipMemcpyHostToDevice,
+                          b->stream));
+    if (b->nactive_blocks > 0) {
+        alignas(16) unsigned char storage[sizeof(bgf::StepBuffers<double>)];
+        for (int k = 0; k < nsteps; k++) {
+            const int in = k & 1, out = 1 - in;
+            if (b->prec == 0) {
+                auto* B = (bgf::StepBuffers<double>*)storage;
+                *B = {(const double*)b->d_h[in], (const double*)b->d_u[in], (const double*)b->d_v[in],
+                      (double*)b->d_h[out], (double*)b->d_u[out], (double*)b->d_v[out], (const double*)b->d_zb,
+                      (const double*)b->d_cf, (double*)b->d_dh, (double*)b->d_dhu, (double*)b->d_dhv};
+            } else {
+                auto* B = (bgf::StepBuffers<float>*)storage;
+                *B = {(const float*)b->d_h[in], (const float*)b->d_u[in], (const float*)b->d_v[in],
+                      (float*)b->d_h[out], (float*)b->d_u[out], (float*)b->d_v[out], (const float*)b->d_zb,
+                      (const float*)b->d_cf, (float*)b->d_dh, (float*)b->d_dhu, (float*)b->d_dhv};
+            }
+            bgf::launch_step(b->prec, b->kp, storage, b->d_loop, 1, b->stream, nullptr);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    b->pending = 1;
+    return BGF_OK;
+}
+
+bgf_status bgf_batch_sync(bgf_batch b) {
+    if (!b) return fail(BGF_ERR_INVALID_ARG, "null batch");
+    if (!b->pending) return fail(BGF_ERR_STATE, "nothing pending");
+    b->pending = 0;
+    HIPCHK(hipMemcpyAsync(b->loops.data(), b->d_loop, sizeof(bgf::DevLoop) * b->cap_members, hipMemcpyDeviceToHost,
+                          b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (b->nactive_blocks > 0) {
+        const void* src[3] = {b->d_h[1], b->d_u[1], b->d_v[1]};
+        void* dst[3] = {b->d_h[0], b->d_u[0], b->d_v[0]};
+        bgf::launch_settle(b->prec, b->kp, src, dst, b->d_loop, b->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(b->stream));
+    }
+    int bad = 0;
+    for (int m = 0; m < b->cap_members; m++) {
+        if (!b->members[m].used) continue;
+        b->cell_updates += (long long)b->loops[m].steps_done * b->members[m].nblk * BW * BW;
+        bad |= b->loops[m].bad;
+    }
+    if (bad) return fail(BGF_ERR_NONFINITE_DT, "non-finite or non-positive time step in a member");
+    return BGF_OK;
+}
+
+bgf_status bgf_batch_loop(bgf_batch b, int m, bgf_loop* loop, int* done) {
+    bgf_status s = check_member(b, m);
+    if (s != BGF_OK) return s;
+    const bgf::DevLoop& L = b->loops[m];
+    if (loop) {
+        loop->nextoutputtime = L.nextoutputtime;
+        loop->dt = L.dt;
+        loop->dtmax = L.dtmax;
+        loop->totaltime = L.totaltime;
+    }
+    if (done) *done = L.done;
+    return BGF_OK;
+}
+
+bgf_status bgf_batch_get_state(bgf_batch b, int m, void* zs, void* hh, void* u, void* v) {
+    bgf_status s = check_member(b, m);
+    if (s != BGF_OK) return s;
+    const Member& M = b->members[m];
+    const size_t n = (size_t)M.nblkmem * BS, bytes = b->es * n, o = b->es * (size_t)M.off * BS;
+    std::vector<unsigned char> t(bytes), tz;
+    auto down = [&](const void* src, void* dst) -> bool {
+        if (hipMemcpy(t.data(), (const char*)src + o, bytes, hipMemcpyDeviceToHost) != hipSuccess) return false;
+        for (int ib : M.active) {
+            const int lb = ib - M.off;
+            for (int j = 0; j < BW; j++) {
+                const size_t c = (size_t)memloc(0, j, lb) * b->es;
+                std::memcpy((char*)dst + c, t.data() + c, BW * b->es);
+            }
+        }
+        return true;
+    };
+    if (u && !down(b->d_u[0], u)) return fail(BGF_ERR_HIP, "download failed");
+    if (v && !down(b->d_v[0], v)) return fail(BGF_ERR_HIP, "download failed");
+    if (hh && !down(b->d_h[0], hh)) return fail(BGF_ERR_HIP, "download failed");
+    if (zs) {
+        if (hipMemcpy(t.data(), (const char*)b->d_h[0] + o, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(BGF_ERR_HIP, "download failed");
+        tz.resize(bytes);
+        if (hipMemcpy(tz.data(), (const char*)b->d_zb + o, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(BGF_ERR_HIP, "download failed");
+        for (int ib : M.active) {
+            const int lb = ib - M.off;
+            for (int j = 0; j < BW; j++)
+                for (int i = 0; i < BW; i++) {
+                    const size_t c = memloc(i, j, lb);
+                    if (b->prec == 0) ((double*)zs)[c] = ((double*)tz.data())[c] + ((double*)t.data())[c];
+                    else ((float*)zs)[c] = ((float*)tz.data())[c] + ((float*)t.data())[c];
+                }
+        }
+    }
+    return BGF_OK;
+}
+
+bgf_status bgf_batch_volume(bgf_batch b, int m, double* volume) {
+    bgf_status s = check_member(b, m);
+    if (s != BGF_OK || !volume) return s != BGF_OK ? s : fail(BGF_ERR_INVALID_ARG, "null argument");
+    const bgf::KParams P = member_view(b, m, active_offset(b, m));
+    bgf::launch_volume(b->prec, P, b->d_h[0], b->d_volpart, b->d_volout, b->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(volume, b->d_volout, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return BGF_OK;
+}
+
+bgf_status bgf_batch_update_maxdepth(bgf_batch b) {
+    if (!b) return fail(BGF_ERR_INVALID_ARG, "null batch");
+    if (b->pending) return fail(BGF_ERR_STATE, "bgf_batch_run_async pending: call bgf_batch_sync first");
+    if (b->nactive_blocks > 0) {
+        bgf::launch_maxdepth(b->prec, b->kp, b->d_h[0], (float*)b->d_hmax, b->stream);
+        HIPCHK(hipGetLastError());
+    }
+    return BGF_OK;
+}
+
+bgf_status bgf_batch_get_maxdepth(bgf_batch b, int m, float* hmax) {
+    bgf_status s = check_member(b, m);
+    if (s != BGF_OK || !hmax) return s != BGF_OK ? s : fail(BGF_ERR_INVALID_ARG, "null argument");
+    const Member& M = b->members[m];
+    HIPCHK(hipMemcpyAsync(hmax, (const float*)b->d_hmax + (size_t)M.off * BS, sizeof(float) * (size_t)M.nblkmem * BS,
+                          hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return BGF_OK;
+}
+
+bgf_status bgf_batch_set_gauges(bgf_batch b, int m, int n, const int* cells) {
+    bgf_status s = check_member(b, m);
+    if (s != BGF_OK) return s;
+    Member& M = b->members[m];
+    if (n < 0 || n > NGAUGE || (n > 0 && !cells)) return fail(BGF_ERR_INVALID_ARG, "at most 16 gauges");
+    std::vector<int> g(NGAUGE, -1);
+    M.gauges.clear();
+    for (int k = 0; k < n; k++) {
+        if (cells[k] < 0 || cells[k] >= M.nblkmem * BS) return fail(BGF_ERR_INVALID_ARG, "gauge cell out of range");
+        g[k] = M.off * BS + cells[k];
+        M.gauges.push_back(g[k]);
+    }
+    HIPCHK(hipMemcpy(b->d_gcell + NGAUGE * m, g.data(), sizeof(int) * NGAUGE, hipMemcpyHostToDevice));
+    return BGF_OK;
+}
+
+bgf_status bgf_batch_sample_gauges(bgf_batch b, float* values, double* times) {
+    if (!b || !values) return fail(BGF_ERR_INVALID_ARG, "null argument");
+    if (b->pending) return fail(BGF_ERR_STATE, "bgf_batch_run_async pending: call bgf_batch_sync first");
+    bgf::launch_gauges(b->prec, b->d_h[0], b->d_gcell, b->d_gval, NGAUGE * b->cap_members, b->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(values, b->d_gval, sizeof(float) * NGAUGE * b->cap_members, hipMemcpyDeviceToHost,
+                          b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (times)
+        for (int m = 0; m < b->cap_members; m++) times[m] = b->loops[m].totaltime;
+    return BGF_OK;
+}
+
+bgf_status bgf_batch_stats(bgf_batch b, long long* cell_updates, int* nmembers, int* nblocks) {
+    if (!b) return fail(BGF_ERR_INVALID_ARG, "null batch");
+    int n = 0;
+    for (auto& M : b->members) n += M.used ? 1 : 0;
+    if (cell_updates) *cell_updates = b->cell_updates;
+    if (nmembers) *nmembers = n;
+    if (nblocks) *nblocks = b->nactive_blocks;
+    return BGF_OK;
+}
+
+}  // extern "C"

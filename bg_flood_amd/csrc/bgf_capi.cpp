@@ -17,30 +17,9 @@
 #include <vector>
 
 #include "../../include/bgflood_amd.h"
-#include "bgf_kernels.h"
+#include "bgf_host.h"
 
-namespace {
-
-constexpr int BW = 16;
-constexpr int MW = 18;
-constexpr int BS = MW * MW;
-
-thread_local std::string g_err;
-
-bgf_status fail(bgf_status s, const std::string& msg) {
-    g_err = msg;
-    return s;
-}
-
-#define HIPCHK(call)                                                                                   \
-    do {                                                                                               \
-        hipError_t e_ = (call);                                                                        \
-        if (e_ != hipSuccess) return fail(BGF_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-inline int memloc(int i, int j, int ib) { return (i + 1) + (j + 1) * MW + ib * BS; }
-
-}  // namespace
+using namespace bgf_host;
 
 struct bgf_handle_s {
     bgf_param p;
@@ -73,25 +52,6 @@ struct bgf_handle_s {
 
 namespace {
 
-bgf_status check_param(const bgf_param* p) {
-    if (!p) return fail(BGF_ERR_INVALID_ARG, "null param");
-    if (p->engine != 2) return fail(BGF_ERR_UNSUPPORTED, "only engine 2 (Kurganov) is provided");
-    if (p->spherical) return fail(BGF_ERR_UNSUPPORTED, "spherical grids are not provided");
-    if (p->nblk <= 0 || p->nblkmem < p->nblk) return fail(BGF_ERR_INVALID_ARG, "need 0 < nblk <= nblkmem");
-    if (p->precision < 0 || p->precision > 2) return fail(BGF_ERR_INVALID_ARG, "bad precision");
-    if (p->aoibnd != 0 && p->aoibnd != 1)
-        return fail(BGF_ERR_UNSUPPORTED, "remainder boundary type must be 0 (wall) or 1 (Neumann)");
-    if (p->frictionmodel < -1 || p->frictionmodel > 1) return fail(BGF_ERR_INVALID_ARG, "bad frictionmodel");
-    if (!(p->delta > 0.0) || !(p->g > 0.0) || !(p->eps > 0.0)) return fail(BGF_ERR_INVALID_ARG, "bad g/eps/delta");
-    return BGF_OK;
-}
-
-template <class T> double coriolis(double lat) {
-    // src/Advection.cu:187 (Cartesian): sin(T(lat*pi/180))*pi/T(21600) with pi = 3.14159265
-    const double pi = 3.14159265;
-    return (double)T(std::sin(T(lat * pi / 180.0)) * pi / T(21600.0));
-}
-
 void free_handle(bgf_handle h) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
     F(h->d_active); F(h->d_order); F(h->d_level);
@@ -113,6 +73,8 @@ bgf_status upload_loop(bgf_handle h, const bgf_loop* loop, double endtime) {
     L.dt = loop->dt;
     L.dtmax = loop->dtmax;
     L.totaltime = loop->totaltime;
+    L.endtime = endtime;
+    L.rain = (double)(float)h->p.rain;  // Rain.nowvalue is a float (src/Forcing.h)
     L.done = loop->totaltime >= endtime ? 1 : 0;
     for (int k = 0; k < bgf::NSLOT; k++) {
         L.slot32[k] = 0x7f800000u;
@@ -162,16 +124,16 @@ void fill_amr_buffers(bgf_handle h, void* storage) {
     else fill_amr_t<float>(h, (bgf::AmrBuffers<float>*)storage);
 }
 
-bgf_status enqueue_steps(bgf_handle h, int nsteps, double endtime, int advance_time) {
+bgf_status enqueue_steps(bgf_handle h, int nsteps, int advance_time) {
     alignas(16) unsigned char storage[sizeof(bgf::AmrBuffers<double>)];
     if (h->amr) fill_amr_buffers(h, storage);
     for (int k = 0; k < nsteps; k++) {
         if (h->amr) {
-            bgf::launch_amr_step(h->prec == 0 ? 0 : 1, h->kp, storage, h->d_loop, endtime, advance_time, h->stream,
+            bgf::launch_amr_step(h->prec == 0 ? 0 : 1, h->kp, storage, h->d_loop, advance_time, h->stream,
                                  h->timing ? h->ev : nullptr);
         } else {
             fill_buffers(h, k & 1, storage);
-            bgf::launch_step(h->prec, h->kp, storage, h->d_loop, endtime, advance_time, h->stream,
+            bgf::launch_step(h->prec, h->kp, storage, h->d_loop, advance_time, h->stream,
                              h->timing ? h->ev : nullptr);
         }
         if (h->timing) {
@@ -192,42 +154,11 @@ bgf_status settle(bgf_handle h, int steps_done) {
     if (!h->amr && (steps_done & 1)) {
         const void* src[3] = {h->d_h[1], h->d_u[1], h->d_v[1]};
         void* dst[3] = {h->d_h[0], h->d_u[0], h->d_v[0]};
-        bgf::launch_copy_state(h->prec, h->kp, src, dst, h->stream);
+        bgf::launch_settle(h->prec, h->kp, src, dst, h->d_loop, h->stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(h->stream));
     }
     return BGF_OK;
-}
-
-// Processing order of the persistent kernels: the active blocks grouped in
-// tiles of TILE_X x TILE_Y blocks (row-major tiles, row-major blocks inside a
-// tile), by their integer block coordinates.  Each XCD processes consecutive
-// positions of this order, so the blocks it has in flight at once form a
-// compact patch whose neighbours are mostly in its own L2.
-constexpr int TILE_X = 16, TILE_Y = 12;
-std::vector<int> processing_order(const bgf_blocks* b, int nblk, double dx) {
-    std::vector<int> order(b->active, b->active + nblk);
-    std::vector<long long> key(nblk);
-    long long maxbx = 1;
-    std::vector<long long> bx(nblk), by(nblk);
-    for (int k = 0; k < nblk; k++) {
-        const int ib = b->active[k];
-        const int lev = b->level[ib];
-        const double levdx = lev < 0 ? dx * (1 << -lev) : dx / (1 << lev);
-        bx[k] = (long long)std::floor(b->xo[ib] / (BW * levdx));
-        by[k] = (long long)std::floor(b->yo[ib] / (BW * levdx));
-        if (bx[k] + 1 > maxbx) maxbx = bx[k] + 1;
-    }
-    const long long ntx = (maxbx + TILE_X - 1) / TILE_X;
-    for (int k = 0; k < nblk; k++) {
-        const long long tx = bx[k] / TILE_X, ty = by[k] / TILE_Y;
-        key[k] = ((ty * ntx + tx) * TILE_Y + by[k] % TILE_Y) * TILE_X + bx[k] % TILE_X;
-    }
-    std::vector<int> idx(nblk);
-    for (int k = 0; k < nblk; k++) idx[k] = k;
-    std::stable_sort(idx.begin(), idx.end(), [&](int a, int c) { return key[a] < key[c]; });
-    for (int k = 0; k < nblk; k++) order[k] = b->active[idx[k]];
-    return order;
 }
 
 }  // namespace
@@ -460,7 +391,7 @@ bgf_status bgf_step(bgf_handle h, bgf_loop* loop) {
     if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
     bgf_status s = upload_loop(h, loop, 1e300);
     if (s != BGF_OK) return s;
-    s = enqueue_steps(h, 1, 1e300, 0);
+    s = enqueue_steps(h, 1, 0);
     if (s != BGF_OK) return s;
     bgf::DevLoop D;
     s = download_loop(h, &D);
@@ -479,7 +410,7 @@ bgf_status bgf_run_async(bgf_handle h, const bgf_loop* loop, int nsteps, double 
     if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
     bgf_status s = upload_loop(h, loop, endtime);
     if (s != BGF_OK) return s;
-    s = enqueue_steps(h, nsteps, endtime, 1);
+    s = enqueue_steps(h, nsteps, 1);
     if (s != BGF_OK) return s;
     h->pending_steps = nsteps > 0 ? nsteps : -1;
     return BGF_OK;

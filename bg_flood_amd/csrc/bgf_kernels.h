@@ -13,9 +13,11 @@ struct DevLoop {
     double dt;
     double dtmax;
     double totaltime;
+    double endtime;  // the run stops once totaltime reaches it
+    double rain;     // uniform rain rate, mm/h (Rain.nowvalue)
     int done;        // totaltime reached the end time: later steps are no-ops
     int bad;         // a non-finite or non-positive dt was produced
-    int steps_done;  // steps executed by the current run
+    int steps_done;  // steps executed since the loop was uploaded
     int go;          // set by the dt reduction: the corrector half of this step runs
     unsigned int slot32[NSLOT];
     unsigned long long slot64[NSLOT];
@@ -43,6 +45,9 @@ struct KParams {
     const int* BotRight;
     const int* inv;       // position of each block in the active list (-1 if inactive)
     int minlevel, maxlevel;  // levels present among the active blocks
+    // batches of domains: member of each block (DevLoop of that member), member count
+    const int* sid;
+    int nmember;
 };
 
 template <class T> struct StepBuffers {
@@ -60,17 +65,26 @@ template <class T> struct StepBuffers {
 };
 
 // ev (optional): four events recorded before the predictor, after it, after
-// the reduction and after the corrector.
-void launch_step(int precision, const KParams& P, const void* buffers, DevLoop* L, double endtime, int advance_time,
+// the reduction and after the corrector.  With P.sid set, the block list packs
+// P.nmember domains and L holds one loop state per member.
+void launch_step(int precision, const KParams& P, const void* buffers, DevLoop* L, int advance_time,
                  hipStream_t st, hipEvent_t* ev);
 // Workgroups of a persistent launch; wg_per_cu > 0 caps the resident workgroups per CU.
 int persistent_grid(int precision, int corrector, int nblk, int wg_per_cu);
 void launch_init_dt(int precision, const KParams& P, const void* h, DevLoop* L, hipStream_t st);
+// Initial dt of one member of a batch: P restricted to the member's blocks.
+void launch_init_dt_member(int precision, const KParams& P, const void* h, DevLoop* L, int member, hipStream_t st);
+// hmax = max(hmax, h) over the interior cells of P's blocks.
+void launch_maxdepth(int precision, const KParams& P, const void* h, float* hmax, hipStream_t st);
+// val[k] = h[cell[k]] (NaN where cell[k] < 0), k < n.
+void launch_gauges(int precision, const void* h, const int* cell, float* val, int n, hipStream_t st);
 void launch_volume(int precision, const KParams& P, const void* h, double* part, double* out, hipStream_t st);
-void launch_copy_state(int precision, const KParams& P, const void* const* src, void* const* dst, hipStream_t st);
+// Copies h, u, v from src to dst for the blocks whose member ran an odd number of steps.
+void launch_settle(int precision, const KParams& P, const void* const* src, void* const* dst, const DevLoop* L,
+                   hipStream_t st);
 
 // dt reduction of the block-minimum slots, clamp and output landing (one wave).
-void launch_reduce_dt(int precision, DevLoop* L, double endtime, int advance_time, hipStream_t st);
+void launch_reduce_dt(int precision, DevLoop* L, int nloop, int advance_time, hipStream_t st);
 
 // Arrays of the multi-level step (the reference's Model<T> layout).
 template <class T> struct AmrBuffers {
@@ -83,8 +97,8 @@ template <class T> struct AmrBuffers {
     T* hsnap;         // h before the wet/dry prolongation
 };
 // One step on a multi-level mesh (FP64 or FP32, reference arithmetic).
-void launch_amr_step(int precision, const KParams& P, const void* buffers, DevLoop* L, double endtime,
-                     int advance_time, hipStream_t st, hipEvent_t* ev);
+void launch_amr_step(int precision, const KParams& P, const void* buffers, DevLoop* L, int advance_time,
+                     hipStream_t st, hipEvent_t* ev);
 // Halo of a static field (the bed) on a multi-level mesh, src/InitialConditions.cu:135-147.
 void launch_amr_fill(int precision, const KParams& P, void* z, hipStream_t st);
 

@@ -104,7 +104,7 @@ template <class T> __device__ __forceinline__ T minmod2c(T th, T half, T s0, T s
 // Constants of the step in T, computed once on the host (make_cst) and
 // passed by value, so kernels read them from scalar registers.
 template <class T> struct Cst {
-    T g, ga, CFL, theta, epsi, inv_epsi, eps_w, eps, delta, dxh, cmdinv, fc, rainrate, cfl_delta, half;
+    T g, ga, CFL, theta, epsi, inv_epsi, eps_w, eps, delta, dxh, cmdinv, fc, cfl_delta, half;
     T th_s, half_s;  // theta/2 and 1/4: FAST half-cell slope coefficients
     T mask, rain_step_scale;
     double eps_d, mask_d;
@@ -125,7 +125,6 @@ template <class T, bool FAST> Cst<T> make_cst(const KParams& P) {
     c.dxh = c.delta * T(0.5);
     c.cmdinv = T(1.0) / (T(1.0) * c.delta);
     c.fc = T(P.fc);
-    c.rainrate = T(P.rain_mmh);
     c.cfl_delta = c.CFL * c.delta;
     c.half = T(0.5);
     c.th_s = c.theta * T(0.5);
@@ -407,16 +406,27 @@ __device__ __forceinline__ int ldu(const int* p, int i) {
     return ((const __attribute__((address_space(4))) int*)p)[i];
 }
 struct Nb {
-    int ib, nL, nR, nB, nT;
+    int ib, nL, nR, nB, nT, sid;
 };
-__device__ __forceinline__ Nb block_nb(const KParams& P, int bl) {
+template <bool BATCH> __device__ __forceinline__ Nb block_nb(const KParams& P, int bl) {
     Nb n;
     n.ib = ldu(P.order, bl);
+    n.sid = BATCH ? ldu(P.sid, n.ib) : 0;
     n.nL = ldu(P.LeftBot, n.ib);
     n.nR = ldu(P.RightBot, n.ib);
     n.nB = ldu(P.BotLeft, n.ib);
     n.nT = ldu(P.TopLeft, n.ib);
     return n;
+}
+
+// Merges a member's partial minimum into its slots: one atomic per wave, no
+// workgroup barrier (members change often along a workgroup's blocks).
+template <class T> __device__ __forceinline__ void wave_flush(DevLoop* L, T v, int t) {
+    for (int off = 32; off > 0; off >>= 1) {
+        T o = __shfl_xor(v, off, 64);
+        v = o < v ? o : v;
+    }
+    if ((t & 63) == 0) slot_min(L, (blockIdx.x << 2) + (t >> 6), v);
 }
 
 // Block sequence of persistent workgroup w: the contiguous active-list range
@@ -439,10 +449,12 @@ __device__ __forceinline__ Sched schedule(int w, int nwg, int n) {
 // Persistent predictor: each workgroup walks its block sequence, prefetching
 // the next block's cells into registers while the current block is computed
 // from the LDS tile, so the gathers of one block overlap the arithmetic of the
-// previous one.  One atomic min per workgroup.
-template <class T, bool FAST>
+// previous one.  One atomic min per workgroup (per member and workgroup in a
+// batch, whose processing order keeps each member's blocks together; blocks
+// of members that reached their end time are skipped).
+template <class T, bool FAST, bool BATCH>
 __global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuffers<T> B, DevLoop* L) {
-    if (L->done) return;
+    if (!BATCH && L->done) return;
     __shared__ Tile<T> s;
     const int t = threadIdx.x;
     const int x = t & 15, y = t >> 4;
@@ -450,6 +462,7 @@ __global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuff
     const bool wall = P.aoibnd == 0;
     const int cs = strip_cell(t);
     T bmin = C.inv_epsi;
+    int csid = -1;
     T ph = T(0), pz = T(0), pu = T(0), pv = T(0), sh = T(0), sz = T(0), su = T(0), sv = T(0);
     auto fetch = [&](const Nb& n) {
         const int i = memloc(x, y, n.ib);
@@ -461,7 +474,7 @@ __global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuff
     };
     Nb cur;
     if (S.first < S.end) {
-        cur = block_nb(P, S.first);
+        cur = block_nb<BATCH>(P, S.first);
         fetch(cur);
     }
     for (int ibl = S.first; ibl < S.end; ibl += S.stride) {
@@ -478,8 +491,16 @@ __global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuff
         __syncthreads();
         const Nb n = cur;
         if (ibl + S.stride < S.end) {
-            cur = block_nb(P, ibl + S.stride);
+            cur = block_nb<BATCH>(P, ibl + S.stride);
             fetch(cur);
+        }
+        if (BATCH) {
+            if (n.sid != csid) {
+                if (csid >= 0) wave_flush(L + csid, bmin, t);
+                csid = n.sid;
+                bmin = C.inv_epsi;
+            }
+            if (ldu(&L[n.sid].done, 0)) continue;
         }
         T dh, dhu, dhv;
         T dtc = tile_fluxes<T, FAST, true>(s, C, t, wall && n.nL == n.ib, wall && n.nR == n.ib, wall && n.nB == n.ib,
@@ -490,22 +511,24 @@ __global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuff
         B.dhv[i] = dhv;
         bmin = dtc < bmin ? dtc : bmin;
     }
+    if (BATCH) {
+        if (csid >= 0) wave_flush(L + csid, bmin, t);
+        return;
+    }
     T m = block_min(bmin, s.red, t);
     if (t == 0) slot_min(L, blockIdx.x, m);
 }
 
 // Persistent corrector with the same schedule and prefetch.
-template <class T, bool FAST>
+template <class T, bool FAST, bool BATCH>
 __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuffers<T> B, DevLoop* L) {
-    if (!L->go) return;
+    if (!BATCH && !L->go) return;
     __shared__ Tile<T> s;
     const int t = threadIdx.x;
     const int x = t & 15, y = t >> 4;
     const Sched S = schedule(blockIdx.x, gridDim.x, P.nblk);
     const bool wall = P.aoibnd == 0;
     const int cs = strip_cell(t);
-    const T dt = T(L->dt);
-    const T hdt = dt * T(0.5);
     const int fmc = P.ForceMassConserve;
     T ph = T(0), pu = T(0), pv = T(0), pdh = T(0), pdhu = T(0), pdhv = T(0), pz = T(0), pcf = T(0);
     T sh = T(0), su = T(0), sv = T(0), sdh = T(0), sdhu = T(0), sdhv = T(0), sz = T(0);
@@ -521,10 +544,20 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
     };
     Nb cur;
     if (S.first < S.end) {
-        cur = block_nb(P, S.first);
+        cur = block_nb<BATCH>(P, S.first);
         fetch(cur);
     }
     for (int ibl = S.first; ibl < S.end; ibl += S.stride) {
+        const DevLoop* Lm = L + cur.sid;
+        if (BATCH && !ldu(&Lm->go, 0)) {
+            if (ibl + S.stride < S.end) {
+                cur = block_nb<BATCH>(P, ibl + S.stride);
+                fetch(cur);
+            }
+            continue;
+        }
+        const T dt = T(Lm->dt);
+        const T hdt = dt * T(0.5);
         // this block's U^n (kept for the full-step advance) and its U* tile
         const T h0 = ph, u0 = pu, v0 = pv, zb0 = pz, cf0 = pcf;
         // Consume the prefetched values here, where the loads are waited for
@@ -546,7 +579,7 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
         __syncthreads();
         const Nb n = cur;
         if (ibl + S.stride < S.end) {
-            cur = block_nb(P, ibl + S.stride);
+            cur = block_nb<BATCH>(P, ibl + S.stride);
             fetch(cur);
         }
         const int ib = n.ib, nL = n.nL, nR = n.nR, nB = n.nB, nT = n.nT;
@@ -584,7 +617,7 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
         // implicit uniform rain after cleanup, src/Updateforcing.cu:530-581
         if (P.rain_on) {
             T act = (double)zb0 < C.mask_d ? T(1.0) : T(0.0);
-            T rainhh = dmax(C.rainrate / T(1000.0) / T(3600.0) * dt, T(0.0)) * act;
+            T rainhh = dmax(T(Lm->rain) / T(1000.0) / T(3600.0) * dt, T(0.0)) * act;
             T qvol = FAST ? h1 * Ar<T, FAST>::rcp(h1 + rainhh) : h1 / (h1 + rainhh);
             bool wetold = (double)h1 > C.eps_d;
             h1 = h1 + rainhh;
@@ -600,11 +633,13 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
     }
 }
 
-// One wave: min of the 64 block-minimum slots and 1/epsi (slots reset for
-// the next step), then the clamp and the output-time landing of
-// CalctimestepCPU in FP64.
+// One wave per loop state: min of the 64 block-minimum slots and 1/epsi
+// (slots reset for the next step), then the clamp and the output-time landing
+// of CalctimestepCPU in FP64.  mode 0: a step that leaves totaltime to the
+// caller; 1: a step followed by totaltime += dt; 2: the initial dt only.
 template <class T>
-__global__ __launch_bounds__(64) void k_reduce_dt(DevLoop* L, double endtime, int advance_time) {
+__global__ __launch_bounds__(64) void k_reduce_dt(DevLoop* L, int mode) {
+    L += blockIdx.x;
     const int t = threadIdx.x;
     if (L->done) {
         if (t == 0) L->go = 0;
@@ -633,12 +668,14 @@ __global__ __launch_bounds__(64) void k_reduce_dt(DevLoop* L, double endtime, in
         if (ceil(rem / (double)dt) > 0.0) dt = T(rem / ceil(rem / (double)dt));
         L->dt = (double)dt;
         L->dtmax = (double)dt;
-        L->go = 1;
         if (!isfinite((double)dt) || dt <= T(0)) L->bad = 1;
-        if (advance_time) {
-            L->totaltime = L->totaltime + L->dt;
+        if (mode != 2) {
+            L->go = 1;
             L->steps_done = L->steps_done + 1;
-            if (L->totaltime >= endtime) L->done = 1;
+        }
+        if (mode == 1) {
+            L->totaltime = L->totaltime + L->dt;
+            if (L->totaltime >= L->endtime) L->done = 1;
         }
     }
 }
@@ -653,7 +690,21 @@ __global__ __launch_bounds__(256) void k_init_dt(KParams P, const T* __restrict_
     const int i = memloc(t & 15, t >> 4, ib);
     T d = calcres(T(P.delta0), P.level[ib]) / sqrt(T(P.g) * (h[i] > T(P.eps) ? h[i] : T(P.eps)));
     T m = block_min(d, red, t);
-    if (t == 0) slot_min(L, blockIdx.x, m);
+    if (t == 0) slot_min(L + (P.sid ? P.sid[ib] : 0), blockIdx.x, m);
+}
+
+// Running maximum depth of the interior cells (output path of src/Meanmax.cu:15-120).
+template <class T>
+__global__ __launch_bounds__(256) void k_maxdepth(KParams P, const T* __restrict__ h, float* __restrict__ hmax) {
+    const int t = threadIdx.x;
+    const int i = memloc(t & 15, t >> 4, P.active[blockIdx.x]);
+    const float v = (float)h[i];
+    if (v > hmax[i]) hmax[i] = v;
+}
+
+template <class T> __global__ void k_gauges(const T* __restrict__ h, const int* __restrict__ cell, float* val, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) val[k] = cell[k] >= 0 ? (float)h[cell[k]] : __builtin_nanf("");
 }
 
 // Block partial sums of h*delta^2 in FP64 (deterministic two-level sum).
@@ -685,13 +736,15 @@ __global__ __launch_bounds__(1024) void k_sum(const double* __restrict__ part, i
     }
 }
 
-// cleanupGPU semantics (src/Advection.cu:399-417): copy interior h, u, v.
+// cleanupGPU semantics (src/Advection.cu:399-417): copy interior h, u, v of
+// the blocks whose state ended in the second buffer (odd step count).
 template <class T>
-__global__ __launch_bounds__(256) void k_copy_state(KParams P, const T* __restrict__ h, const T* __restrict__ u,
-                                                    const T* __restrict__ v, T* __restrict__ ho, T* __restrict__ uo,
-                                                    T* __restrict__ vo) {
+__global__ __launch_bounds__(256) void k_settle(KParams P, const T* __restrict__ h, const T* __restrict__ u,
+                                                const T* __restrict__ v, T* __restrict__ ho, T* __restrict__ uo,
+                                                T* __restrict__ vo, const DevLoop* L) {
     const int t = threadIdx.x;
     const int ib = P.active[blockIdx.x];
+    if (!(L[P.sid ? P.sid[ib] : 0].steps_done & 1)) return;
     const int i = memloc(t & 15, t >> 4, ib);
     ho[i] = h[i];
     uo[i] = u[i];
@@ -699,27 +752,34 @@ __global__ __launch_bounds__(256) void k_copy_state(KParams P, const T* __restri
 }
 
 // ------------------------------------------------------------------ launch
-template <class T, bool FAST>
-void launch_step_t(const KParams& P, const StepBuffers<T>& B, DevLoop* L, double endtime, int advance_time,
-                   hipStream_t st, hipEvent_t* ev) {
-    if (ev) hipEventRecord(ev[0], st);
+template <class T, bool FAST, bool BATCH>
+void launch_step_t(const KParams& P, const StepBuffers<T>& B, DevLoop* L, int advance_time, hipStream_t st,
+                   hipEvent_t* ev) {
+    if (ev) (void)hipEventRecord(ev[0], st);
     const Cst<T> C = make_cst<T, FAST>(P);
-    hipLaunchKernelGGL((k_predictor<T, FAST>), dim3(P.grid_pred), dim3(256), 0, st, P, C, B, L);
-    if (ev) hipEventRecord(ev[1], st);
-    hipLaunchKernelGGL((k_reduce_dt<T>), dim3(1), dim3(64), 0, st, L, endtime, advance_time);
-    if (ev) hipEventRecord(ev[2], st);
-    hipLaunchKernelGGL((k_corrector<T, FAST>), dim3(P.grid_corr), dim3(256), 0, st, P, C, B, L);
-    if (ev) hipEventRecord(ev[3], st);
+    hipLaunchKernelGGL((k_predictor<T, FAST, BATCH>), dim3(P.grid_pred), dim3(256), 0, st, P, C, B, L);
+    if (ev) (void)hipEventRecord(ev[1], st);
+    hipLaunchKernelGGL((k_reduce_dt<T>), dim3(BATCH ? P.nmember : 1), dim3(64), 0, st, L, advance_time);
+    if (ev) (void)hipEventRecord(ev[2], st);
+    hipLaunchKernelGGL((k_corrector<T, FAST, BATCH>), dim3(P.grid_corr), dim3(256), 0, st, P, C, B, L);
+    if (ev) (void)hipEventRecord(ev[3], st);
 }
 
-void launch_step(int precision, const KParams& P, const void* buffers, DevLoop* L, double endtime, int advance_time,
-                 hipStream_t st, hipEvent_t* ev) {
+template <class T, bool FAST>
+void launch_step_b(const KParams& P, const StepBuffers<T>& B, DevLoop* L, int advance_time, hipStream_t st,
+                   hipEvent_t* ev) {
+    if (P.sid) launch_step_t<T, FAST, true>(P, B, L, advance_time, st, ev);
+    else launch_step_t<T, FAST, false>(P, B, L, advance_time, st, ev);
+}
+
+void launch_step(int precision, const KParams& P, const void* buffers, DevLoop* L, int advance_time, hipStream_t st,
+                 hipEvent_t* ev) {
     if (precision == 0)
-        launch_step_t<double, false>(P, *(const StepBuffers<double>*)buffers, L, endtime, advance_time, st, ev);
+        launch_step_b<double, false>(P, *(const StepBuffers<double>*)buffers, L, advance_time, st, ev);
     else if (precision == 1)
-        launch_step_t<float, false>(P, *(const StepBuffers<float>*)buffers, L, endtime, advance_time, st, ev);
+        launch_step_b<float, false>(P, *(const StepBuffers<float>*)buffers, L, advance_time, st, ev);
     else
-        launch_step_t<float, true>(P, *(const StepBuffers<float>*)buffers, L, endtime, advance_time, st, ev);
+        launch_step_b<float, true>(P, *(const StepBuffers<float>*)buffers, L, advance_time, st, ev);
 }
 
 // Persistent grid: all workgroups resident at once (CUs x occupancy), a
@@ -741,25 +801,49 @@ template <class K> int grid_for(K kernel, int nblk, int wg_per_cu) {
 int persistent_grid(int precision, int corrector, int nblk, int wg_per_cu) {
     const int w = wg_per_cu;
     if (precision == 0)
-        return corrector ? grid_for(k_corrector<double, false>, nblk, w) : grid_for(k_predictor<double, false>, nblk, w);
+        return corrector ? grid_for(k_corrector<double, false, false>, nblk, w)
+                         : grid_for(k_predictor<double, false, false>, nblk, w);
     if (precision == 1)
-        return corrector ? grid_for(k_corrector<float, false>, nblk, w) : grid_for(k_predictor<float, false>, nblk, w);
-    return corrector ? grid_for(k_corrector<float, true>, nblk, w) : grid_for(k_predictor<float, true>, nblk, w);
+        return corrector ? grid_for(k_corrector<float, false, false>, nblk, w)
+                         : grid_for(k_predictor<float, false, false>, nblk, w);
+    return corrector ? grid_for(k_corrector<float, true, false>, nblk, w)
+                     : grid_for(k_predictor<float, true, false>, nblk, w);
 }
 
 void launch_init_dt(int precision, const KParams& P, const void* h, DevLoop* L, hipStream_t st) {
     if (precision == 0) {
         hipLaunchKernelGGL(k_init_dt<double>, dim3(P.nblk), dim3(256), 0, st, P, (const double*)h, L);
-        hipLaunchKernelGGL(k_reduce_dt<double>, dim3(1), dim3(64), 0, st, L, 0.0, 0);
+        hipLaunchKernelGGL(k_reduce_dt<double>, dim3(P.sid ? P.nmember : 1), dim3(64), 0, st, L, 2);
     } else {
         hipLaunchKernelGGL(k_init_dt<float>, dim3(P.nblk), dim3(256), 0, st, P, (const float*)h, L);
-        hipLaunchKernelGGL(k_reduce_dt<float>, dim3(1), dim3(64), 0, st, L, 0.0, 0);
+        hipLaunchKernelGGL(k_reduce_dt<float>, dim3(P.sid ? P.nmember : 1), dim3(64), 0, st, L, 2);
     }
 }
 
-void launch_reduce_dt(int precision, DevLoop* L, double endtime, int advance_time, hipStream_t st) {
-    if (precision == 0) hipLaunchKernelGGL(k_reduce_dt<double>, dim3(1), dim3(64), 0, st, L, endtime, advance_time);
-    else hipLaunchKernelGGL(k_reduce_dt<float>, dim3(1), dim3(64), 0, st, L, endtime, advance_time);
+void launch_init_dt_member(int precision, const KParams& P, const void* h, DevLoop* L, int member, hipStream_t st) {
+    if (precision == 0) {
+        hipLaunchKernelGGL(k_init_dt<double>, dim3(P.nblk), dim3(256), 0, st, P, (const double*)h, L);
+        hipLaunchKernelGGL(k_reduce_dt<double>, dim3(1), dim3(64), 0, st, L + member, 2);
+    } else {
+        hipLaunchKernelGGL(k_init_dt<float>, dim3(P.nblk), dim3(256), 0, st, P, (const float*)h, L);
+        hipLaunchKernelGGL(k_reduce_dt<float>, dim3(1), dim3(64), 0, st, L + member, 2);
+    }
+}
+
+void launch_maxdepth(int precision, const KParams& P, const void* h, float* hmax, hipStream_t st) {
+    if (precision == 0) hipLaunchKernelGGL(k_maxdepth<double>, dim3(P.nblk), dim3(256), 0, st, P, (const double*)h, hmax);
+    else hipLaunchKernelGGL(k_maxdepth<float>, dim3(P.nblk), dim3(256), 0, st, P, (const float*)h, hmax);
+}
+
+void launch_gauges(int precision, const void* h, const int* cell, float* val, int n, hipStream_t st) {
+    const dim3 g((n + 255) / 256), b(256);
+    if (precision == 0) hipLaunchKernelGGL(k_gauges<double>, g, b, 0, st, (const double*)h, cell, val, n);
+    else hipLaunchKernelGGL(k_gauges<float>, g, b, 0, st, (const float*)h, cell, val, n);
+}
+
+void launch_reduce_dt(int precision, DevLoop* L, int nloop, int advance_time, hipStream_t st) {
+    if (precision == 0) hipLaunchKernelGGL(k_reduce_dt<double>, dim3(nloop), dim3(64), 0, st, L, advance_time);
+    else hipLaunchKernelGGL(k_reduce_dt<float>, dim3(nloop), dim3(64), 0, st, L, advance_time);
 }
 
 void launch_volume(int precision, const KParams& P, const void* h, double* part, double* out, hipStream_t st) {
@@ -768,15 +852,16 @@ void launch_volume(int precision, const KParams& P, const void* h, double* part,
     hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, st, part, P.nblk, out);
 }
 
-void launch_copy_state(int precision, const KParams& P, const void* const* src, void* const* dst, hipStream_t st) {
+void launch_settle(int precision, const KParams& P, const void* const* src, void* const* dst, const DevLoop* L,
+                   hipStream_t st) {
     if (precision == 0)
-        hipLaunchKernelGGL(k_copy_state<double>, dim3(P.nblk), dim3(256), 0, st, P, (const double*)src[0],
+        hipLaunchKernelGGL(k_settle<double>, dim3(P.nblk), dim3(256), 0, st, P, (const double*)src[0],
                            (const double*)src[1], (const double*)src[2], (double*)dst[0], (double*)dst[1],
-                           (double*)dst[2]);
+                           (double*)dst[2], L);
     else
-        hipLaunchKernelGGL(k_copy_state<float>, dim3(P.nblk), dim3(256), 0, st, P, (const float*)src[0],
+        hipLaunchKernelGGL(k_settle<float>, dim3(P.nblk), dim3(256), 0, st, P, (const float*)src[0],
                            (const float*)src[1], (const float*)src[2], (float*)dst[0], (float*)dst[1],
-                           (float*)dst[2]);
+                           (float*)dst[2], L);
 }
 
 }  // namespace bgf

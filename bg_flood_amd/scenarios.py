@@ -281,3 +281,21 @@ def amr_basin(nbx=4, nby=4, dx=10.0, maxlevel=2, depth=10.0, amp=1.0, radius=Non
     z = np.zeros(S, T)
     cfa = np.full(S, T(cf), T)
     return dict(param=p, blocks=blocks, nbx=nbx, nby=nby, zb=zb, h=h, u=z.copy(), v=z.copy(), cf=cfa, leaves=leaves)
+
+
+def ensemble_specs(n=64, seed=7, lmin=256, lmax=2048):
+    """Seeded member specifications of the rain-on-terrain ensemble: side L log-uniform in
+    [lmin, lmax] rounded to 16 cells, bed amplitude U[10, 40] m, rain U[10, 100] mm/h,
+    Manning n U[0.02, 0.05]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        L = int(round(math.exp(rng.uniform(math.log(lmin), math.log(lmax))) / BW)) * BW
+        out.append(dict(n=max(L, BW), amp=float(rng.uniform(10, 40)), rain=float(rng.uniform(10, 100)),
+                        manning=float(rng.uniform(0.02, 0.05)), seed=seed * 1000 + k))
+    return out
+
+
+def ensemble_member(spec, precision="fp32", dx=5.0):
+    return rain_terrain(n=spec["n"], dx=dx, amp=spec["amp"], rain=spec["rain"], manning=spec["manning"],
+                        seed=spec["seed"], precision=precision)

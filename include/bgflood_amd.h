@@ -169,6 +169,59 @@ bgf_status bgf_mesh_from_leaves(int nleaf, const int* lev, const int* ix, const 
                                 int* LeftTop, int* RightBot, int* RightTop, int* TopLeft, int* TopRight,
                                 int* BotLeft, int* BotRight);
 
+/* ------------------------------------------------------------------ batches
+ * A batch advances many independent one-level domains ("members") with one
+ * set of launches per step: ensembles, and continuous batching of solve
+ * requests.  It has no counterpart in the reference, which runs one domain
+ * per process (src/Mainloop.cu:17-91); each member follows FlowGPU<T>
+ * semantics exactly as a bgf_handle would, with its own time step, time, end
+ * time and rain rate.  Members are added and removed between runs; their
+ * state stays on the GPU. */
+typedef struct bgf_batch_s* bgf_batch;
+
+/* Capacity: max_blocks block slots (a member takes its nblkmem) and
+ * max_members members.  common gives the parameters shared by all members
+ * (precision, g, eps, CFL, theta, delta, friction, boundary type, ...). */
+bgf_status bgf_batch_create(const bgf_param* common, int max_blocks, int max_members, bgf_batch* out);
+bgf_status bgf_batch_destroy(bgf_batch b);
+
+/* Adds a member: its own one-level block list (level 0) and state arrays
+ * (nblkmem*324 values each, as bgf_set_state); p must match the batch's
+ * common parameters except nblk, nblkmem and rain.  Its loop starts at
+ * totaltime 0 with the initial step of initdt (src/Mainloop.cu:515-554;
+ * dtinit <= 0 for the default) and lands exactly on endtime.  BGF_ERR_OOM
+ * when the batch is full. */
+bgf_status bgf_batch_add(bgf_batch b, const bgf_param* p, const bgf_blocks* blk, const void* zb, const void* hh,
+                         const void* u, const void* v, const void* cf, double dtinit, double endtime, int* member);
+bgf_status bgf_batch_remove(bgf_batch b, int member);
+
+/* Enqueues nsteps batch steps: every member that has not reached its end
+ * time takes one step of its own dt per batch step.  bgf_batch_sync waits. */
+bgf_status bgf_batch_run_async(bgf_batch b, int nsteps);
+bgf_status bgf_batch_sync(bgf_batch b);
+
+/* Loop state of a member and whether it reached its end time. */
+bgf_status bgf_batch_loop(bgf_batch b, int member, bgf_loop* loop, int* done);
+/* Member state, as bgf_get_state. */
+bgf_status bgf_batch_get_state(bgf_batch b, int member, void* zs, void* hh, void* u, void* v);
+/* FP64 volume of a member, as bgf_volume. */
+bgf_status bgf_batch_volume(bgf_batch b, int member, double* volume);
+
+/* Output path (src/Meanmax.cu): folds the current depth of every member into
+ * its running maximum (FP32, nblkmem*324 values per member); read it with
+ * bgf_batch_get_maxdepth. */
+bgf_status bgf_batch_update_maxdepth(bgf_batch b);
+bgf_status bgf_batch_get_maxdepth(bgf_batch b, int member, float* hmax);
+
+/* Gauges: up to 16 cells (memloc indices in the member's arrays) per member;
+ * bgf_batch_sample_gauges writes 16*max_members depths (NaN for unused
+ * gauges) and, if times is not NULL, each member's totaltime. */
+bgf_status bgf_batch_set_gauges(bgf_batch b, int member, int n, const int* cells);
+bgf_status bgf_batch_sample_gauges(bgf_batch b, float* values, double* times);
+
+/* Cell-updates performed so far, members and active blocks now. */
+bgf_status bgf_batch_stats(bgf_batch b, long long* cell_updates, int* nmembers, int* nblocks);
+
 #ifdef __cplusplus
 }
 #endif
