@@ -99,6 +99,77 @@ def cpu_baseline(d, sc, window=2048, nsteps=24):
                        % (window, window, nsteps, secs))
 
 
+def ensemble_bench(nmem=64, seed=7, horizon=7200.0, chunk=50, budget_s=90.0, plain_steps=8):
+    """The batched rain-on-terrain ensemble: nmem members (side log-uniform 256..2048 cells,
+    seeded bed 10-40 m, rain 10-100 mm/h, Manning 0.02-0.05), FP32 fast, each run to its own
+    horizon with its own dt in one batch.  Returns throughput, per-member completion latency and the
+    plain one-domain-per-launch baseline on the same members."""
+    import numpy as np
+    import bg_flood_amd as bgf
+    from bg_flood_amd import scenarios
+    t = time.time()
+    specs = scenarios.ensemble_specs(nmem, seed)
+    mem = [scenarios.ensemble_member(sp, precision="fp32") for sp in specs]
+    log("ensemble: %d members, %d cells, built in %.1f s" % (nmem, sum(m["param"]["nblk"] for m in mem) * 256,
+                                                             time.time() - t))
+    common = dict(mem[0]["param"], precision="fp32_fast")
+    B = bgf.Batch(common, max_blocks=sum(m["param"]["nblkmem"] for m in mem), max_members=nmem)
+    ids = [B.add(m["param"], m["blocks"], m["zb"], m["h"], m["u"], m["v"], m["cf"], endtime=horizon) for m in mem]
+    gpu_sync()
+    lat = {}
+    t0 = time.perf_counter()
+    steps = 0
+    while len(lat) < nmem and time.perf_counter() - t0 < budget_s:
+        B.run(chunk)
+        steps += chunk
+        now = time.perf_counter() - t0
+        for m in ids:
+            if m not in lat and B.loop(m)[1]:
+                lat[m] = now
+    wall = time.perf_counter() - t0
+    st = B.stats()
+    gcups = st["cell_updates"] / wall / 1e9
+    done = sorted(lat.values())
+    sim = [B.loop(m)[0].totaltime for m in ids]
+    B.close()
+    log("ensemble batch: %.2f Gcell/s over %.1f s, %d batch steps, %d/%d members done" % (gcups, wall, steps,
+                                                                                         len(done), nmem))
+    # plain: one domain per handle, one synchronous FlowGPU step per call (bgf_step), and the
+    # same with the steps kept on the GPU (bgf_run), a few steps of every member
+    t_step = t_run = 0.0
+    cu = 0
+    for m in mem:
+        d = bgf.Domain(dict(m["param"], precision="fp32_fast"), m["blocks"])
+        d.set_state(m["zb"], m["h"], m["u"], m["v"], m["cf"])
+        L = bgf.Loop(nextoutputtime=horizon, dt=0, dtmax=0, totaltime=0)
+        d.init_dt(L)
+        d.step(L)
+        L.totaltime += L.dt
+        a = time.perf_counter()
+        for _ in range(plain_steps):
+            d.step(L)
+            L.totaltime += L.dt
+        b = time.perf_counter()
+        d.run(L, plain_steps, endtime=horizon)
+        c = time.perf_counter()
+        t_step += b - a
+        t_run += c - b
+        cu += m["param"]["nblk"] * 256 * plain_steps
+        d.close()
+    plain_step = cu / t_step / 1e9
+    plain_run = cu / t_run / 1e9
+    log("ensemble plain: bgf_step %.2f Gcell/s, bgf_run %.2f Gcell/s" % (plain_step, plain_run))
+    pct = lambda q: round(float(np.percentile(done, q)), 3) if done else None
+    return dict(members=nmem, cells=sum(m["param"]["nblk"] for m in mem) * 256, horizon_s=horizon,
+                batch_gcups=round(gcups, 3), wall_s=round(wall, 2), batch_steps=steps, members_done=len(done),
+                latency_p50_s=pct(50), latency_p99_s=pct(99), min_sim_time_s=round(min(sim), 1),
+                plain_step_gcups=round(plain_step, 3), plain_run_gcups=round(plain_run, 3),
+                speedup_vs_plain_step=round(gcups / plain_step, 2), speedup_vs_plain_run=round(gcups / plain_run, 2),
+                roofline_frac=round(gcups * 1e9 * 72 / (HBM_PEAK_GBS * 1e9), 4),
+                note="latency = wall time from the first batch step until the member reached its horizon, "
+                     "checked every %d batch steps" % chunk)
+
+
 def load_traffic(name):
     path = os.path.join(ROOT, "profiles", "traffic_%s.json" % name)
     if os.path.exists(path):
@@ -128,6 +199,8 @@ def main():
     ap.add_argument("--precision", default="fp32_fast", choices=["fp32_fast", "fp32", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-only", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (scale reference for HBM byte counters)")
     args = ap.parse_args()
@@ -138,6 +211,10 @@ def main():
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is described in DESIGN.md "
               "and not launched (requested --gpus %d)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.ensemble_only:
+        print(json.dumps(ensemble_bench()), flush=True)
+        return
 
     from bg_flood_amd import scenarios
     t = time.time()
@@ -178,6 +255,15 @@ def main():
         except Exception as e:  # report, never hide
             cpu = dict(value=None, unit="Gcell-updates/s", cores=4, kind="port", sample="failed: %r" % (e,))
 
+    d.close()
+    del sc
+    ens = None
+    if not args.no_ensemble:
+        try:
+            ens = ensemble_bench()
+        except Exception as e:  # report, never hide
+            ens = dict(error=repr(e))
+
     out = {
         "metric": "Gcell-updates/s + %HBM roofline; batched-ensemble p50/p99 scenario latency",
         "value": round(r["gcups"], 4),
@@ -199,6 +285,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "variants": variants,
+        "ensemble": ens,
     }
     print(json.dumps(out), flush=True)
 

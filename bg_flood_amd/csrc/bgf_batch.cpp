@@ -58,6 +58,8 @@ struct bgf_batch_s {
     long long cell_updates = 0;
     int nactive_blocks = 0;
     int wg_pred = 0, wg_corr = 0;
+    bool use_graph = true;
+    hipGraphExec_t graph = nullptr;  // an even/odd pair of batch steps; rebuilt after membership changes
 };
 
 namespace {
@@ -71,6 +73,7 @@ void free_batch(bgf_batch b) {
     for (int k = 0; k < 2; k++) { F(b->d_h[k]); F(b->d_u[k]); F(b->d_v[k]); }
     F(b->d_zb); F(b->d_cf); F(b->d_dh); F(b->d_dhu); F(b->d_dhv); F(b->d_hmax);
     F(b->d_volpart); F(b->d_volout); F(b->d_gcell); F(b->d_gval); F(b->d_loop);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -98,6 +101,10 @@ bgf_status rebuild(bgf_batch b) {
         for (int k = 0; k < M.nblkmem; k++) sid[M.off + k] = m;
     }
     b->nactive_blocks = (int)active.size();
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
     if (!active.empty()) {
         HIPCHK(hipMemcpy(b->d_active, active.data(), sizeof(int) * active.size(), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(b->d_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
@@ -132,6 +139,26 @@ bgf_status check_member(bgf_batch b, int m) {
     if (m < 0 || m >= b->cap_members || !b->members[m].used) return fail(BGF_ERR_INVALID_ARG, "no such member");
     if (b->pending) return fail(BGF_ERR_STATE, "bgf_batch_run_async pending: call bgf_batch_sync first");
     return BGF_OK;
+}
+
+// Enqueues nsteps batch steps starting at buffer parity 0.
+void enqueue_steps(bgf_batch b, int nsteps) {
+    alignas(16) unsigned char storage[sizeof(bgf::StepBuffers<double>)];
+    for (int k = 0; k < nsteps; k++) {
+        const int in = k & 1, out = 1 - in;
+        if (b->prec == 0) {
+            auto* B = (bgf::StepBuffers<double>*)storage;
+            *B = {(const double*)b->d_h[in], (const double*)b->d_u[in], (const double*)b->d_v[in],
+                  (double*)b->d_h[out], (double*)b->d_u[out], (double*)b->d_v[out], (const double*)b->d_zb,
+                  (const double*)b->d_cf, (double*)b->d_dh, (double*)b->d_dhu, (double*)b->d_dhv};
+        } else {
+            auto* B = (bgf::StepBuffers<float>*)storage;
+            *B = {(const float*)b->d_h[in], (const float*)b->d_u[in], (const float*)b->d_v[in],
+                  (float*)b->d_h[out], (float*)b->d_u[out], (float*)b->d_v[out], (const float*)b->d_zb,
+                  (const float*)b->d_cf, (float*)b->d_dh, (float*)b->d_dhu, (float*)b->d_dhv};
+        }
+        bgf::launch_step(b->prec, b->kp, storage, b->d_loop, 1, b->stream, nullptr);
+    }
 }
 
 }  // namespace
@@ -342,22 +369,23 @@ bgf_status bgf_batch_run_async(bgf_batch b, int nsteps) {
     HIPCHK(hipMemcpyAsync(b->d_loop, b->loops.data(), sizeof(bgf::DevLoop) * b->cap_members, hipMemcpyHostToDevice,
                           b->stream));
     if (b->nactive_blocks > 0) {
-        alignas(16) unsigned char storage[sizeof(bgf::StepBuffers<double>)];
-        for (int k = 0; k < nsteps; k++) {
-            const int in = k & 1, out = 1 - in;
-            if (b->prec == 0) {
-                auto* B = (bgf::StepBuffers<double>*)storage;
-                *B = {(const double*)b->d_h[in], (const double*)b->d_u[in], (const double*)b->d_v[in],
-                      (double*)b->d_h[out], (double*)b->d_u[out], (double*)b->d_v[out], (const double*)b->d_zb,
-                      (const double*)b->d_cf, (double*)b->d_dh, (double*)b->d_dhu, (double*)b->d_dhv};
-            } else {
-                auto* B = (bgf::StepBuffers<float>*)storage;
-                *B = {(const float*)b->d_h[in], (const float*)b->d_u[in], (const float*)b->d_v[in],
-                      (float*)b->d_h[out], (float*)b->d_u[out], (float*)b->d_v[out], (const float*)b->d_zb,
-                      (const float*)b->d_cf, (float*)b->d_dh, (float*)b->d_dhu, (float*)b->d_dhv};
+        if (b->use_graph && nsteps >= 4) {
+            if (!b->graph) {
+                hipGraph_t g = nullptr;
+                HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+                enqueue_steps(b, 2);
+                HIPCHK(hipStreamEndCapture(b->stream, &g));
+                hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (e != hipSuccess) {
+                    b->graph = nullptr;
+                    return fail(BGF_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+                }
             }
-            bgf::launch_step(b->prec, b->kp, storage, b->d_loop, 1, b->stream, nullptr);
+            for (int k = 0; k < nsteps / 2; k++) HIPCHK(hipGraphLaunch(b->graph, b->stream));
+            nsteps &= 1;
         }
+        enqueue_steps(b, nsteps);
     }
     HIPCHK(hipGetLastError());
     b->pending = 1;

@@ -48,6 +48,8 @@ struct bgf_handle_s {
     bool amr = false;
     int* d_inv = nullptr;
     void* d_amr[17] = {nullptr};  // 8 gradients, 8 face fluxes, hsnap
+    bool use_graph = true;
+    hipGraphExec_t graph = nullptr;  // one period of steps, captured on first use
 };
 
 namespace {
@@ -62,6 +64,7 @@ void free_handle(bgf_handle h) {
     F(h->d_inv);
     for (int k = 0; k < 17; k++) F(h->d_amr[k]);
     for (int k = 0; k < 4; k++) if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
+    if (h->graph) (void)hipGraphExecDestroy(h->graph);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -124,7 +127,48 @@ void fill_amr_buffers(bgf_handle h, void* storage) {
     else fill_amr_t<float>(h, (bgf::AmrBuffers<float>*)storage);
 }
 
+// Enqueues `period` steps (one AMR step, or an even/odd pair of one-level
+// steps) starting at buffer parity 0.
+void enqueue_period(bgf_handle h, int advance_time) {
+    alignas(16) unsigned char storage[sizeof(bgf::AmrBuffers<double>)];
+    if (h->amr) {
+        fill_amr_buffers(h, storage);
+        bgf::launch_amr_step(h->prec == 0 ? 0 : 1, h->kp, storage, h->d_loop, advance_time, h->stream, nullptr);
+        return;
+    }
+    for (int k = 0; k < 2; k++) {
+        fill_buffers(h, k, storage);
+        bgf::launch_step(h->prec, h->kp, storage, h->d_loop, advance_time, h->stream, nullptr);
+    }
+}
+
+// Runs of many steps replay a captured HIP graph of one period instead of
+// relaunching every kernel (the step is launch-bound on small domains and
+// multi-level meshes).  The graph holds kernel launches only.
+bgf_status graph_steps(bgf_handle h, int nperiods) {
+    if (!h->graph) {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_period(h, 1);
+        HIPCHK(hipStreamEndCapture(h->stream, &g));
+        hipError_t e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            h->graph = nullptr;
+            return fail(BGF_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+    }
+    for (int k = 0; k < nperiods; k++) HIPCHK(hipGraphLaunch(h->graph, h->stream));
+    return BGF_OK;
+}
+
 bgf_status enqueue_steps(bgf_handle h, int nsteps, int advance_time) {
+    const int period = h->amr ? 1 : 2;
+    if (advance_time == 1 && !h->timing && h->use_graph && nsteps >= 2 * period) {
+        bgf_status s = graph_steps(h, nsteps / period);
+        if (s != BGF_OK) return s;
+        nsteps %= period;  // at most one step left, at parity 0
+    }
     alignas(16) unsigned char storage[sizeof(bgf::AmrBuffers<double>)];
     if (h->amr) fill_amr_buffers(h, storage);
     for (int k = 0; k < nsteps; k++) {
@@ -282,6 +326,7 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
     // predictor and the corrector (a tuning override; default: occupancy).
     int wp = 0, wc = 0;
     if (const char* e = std::getenv("BGF_WG_PER_CU")) std::sscanf(e, "%d,%d", &wp, &wc);
+    if (const char* e = std::getenv("BGF_GRAPH")) h->use_graph = std::atoi(e) != 0;
     kp.grid_pred = bgf::persistent_grid(h->prec, 0, nblk, wp);
     kp.grid_corr = bgf::persistent_grid(h->prec, 1, nblk, wc);
     if (kp.grid_pred <= 0 || kp.grid_corr <= 0) {
