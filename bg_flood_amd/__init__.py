@@ -21,8 +21,8 @@ BW = 16
 MW = 18
 BS = MW * MW
 
-FP64, FP32, FP32_FAST = 0, 1, 2
-PRECISIONS = {"fp64": FP64, "fp32": FP32, "fp32_fast": FP32_FAST}
+FP64, FP32, FP32_FAST, FP32_COMP = 0, 1, 2, 3
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "fp32_fast": FP32_FAST, "fp32_comp": FP32_COMP}
 
 
 class BgfError(RuntimeError):

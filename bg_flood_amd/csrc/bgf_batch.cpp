@@ -48,6 +48,7 @@ struct bgf_batch_s {
     int* d_nb[8] = {nullptr};
     void *d_h[2] = {nullptr, nullptr}, *d_u[2] = {nullptr, nullptr}, *d_v[2] = {nullptr, nullptr};
     void *d_zb = nullptr, *d_cf = nullptr, *d_dh = nullptr, *d_dhu = nullptr, *d_dhv = nullptr, *d_hmax = nullptr;
+    void* d_ch = nullptr;  // compensation of h (BGF_FP32_COMP)
     double *d_volpart = nullptr, *d_volout = nullptr;
     int* d_gcell = nullptr;    // gauge cells, 16 per member slot
     float* d_gval = nullptr;   // sampled depths
@@ -71,7 +72,7 @@ void free_batch(bgf_batch b) {
     F(b->d_active); F(b->d_order); F(b->d_sid); F(b->d_level);
     for (int k = 0; k < 8; k++) F(b->d_nb[k]);
     for (int k = 0; k < 2; k++) { F(b->d_h[k]); F(b->d_u[k]); F(b->d_v[k]); }
-    F(b->d_zb); F(b->d_cf); F(b->d_dh); F(b->d_dhu); F(b->d_dhv); F(b->d_hmax);
+    F(b->d_zb); F(b->d_cf); F(b->d_dh); F(b->d_dhu); F(b->d_dhv); F(b->d_hmax); F(b->d_ch);
     F(b->d_volpart); F(b->d_volout); F(b->d_gcell); F(b->d_gval); F(b->d_loop);
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -150,12 +151,12 @@ void enqueue_steps(bgf_batch b, int nsteps) {
             auto* B = (bgf::StepBuffers<double>*)storage;
             *B = {(const double*)b->d_h[in], (const double*)b->d_u[in], (const double*)b->d_v[in],
                   (double*)b->d_h[out], (double*)b->d_u[out], (double*)b->d_v[out], (const double*)b->d_zb,
-                  (const double*)b->d_cf, (double*)b->d_dh, (double*)b->d_dhu, (double*)b->d_dhv};
+                  (const double*)b->d_cf, (double*)b->d_dh, (double*)b->d_dhu, (double*)b->d_dhv, nullptr};
         } else {
             auto* B = (bgf::StepBuffers<float>*)storage;
             *B = {(const float*)b->d_h[in], (const float*)b->d_u[in], (const float*)b->d_v[in],
                   (float*)b->d_h[out], (float*)b->d_u[out], (float*)b->d_v[out], (const float*)b->d_zb,
-                  (const float*)b->d_cf, (float*)b->d_dh, (float*)b->d_dhu, (float*)b->d_dhv};
+                  (const float*)b->d_cf, (float*)b->d_dh, (float*)b->d_dhu, (float*)b->d_dhv, (float*)b->d_ch};
         }
         bgf::launch_step(b->prec, b->kp, storage, b->d_loop, 1, b->stream, nullptr);
     }
@@ -197,6 +198,7 @@ bgf_status bgf_batch_create(const bgf_param* common, int max_blocks, int max_mem
     ok = ok && A(&b->d_zb, b->es * b->S) && A(&b->d_cf, b->es * b->S) && A(&b->d_dh, b->es * b->S) &&
          A(&b->d_dhu, b->es * b->S) && A(&b->d_dhv, b->es * b->S) && A(&b->d_hmax, sizeof(float) * b->S);
     ok = ok && A((void**)&b->d_volpart, sizeof(double) * max_blocks) && A((void**)&b->d_volout, sizeof(double));
+    if (b->prec == BGF_FP32_COMP) ok = ok && A(&b->d_ch, b->es * b->S);
     ok = ok && A((void**)&b->d_gcell, sizeof(int) * NGAUGE * max_members) &&
          A((void**)&b->d_gval, sizeof(float) * NGAUGE * max_members);
     ok = ok && A((void**)&b->d_loop, sizeof(bgf::DevLoop) * max_members);
@@ -321,6 +323,8 @@ bgf_status bgf_batch_add(bgf_batch b, const bgf_param* p, const bgf_blocks* blk,
     if (hipMemset((char*)b->d_hmax + sizeof(float) * (size_t)off * BS, 0, sizeof(float) * (size_t)nbm * BS) !=
         hipSuccess)
         return undo(fail(BGF_ERR_HIP, "max-depth reset failed"));
+    if (b->d_ch && hipMemset((char*)b->d_ch + o, 0, bytes) != hipSuccess)
+        return undo(fail(BGF_ERR_HIP, "compensation reset failed"));
     s = rebuild(b);
     if (s != BGF_OK) return undo(s);
     // initial time step of the member (src/Mainloop.cu:515-554)

@@ -48,6 +48,7 @@ struct bgf_handle_s {
     bool amr = false;
     int* d_inv = nullptr;
     void* d_amr[17] = {nullptr};  // 8 gradients, 8 face fluxes, hsnap
+    void* d_ch = nullptr;          // compensation of h (BGF_FP32_COMP)
     bool use_graph = true;
     hipGraphExec_t graph = nullptr;  // one period of steps, captured on first use
 };
@@ -62,6 +63,7 @@ void free_handle(bgf_handle h) {
     F(h->d_zb); F(h->d_cf); F(h->d_dh); F(h->d_dhu); F(h->d_dhv);
     F(h->d_volpart); F(h->d_volout); F(h->d_loop);
     F(h->d_inv);
+    F(h->d_ch);
     for (int k = 0; k < 17; k++) F(h->d_amr[k]);
     for (int k = 0; k < 4; k++) if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
     if (h->graph) (void)hipGraphExecDestroy(h->graph);
@@ -102,12 +104,14 @@ void fill_buffers(bgf_handle h, int in, void* out_storage) {
         B->ho = (double*)h->d_h[out]; B->uo = (double*)h->d_u[out]; B->vo = (double*)h->d_v[out];
         B->zb = (const double*)h->d_zb; B->cf = (const double*)h->d_cf;
         B->dh = (double*)h->d_dh; B->dhu = (double*)h->d_dhu; B->dhv = (double*)h->d_dhv;
+        B->ch = nullptr;
     } else {
         auto* B = (bgf::StepBuffers<float>*)out_storage;
         B->h = (const float*)h->d_h[in]; B->u = (const float*)h->d_u[in]; B->v = (const float*)h->d_v[in];
         B->ho = (float*)h->d_h[out]; B->uo = (float*)h->d_u[out]; B->vo = (float*)h->d_v[out];
         B->zb = (const float*)h->d_zb; B->cf = (const float*)h->d_cf;
         B->dh = (float*)h->d_dh; B->dhu = (float*)h->d_dhu; B->dhv = (float*)h->d_dhv;
+        B->ch = (float*)h->d_ch;
     }
 }
 
@@ -283,6 +287,7 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
     ok = ok && A((void**)&h->d_volpart, sizeof(double) * nblk) && A((void**)&h->d_volout, sizeof(double));
     ok = ok && A((void**)&h->d_loop, sizeof(bgf::DevLoop));
     h->amr = multilevel;
+    if (p->precision == BGF_FP32_COMP && !multilevel) ok = ok && A(&h->d_ch, h->es * h->S);
     if (multilevel) {
         ok = ok && A((void**)&h->d_inv, sizeof(int) * nbm);
         for (int k = 0; k < 17; k++) ok = ok && A(&h->d_amr[k], h->es * h->S);
@@ -365,6 +370,7 @@ bgf_status bgf_set_state(bgf_handle h, const void* zb, const void* hh, const voi
     HIPCHK(hipMemcpyAsync(h->d_u[0], u, bytes, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(h->d_v[0], v, bytes, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(h->d_cf, cf, bytes, hipMemcpyHostToDevice, h->stream));
+    if (h->d_ch) HIPCHK(hipMemsetAsync(h->d_ch, 0, h->es * h->S, h->stream));
     if (h->amr) {
         // halo of the bed, as InitzbHalo (src/InitialConditions.cu:135-147)
         bgf::launch_amr_fill(h->prec == 0 ? 0 : 1, h->kp, h->d_zb, h->stream);

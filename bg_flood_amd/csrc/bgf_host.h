@@ -40,7 +40,7 @@ inline bgf_status check_param(const bgf_param* p) {
     if (p->engine != 2) return fail(BGF_ERR_UNSUPPORTED, "only engine 2 (Kurganov) is provided");
     if (p->spherical) return fail(BGF_ERR_UNSUPPORTED, "spherical grids are not provided");
     if (p->nblk <= 0 || p->nblkmem < p->nblk) return fail(BGF_ERR_INVALID_ARG, "need 0 < nblk <= nblkmem");
-    if (p->precision < 0 || p->precision > 2) return fail(BGF_ERR_INVALID_ARG, "bad precision");
+    if (p->precision < 0 || p->precision > 3) return fail(BGF_ERR_INVALID_ARG, "bad precision");
     if (p->aoibnd != 0 && p->aoibnd != 1)
         return fail(BGF_ERR_UNSUPPORTED, "remainder boundary type must be 0 (wall) or 1 (Neumann)");
     if (p->frictionmodel < -1 || p->frictionmodel > 1) return fail(BGF_ERR_INVALID_ARG, "bad frictionmodel");

@@ -62,6 +62,7 @@ template <class T> struct StepBuffers {
     T* dh;
     T* dhu;
     T* dhv;
+    T* ch;  // compensation of h (BGF_FP32_COMP): the low-order part lost by the last updates
 };
 
 // ev (optional): four events recorded before the predictor, after it, after

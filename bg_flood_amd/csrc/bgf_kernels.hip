@@ -519,8 +519,18 @@ __global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuff
     if (t == 0) slot_min(L, blockIdx.x, m);
 }
 
-// Persistent corrector with the same schedule and prefetch.
-template <class T, bool FAST, bool BATCH>
+// Kahan step: h + d with the running compensation c (Kahan 1965).
+template <class T> __device__ __forceinline__ T kahan_add(T h, T d, T& c) {
+    const T y = d - c;
+    const T t = h + y;
+    c = (t - h) - y;
+    return t;
+}
+
+// Persistent corrector with the same schedule and prefetch.  COMP: the depth
+// update accumulates with a compensated (Kahan) sum, so the small increments
+// of rain and of slow flows are not lost to FP32 rounding of h.
+template <class T, bool FAST, bool BATCH, bool COMP>
 __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuffers<T> B, DevLoop* L) {
     if (!BATCH && !L->go) return;
     __shared__ Tile<T> s;
@@ -587,7 +597,19 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
         (void)tile_fluxes<T, FAST, false>(s, C, t, wall && nL == ib, wall && nR == ib, wall && nB == ib,
                                           wall && nT == ib, dh, dhu, dhv);
         T h1, u1, v1;
-        advance<T, FAST>(C, fmc, dt, h0, u0, v0, dh, dhu, dhv, h1, u1, v1);
+        const int i = memloc(x, y, ib);
+        T comp = COMP ? B.ch[i] : T(0);
+        if (COMP) {
+            // advance() with the depth increment added through the compensated sum
+            T edt = dt;
+            if (!(fmc || dh >= T(0.0))) edt = dmin(dt, dmax(h0, C.eps) * Ar<T, FAST>::rcp(fabs(dh)));
+            h1 = kahan_add(h0, edt * dh, comp);
+            const T r = Ar<T, FAST>::rcp(h1);
+            u1 = h1 > C.eps ? fma(h0, u0, edt * dhu) * r : T(0.0);
+            v1 = h1 > C.eps ? fma(h0, v0, edt * dhv) * r : T(0.0);
+        } else {
+            advance<T, FAST>(C, fmc, dt, h0, u0, v0, dh, dhu, dhv, h1, u1, v1);
+        }
         // bottom friction, src/Friction.cu:89-148
         if (h1 > C.eps) {
             typedef Ar<T, FAST> A;
@@ -620,16 +642,16 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
             T rainhh = dmax(T(Lm->rain) / T(1000.0) / T(3600.0) * dt, T(0.0)) * act;
             T qvol = FAST ? h1 * Ar<T, FAST>::rcp(h1 + rainhh) : h1 / (h1 + rainhh);
             bool wetold = (double)h1 > C.eps_d;
-            h1 = h1 + rainhh;
+            h1 = COMP ? kahan_add(h1, rainhh, comp) : h1 + rainhh;
             if (wetold) {
                 u1 = u1 * qvol;
                 v1 = v1 * qvol;
             }
         }
-        const int i = memloc(x, y, ib);
         B.ho[i] = h1;
         B.uo[i] = u1;
         B.vo[i] = v1;
+        if (COMP) B.ch[i] = comp;
     }
 }
 
@@ -752,7 +774,7 @@ __global__ __launch_bounds__(256) void k_settle(KParams P, const T* __restrict__
 }
 
 // ------------------------------------------------------------------ launch
-template <class T, bool FAST, bool BATCH>
+template <class T, bool FAST, bool BATCH, bool COMP>
 void launch_step_t(const KParams& P, const StepBuffers<T>& B, DevLoop* L, int advance_time, hipStream_t st,
                    hipEvent_t* ev) {
     if (ev) (void)hipEventRecord(ev[0], st);
@@ -761,25 +783,27 @@ void launch_step_t(const KParams& P, const StepBuffers<T>& B, DevLoop* L, int ad
     if (ev) (void)hipEventRecord(ev[1], st);
     hipLaunchKernelGGL((k_reduce_dt<T>), dim3(BATCH ? P.nmember : 1), dim3(64), 0, st, L, advance_time);
     if (ev) (void)hipEventRecord(ev[2], st);
-    hipLaunchKernelGGL((k_corrector<T, FAST, BATCH>), dim3(P.grid_corr), dim3(256), 0, st, P, C, B, L);
+    hipLaunchKernelGGL((k_corrector<T, FAST, BATCH, COMP>), dim3(P.grid_corr), dim3(256), 0, st, P, C, B, L);
     if (ev) (void)hipEventRecord(ev[3], st);
 }
 
-template <class T, bool FAST>
+template <class T, bool FAST, bool COMP>
 void launch_step_b(const KParams& P, const StepBuffers<T>& B, DevLoop* L, int advance_time, hipStream_t st,
                    hipEvent_t* ev) {
-    if (P.sid) launch_step_t<T, FAST, true>(P, B, L, advance_time, st, ev);
-    else launch_step_t<T, FAST, false>(P, B, L, advance_time, st, ev);
+    if (P.sid) launch_step_t<T, FAST, true, COMP>(P, B, L, advance_time, st, ev);
+    else launch_step_t<T, FAST, false, COMP>(P, B, L, advance_time, st, ev);
 }
 
 void launch_step(int precision, const KParams& P, const void* buffers, DevLoop* L, int advance_time, hipStream_t st,
                  hipEvent_t* ev) {
     if (precision == 0)
-        launch_step_b<double, false>(P, *(const StepBuffers<double>*)buffers, L, advance_time, st, ev);
+        launch_step_b<double, false, false>(P, *(const StepBuffers<double>*)buffers, L, advance_time, st, ev);
     else if (precision == 1)
-        launch_step_b<float, false>(P, *(const StepBuffers<float>*)buffers, L, advance_time, st, ev);
+        launch_step_b<float, false, false>(P, *(const StepBuffers<float>*)buffers, L, advance_time, st, ev);
+    else if (precision == 2)
+        launch_step_b<float, true, false>(P, *(const StepBuffers<float>*)buffers, L, advance_time, st, ev);
     else
-        launch_step_b<float, true>(P, *(const StepBuffers<float>*)buffers, L, advance_time, st, ev);
+        launch_step_b<float, true, true>(P, *(const StepBuffers<float>*)buffers, L, advance_time, st, ev);
 }
 
 // Persistent grid: all workgroups resident at once (CUs x occupancy), a
@@ -801,12 +825,15 @@ template <class K> int grid_for(K kernel, int nblk, int wg_per_cu) {
 int persistent_grid(int precision, int corrector, int nblk, int wg_per_cu) {
     const int w = wg_per_cu;
     if (precision == 0)
-        return corrector ? grid_for(k_corrector<double, false, false>, nblk, w)
+        return corrector ? grid_for(k_corrector<double, false, false, false>, nblk, w)
                          : grid_for(k_predictor<double, false, false>, nblk, w);
     if (precision == 1)
-        return corrector ? grid_for(k_corrector<float, false, false>, nblk, w)
+        return corrector ? grid_for(k_corrector<float, false, false, false>, nblk, w)
                          : grid_for(k_predictor<float, false, false>, nblk, w);
-    return corrector ? grid_for(k_corrector<float, true, false>, nblk, w)
+    if (precision == 2)
+        return corrector ? grid_for(k_corrector<float, true, false, false>, nblk, w)
+                         : grid_for(k_predictor<float, true, false>, nblk, w);
+    return corrector ? grid_for(k_corrector<float, true, false, true>, nblk, w)
                      : grid_for(k_predictor<float, true, false>, nblk, w);
 }
 
