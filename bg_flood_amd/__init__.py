@@ -39,7 +39,7 @@ class BgfParam(ctypes.Structure):
         ("ForceMassConserve", ctypes.c_int), ("wetdryfix", ctypes.c_int), ("spherical", ctypes.c_int),
         ("nblk", ctypes.c_int), ("nblkmem", ctypes.c_int), ("minlevel", ctypes.c_int),
         ("maxlevel", ctypes.c_int), ("rain_on", ctypes.c_int), ("rain", ctypes.c_double),
-        ("precision", ctypes.c_int),
+        ("precision", ctypes.c_int), ("Radius", ctypes.c_double),
     ]
 
 
@@ -174,7 +174,7 @@ def default_param(**kw):
     p = dict(g=9.81, eps=1e-4, CFL=0.5, theta=1.3, dx=1.0, delta=None, xo=0.0, yo=0.0, lat=0.0, mask=9999.0,
              frictionmodel=0, engine=2, aoibnd=0, ForceMassConserve=0, wetdryfix=1, spherical=0, nblk=0,
              nblkmem=0, minlevel=0, maxlevel=0, rain_on=0, rain=0.0, precision="fp64", cf=1e-4,
-             dtinit=-1.0)
+             dtinit=-1.0, Radius=6371220.0)
     p.update(kw)
     if p["delta"] is None:
         p["delta"] = p["dx"]
@@ -183,6 +183,7 @@ def default_param(**kw):
 
 def _precision_code(prec):
     return PRECISIONS[prec] if isinstance(prec, str) else int(prec)
+
 
 
 def _param_struct(param):

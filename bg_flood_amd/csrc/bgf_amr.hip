@@ -467,9 +467,9 @@ template <class T> struct FluxOut {
 // shared with a coarser block (src/Kurganov.cu:132-145).  Returns whether the
 // face is wet; dt receives the face's CFL step.
 template <class T>
-__device__ __forceinline__ bool kface(const KParams& P, T delta, const T* h, const T* zb, const T* un_, const T* ut_,
-                                      const T* dhd, const T* dzsd, const T* dund, const T* dutd, int i, int n,
-                                      int subi, int subn, T* F0, T* F1, T* F2, T* F3, T& dt) {
+__device__ __forceinline__ bool kface(const KParams& P, T delta, T cm, T fm, const T* h, const T* zb, const T* un_,
+                                      const T* ut_, const T* dhd, const T* dzsd, const T* dund, const T* dutd, int i,
+                                      int n, int subi, int subn, T* F0, T* F1, T* F2, T* F3, T& dt) {
     const T g = T(P.g), CFL = T(P.CFL);
     const T epsi = epsi_of<T>();
     const T eps = T(P.eps) + epsi;
@@ -497,7 +497,7 @@ __device__ __forceinline__ bool kface(const KParams& P, T delta, const T* h, con
     const T hm = smax(T(0.0), hr + zr - zlr);
     const T ga = g * T(0.5);
     T fh, fu, fv;
-    dt = kurg_solver(g, delta, epsi, CFL, T(1.0), T(1.0), hp, hm, up, um, fh, fu);
+    dt = kurg_solver(g, delta, epsi, CFL, cm, fm, hp, hm, up, um, fh, fu);
     if (fh > T(0.0)) fv = (ut_[n] + dx * dutd[n]) * fh;
     else fv = (ut_[i] - dx * dutd[i]) * fh;
     if (subi >= 0) {
@@ -510,11 +510,10 @@ __device__ __forceinline__ bool kface(const KParams& P, T delta, const T* h, con
     }
     const T sl = ga * (hp * hp - hl * hl + (hl + hi) * (zi - zl));
     const T sr = ga * (hm * hm - hr * hr + (hr + hn) * (zn - zr));
-    const T fmu = T(1.0);
-    F0[i] = fmu * fh;
-    F1[i] = fmu * (fu - sl);
-    F2[i] = fmu * (fu - sr);
-    F3[i] = fmu * fv;
+    F0[i] = fm * fh;
+    F1[i] = fm * (fu - sl);
+    F2[i] = fm * (fu - sr);
+    F3[i] = fm * fv;
     return true;
 }
 
@@ -541,7 +540,9 @@ __device__ __forceinline__ bool xface(const KParams& P, T delta, int ib, int lev
             subn = memloc(BW - 1, jj, LB);
         }
     }
-    return kface(P, delta, E.h, zb, E.u, E.v, G.dhdx, G.dzsdx, G.dudx, G.dvdx, memloc(ix, iy, ib),
+    // spherical: cm of the row, fm = 1 on X faces
+    const T cm = P.spherical ? ((const T*)P.metric)[(size_t)ib * METRIC_STRIDE + MET_CMX + iy] : T(1.0);
+    return kface(P, delta, cm, T(1.0), E.h, zb, E.u, E.v, G.dhdx, G.dzsdx, G.dudx, G.dvdx, memloc(ix, iy, ib),
                  memloc(ix - 1, iy, ib), subi, subn, F.Fhu, F.Fqux, F.Su, F.Fqvx, dt);
 }
 
@@ -565,7 +566,10 @@ __device__ __forceinline__ bool yface(const KParams& P, T delta, int ib, int lev
             subn = memloc(jj, BW - 1, BL);
         }
     }
-    return kface(P, delta, E.h, zb, E.v, E.u, G.dhdy, G.dzsdy, G.dvdy, G.dudy, memloc(ix, iy, ib),
+    const T* m = (const T*)P.metric + (size_t)ib * METRIC_STRIDE;
+    const T cm = P.spherical ? m[MET_CMY + iy] : T(1.0);
+    const T fm = P.spherical ? m[MET_FMY + iy] : T(1.0);
+    return kface(P, delta, cm, fm, E.h, zb, E.v, E.u, G.dhdy, G.dzsdy, G.dvdy, G.dudy, memloc(ix, iy, ib),
                  memloc(ix, iy - 1, ib), subi, subn, F.Fhv, F.Fqvy, F.Sv, F.Fquy, dt);
 }
 
@@ -648,9 +652,12 @@ __global__ __launch_bounds__(256) void k_amr_advance(KParams P, AdvArgs<T> A, Fl
     const int i = memloc(x, y, ib), iright = memloc(x + 1, y, ib), itop = memloc(x, y + 1, ib);
     const T delta = calcres(T(P.delta0), P.level[ib]);
     const T g = T(P.g);
-    const T fc = T(P.fc);
-    // src/Advection.cu:160-254 (Cartesian metric terms)
-    const T cm = T(1.0), fmu = T(1.0), fmv = T(1.0), fmup = T(1.0), fmvp = T(1.0);
+    // src/Advection.cu:160-254; spherical grids take the row's metric terms
+    const T* met = (const T*)P.metric + (size_t)ib * METRIC_STRIDE;
+    const bool sph = P.spherical;
+    const T fc = sph ? met[MET_FC + y] : T(P.fc);
+    const T cm = sph ? met[MET_CME + y] : T(1.0), fmu = T(1.0), fmup = T(1.0);
+    const T fmv = sph ? met[MET_FMV + y] : T(1.0), fmvp = sph ? met[MET_FMVP + y] : T(1.0);
     const T hi = A.h[i], uui = A.u[i], vvi = A.v[i];
     const T cmdinv = T(1.0) / (cm * delta);
     const T ga = T(0.5) * g;

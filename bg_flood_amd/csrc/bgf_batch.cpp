@@ -254,6 +254,7 @@ bgf_status bgf_batch_add(bgf_batch b, const bgf_param* p, const bgf_blocks* blk,
     if (b->pending) return fail(BGF_ERR_STATE, "bgf_batch_run_async pending: call bgf_batch_sync first");
     bgf_status s = check_param(p);
     if (s != BGF_OK) return s;
+    if (p->spherical) return fail(BGF_ERR_UNSUPPORTED, "batch members must be Cartesian grids");
     const bgf_param& c = b->common;
     if (p->precision != c.precision || p->g != c.g || p->eps != c.eps || p->CFL != c.CFL || p->theta != c.theta ||
         p->delta != c.delta || p->frictionmodel != c.frictionmodel || p->aoibnd != c.aoibnd ||

@@ -49,6 +49,7 @@ struct bgf_handle_s {
     int* d_inv = nullptr;
     void* d_amr[17] = {nullptr};  // 8 gradients, 8 face fluxes, hsnap
     void* d_ch = nullptr;          // compensation of h (BGF_FP32_COMP)
+    void* d_metric = nullptr;      // spherical metric terms per block
     bool use_graph = true;
     hipGraphExec_t graph = nullptr;  // one period of steps, captured on first use
 };
@@ -64,6 +65,7 @@ void free_handle(bgf_handle h) {
     F(h->d_volpart); F(h->d_volout); F(h->d_loop);
     F(h->d_inv);
     F(h->d_ch);
+    F(h->d_metric);
     for (int k = 0; k < 17; k++) F(h->d_amr[k]);
     for (int k = 0; k < 4; k++) if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
     if (h->graph) (void)hipGraphExecDestroy(h->graph);
@@ -260,6 +262,8 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
         }
     }
     if (minlev != maxlev) multilevel = true;
+    // spherical grids run on the general (multi-level) kernels, which take per-row metric terms
+    if (p->spherical) multilevel = true;
     auto* h = new bgf_handle_s();
     h->p = *p;
     h->prec = p->precision;
@@ -288,6 +292,7 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
     ok = ok && A((void**)&h->d_loop, sizeof(bgf::DevLoop));
     h->amr = multilevel;
     if (p->precision == BGF_FP32_COMP && !multilevel) ok = ok && A(&h->d_ch, h->es * h->S);
+    if (p->spherical) ok = ok && A(&h->d_metric, h->es * (size_t)nbm * bgf::METRIC_STRIDE);
     if (multilevel) {
         ok = ok && A((void**)&h->d_inv, sizeof(int) * nbm);
         for (int k = 0; k < 17; k++) ok = ok && A(&h->d_amr[k], h->es * h->S);
@@ -303,6 +308,15 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
     cp = cp && hipMemcpy(h->d_order, order.data(), sizeof(int) * nblk, hipMemcpyHostToDevice) == hipSuccess;
     cp = cp && hipMemcpy(h->d_level, b->level, sizeof(int) * nbm, hipMemcpyHostToDevice) == hipSuccess;
     for (int k = 0; k < 8; k++) cp = cp && hipMemcpy(h->d_nb[k], nbs[k], sizeof(int) * nbm, hipMemcpyHostToDevice) == hipSuccess;
+    if (p->spherical) {
+        if (h->prec == 0) {
+            std::vector<double> M = metric_table<double>(p, b, nbm);
+            cp = cp && hipMemcpy(h->d_metric, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice) == hipSuccess;
+        } else {
+            std::vector<float> M = metric_table<float>(p, b, nbm);
+            cp = cp && hipMemcpy(h->d_metric, M.data(), sizeof(float) * M.size(), hipMemcpyHostToDevice) == hipSuccess;
+        }
+    }
     if (multilevel) {
         cp = cp && hipMemcpy(h->d_inv, inv.data(), sizeof(int) * nbm, hipMemcpyHostToDevice) == hipSuccess;
     }
@@ -350,6 +364,8 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
     kp.TopRight = h->d_nb[5];
     kp.BotRight = h->d_nb[7];
     kp.inv = h->d_inv;
+    kp.metric = h->d_metric;
+    kp.spherical = p->spherical;
     *out = h;
     g_err.clear();
     return BGF_OK;

@@ -35,10 +35,67 @@ inline int memloc(int i, int j, int ib) { return (i + 1) + (j + 1) * MW + ib * B
 
 inline double level_dx(double dx, int lev) { return lev < 0 ? dx * (1 << -lev) : dx / (1 << lev); }
 
+constexpr double kPi = 3.14159265;  // src/General.h:6
+
+// src/Spherical.cu:15-30 in T, with the reference's mixed precision: y is
+// formed in double ((iy+0.5) is a double) and the last division is by the
+// double 2*dphi.
+template <class T> inline T calc_cm(T Radius, T delta, T yo, int iy) {
+    T y = yo + (iy + 0.5) * delta / Radius * T(180.0 / kPi);
+    T phi = y * T(kPi / 180.0);
+    T dphi = delta / (T(2.0 * Radius));
+    return T((std::sin(phi + dphi) - std::sin(phi - dphi)) / (2.0 * dphi));
+}
+
+// src/Spherical.cu:43-64 in T.
+template <class T> inline T calc_fm(T Radius, T delta, T yo, T iy) {
+    T dy = delta / Radius * T(180.0 / kPi);
+    T y = yo + iy * dy;
+    T phi = y * T(kPi / 180.0);
+    return std::cos(phi);
+}
+
+template <class T> inline T calcres_t(T dx, int lev) { return lev < 0 ? dx * (1 << -lev) : dx / (1 << lev); }
+
+// Metric terms of every block of a spherical grid, in the layout of
+// bgf::MET_*: the X and Y Kurganov faces (src/Kurganov.cu:474-641,
+// 1404-1565) and the tendencies (src/Advection.cu:160-254).  Computed on the
+// host with the same libm as the reference CPU step.
+template <class T>
+inline std::vector<T> metric_table(const bgf_param* p, const bgf_blocks* b, int nblkmem) {
+    std::vector<T> M((size_t)nblkmem * bgf::METRIC_STRIDE, T(0));
+    for (int ibl = 0; ibl < p->nblk; ibl++) {
+        const int ib = b->active[ibl];
+        const int lev = b->level[ib];
+        const T delta = calcres_t(T(p->delta), lev);
+        const T yblk = T(b->yo[ib]);
+        const T ybo_k = T(p->yo + (double)yblk);  // Kurganov: T(XParam.yo + XBlock.yo[ib])
+        const T ybo_e = (T)p->yo + yblk;          // updateEV: (T)XParam.yo + XBlock.yo[ib]
+        const T R = T(p->Radius);
+        T* m = M.data() + (size_t)ib * bgf::METRIC_STRIDE;
+        for (int iy = 0; iy < BW; iy++) m[bgf::MET_CMX + iy] = calc_cm(R, delta, ybo_k, iy);
+        for (int iy = 0; iy <= BW; iy++) {
+            m[bgf::MET_CMY + iy] = calc_cm(R, delta, ybo_k, iy);
+            m[bgf::MET_FMY + iy] = calc_fm(R, delta, ybo_k, T(iy));
+        }
+        const int top = b->TopLeft[ib], bot = b->BotLeft[ib];
+        for (int iy = 0; iy < BW; iy++) {
+            T yup = T(iy) + T(1.0), ydwn = T(iy);
+            if (iy == BW - 1 && b->level[top] > lev) yup = iy + T(0.75);
+            if (iy == 0 && b->level[bot] > lev) ydwn = iy - T(0.25);
+            m[bgf::MET_CME + iy] = calc_cm(R, delta, ybo_e, iy);
+            m[bgf::MET_FMV + iy] = calc_fm(R, delta, ybo_e, ydwn);
+            m[bgf::MET_FMVP + iy] = calc_fm(R, delta, ybo_e, yup);
+            m[bgf::MET_FC + iy] = T(std::sin((ybo_e + calcres_t(T(p->dx), lev) * iy) * kPi / 180.0) * kPi / T(21600.0));
+        }
+    }
+    return M;
+}
+
 inline bgf_status check_param(const bgf_param* p) {
     if (!p) return fail(BGF_ERR_INVALID_ARG, "null param");
     if (p->engine != 2) return fail(BGF_ERR_UNSUPPORTED, "only engine 2 (Kurganov) is provided");
-    if (p->spherical) return fail(BGF_ERR_UNSUPPORTED, "spherical grids are not provided");
+    if (p->spherical && !(p->Radius > 0.0)) return fail(BGF_ERR_INVALID_ARG, "spherical grids need Radius > 0");
     if (p->nblk <= 0 || p->nblkmem < p->nblk) return fail(BGF_ERR_INVALID_ARG, "need 0 < nblk <= nblkmem");
     if (p->precision < 0 || p->precision > 3) return fail(BGF_ERR_INVALID_ARG, "bad precision");
     if (p->aoibnd != 0 && p->aoibnd != 1)

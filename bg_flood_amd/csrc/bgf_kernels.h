@@ -48,7 +48,16 @@ struct KParams {
     // batches of domains: member of each block (DevLoop of that member), member count
     const int* sid;
     int nmember;
+    // spherical grids: per-block rows of metric terms in T (METRIC_STRIDE per block)
+    const void* metric;
+    int spherical;
 };
+
+// Layout of a block's metric terms (spherical grids, src/Spherical.cu:15-64):
+// cm of the X faces by row, cm and fm of the Y faces by face row (17), and for
+// the tendencies cm, fm of the faces below and above, and the Coriolis factor.
+enum { MET_CMX = 0, MET_CMY = 16, MET_FMY = 33, MET_CME = 50, MET_FMV = 66, MET_FMVP = 82, MET_FC = 98,
+       METRIC_STRIDE = 128 };
 
 template <class T> struct StepBuffers {
     const T* h;

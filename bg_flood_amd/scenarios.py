@@ -299,3 +299,41 @@ def ensemble_specs(n=64, seed=7, lmin=256, lmax=2048):
 def ensemble_member(spec, precision="fp32", dx=5.0):
     return rain_terrain(n=spec["n"], dx=dx, amp=spec["amp"], rain=spec["rain"], manning=spec["manning"],
                         seed=spec["seed"], precision=precision)
+
+
+def spherical_basin(nbx=4, nby=4, dx=0.01, lat0=-0.32, lon0=100.0, depth=10.0, amp=1.0, radius=0.05,
+                    lat_c=0.0, precision="fp64", aoibnd=0, Radius=6371220.0, maxlevel=0, refine=None):
+    """A lat/lon grid (dx in degrees) with a Gaussian hump at latitude lat_c over a flat bed: the
+    spherical metric terms (cm, fm) enter every face and the tendencies.  maxlevel > 0 refines
+    towards the hump."""
+    from . import mesh_from_leaves
+    if maxlevel > 0:
+        if refine is None:
+            def refine(l, x, y):
+                s = BW * dx / (1 << l)
+                return math.hypot((x + 0.5) * s - nbx * BW * dx / 2, lat0 + (y + 0.5) * s - lat_c) < (
+                    0.3 if l == 0 else 0.18) * nbx * BW * dx
+        leaves = quadtree_leaves(nbx, nby, maxlevel, refine)
+        blocks = mesh_from_leaves(leaves, nbx, nby, dx)
+        nblk = len(leaves)
+        levels = [b[0] for b in leaves]
+    else:
+        blocks = mesh_uniform(nbx, nby, dx, 0, precision == "fp64")
+        nblk = nbx * nby
+        levels = [0]
+    delta = dx * Radius * 3.14159265 / 180.0  # src/ReadInput.cu:1303 with pi of src/General.h:6
+    p = default_param(dx=dx, delta=delta, xo=lon0, yo=lat0, nblk=nblk, nblkmem=nblkmem_for(nblk),
+                      precision=precision, minlevel=min(levels), maxlevel=max(levels), aoibnd=aoibnd,
+                      frictionmodel=0, cf=0.0, spherical=1, Radius=Radius)
+    T = np.float64 if precision == "fp64" else np.float32
+    S = p["nblkmem"] * BS
+    X, Y = cell_coords_levels(blocks, p["nblkmem"], lon0, lat0, dx, T)
+    zb = np.full(S, T(-depth), T)
+    xc = T(lon0 + nbx * BW * dx / 2)
+    r2 = (X - xc) * (X - xc) + (Y - T(lat_c)) * (Y - T(lat_c))
+    h = (np.maximum(T(0.0), zb) + T(amp) * np.exp(T(-1.0) * r2 / (T(2.0) * T(radius) * T(radius))) - zb).astype(T)
+    inter = np.zeros((p["nblkmem"], MW, MW), bool)
+    inter[:nblk, 1:MW - 1, 1:MW - 1] = True
+    h[~inter.reshape(-1)] = 0
+    z = np.zeros(S, T)
+    return dict(param=p, blocks=blocks, nbx=nbx, nby=nby, zb=zb, h=h, u=z.copy(), v=z.copy(), cf=z.copy())

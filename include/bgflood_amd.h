@@ -55,13 +55,14 @@ typedef struct {
     int aoibnd;           /* Param.aoibnd: remainder boundary type, 0 wall, 1 Neumann */
     int ForceMassConserve;/* Param.ForceMassConserve */
     int wetdryfix;        /* Param.wetdryfix */
-    int spherical;        /* Param.spherical: must be 0 */
+    int spherical;        /* Param.spherical: lat/lon grid, dx in degrees, delta = dx*Radius*pi/180 */
     int nblk;             /* Param.nblk: active blocks */
     int nblkmem;          /* Param.nblkmem: allocated blocks (>= nblk) */
     int minlevel, maxlevel;
     int rain_on;          /* a uniform rain forcing is present (Forcing.Rain) */
     double rain;          /* Rain.nowvalue, mm/h */
     int precision;        /* bgf_precision */
+    double Radius;        /* Param.Radius: sphere radius for spherical grids (6371220 m) */
 } bgf_param;
 
 /* BlockP<T> (src/Arrays.h:233-249): nblkmem entries each; active[ibl] for

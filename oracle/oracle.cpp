@@ -132,7 +132,29 @@ struct Prm {
     int rain_on;
     float rain;
     int nthreads;
+    int spherical;
+    double Radius;
 };
+
+// src/Spherical.cu:15-30: scale factor of the y face length at row iy of a
+// block whose first row is at latitude yo (degrees).  (iy+0.5) is a double,
+// so y is formed in double; the last division is by the double 2.0*dphi.
+template <class T> T calcCM(T Radius, T delta, T yo, int iy) {
+    T y = yo + (iy + 0.5) * delta / Radius * T(180.0 / kPi);
+    T phi = y * T(kPi / 180.0);
+    T dphi = delta / (T(2.0 * Radius));
+    T cm = (std::sin(phi + dphi) - std::sin(phi - dphi)) / (2.0 * dphi);
+    return cm;
+}
+
+// src/Spherical.cu:43-64: cos(latitude) of the face at (fractional) row iy.
+template <class T> T calcFM(T Radius, T delta, T yo, T iy) {
+    T dy = delta / Radius * T(180.0 / kPi);
+    T y = yo + iy * dy;
+    T phi = y * T(kPi / 180.0);
+    T fmu = std::cos(phi);
+    return fmu;
+}
 
 template <class T> struct Blk {
     std::vector<int> active, level, LeftBot, LeftTop, RightBot, RightTop, TopLeft, TopRight, BotLeft, BotRight;
@@ -874,13 +896,15 @@ template <class T> void updateKurgXCPU(Mdl<T>& M, const Ev<T>& E) {
         int ib = B.active[ibl];
         int lev = B.level[ib];
         T delta = calcres(T(P.delta), lev);
+        T ybo = T(P.yo + B.yo[ib]);
         int RB = B.RightBot[ib], levRB = B.level[RB], LBRB = B.LeftBot[RB];
         int LB = B.LeftBot[ib], levLB = B.level[LB], RBLB = B.RightBot[LB];
         for (int iy = 0; iy < BW; iy++)
             for (int ix = 0; ix < BW + HW; ix++) {
                 int i = memloc(ix, iy, ib), ileft = memloc(ix - 1, iy, ib);
                 T dhdxi = M.dhdx[i], dhdxmin = M.dhdx[ileft];
-                T cm = T(1.0), fmu = T(1.0);
+                T cm = P.spherical ? calcCM(T(P.Radius), delta, ybo, iy) : T(1.0);
+                T fmu = T(1.0);
                 T hi = E.h[i], hn = E.h[ileft];
                 if (hi > eps || hn > eps) {
                     T dx, zi, zl, zn, zr, zlr, hl, up, hp, hr, um, hm, ga;
@@ -941,10 +965,12 @@ template <class T> void updateKurgYCPU(Mdl<T>& M, const Ev<T>& E) {
         int BL = B.BotLeft[ib], levBL = B.level[BL], TLBL = B.TopLeft[BL];
         int lev = B.level[ib];
         T delta = T(calcres(P.delta, lev));
+        T ybo = T(P.yo + B.yo[ib]);
         for (int iy = 0; iy < BW + HW; iy++)
             for (int ix = 0; ix < BW; ix++) {
                 int i = memloc(ix, iy, ib), ibot = memloc(ix, iy - 1, ib);
-                T cm = T(1.0), fmv = T(1.0);
+                T cm = P.spherical ? calcCM(T(P.Radius), delta, ybo, iy) : T(1.0);
+                T fmv = P.spherical ? calcFM(T(P.Radius), delta, ybo, T(iy)) : T(1.0);
                 T dhdyi = M.dhdy[i], dhdymin = M.dhdy[ibot];
                 T hi = E.h[i], hn = E.h[ibot];
                 T dx, zi, zl, zn, zr, zlr, hl, up, hp, hr, um, hm, ga;
@@ -1105,11 +1131,23 @@ template <class T> void updateEVCPU(Mdl<T>& M, const Ev<T>& E) {
         int ib = B.active[ibl];
         int lev = B.level[ib];
         T delta = calcres(T(P.delta), lev);
+        T ybo = (T)P.yo + B.yo[ib];
         for (int iy = 0; iy < BW; iy++)
             for (int ix = 0; ix < BW; ix++) {
-                T fc = T(std::sin(T(P.lat * kPi / 180.0)) * kPi / T(21600.0));
+                T fc = P.spherical ? T(std::sin((ybo + calcres(T(P.dx), lev) * iy) * kPi / 180.0) * kPi / T(21600.0))
+                                   : T(std::sin(T(P.lat * kPi / 180.0)) * kPi / T(21600.0));
                 int i = memloc(ix, iy, ib), iright = memloc(ix + 1, iy, ib), itop = memloc(ix, iy + 1, ib);
-                T cm = T(1.0), fmu = T(1.0), fmv = T(1.0), fmup = T(1.0), fmvp = T(1.0);
+                // face rows of the v faces below and above; beside a finer block the
+                // face sits at the fine faces' mean position (src/Advection.cu:195-213)
+                T yup = T(iy) + T(1.0);
+                T ydwn = T(iy);
+                if (iy == BW - 1 && B.level[B.TopLeft[ib]] > B.level[ib]) yup = iy + T(0.75);
+                if (iy == 0 && B.level[B.BotLeft[ib]] > B.level[ib]) ydwn = iy - T(0.25);
+                T cm = P.spherical ? calcCM(T(P.Radius), delta, ybo, iy) : T(1.0);
+                T fmu = T(1.0);
+                T fmv = P.spherical ? calcFM(T(P.Radius), delta, ybo, ydwn) : T(1.0);
+                T fmup = T(1.0);
+                T fmvp = P.spherical ? calcFM(T(P.Radius), delta, ybo, yup) : T(1.0);
                 T hi = E.h[i], uui = E.u[i], vvi = E.v[i];
                 T cmdinv = T(1.0) / (cm * delta);
                 T ga = T(0.5) * g;
@@ -1350,6 +1388,7 @@ Prm to_prm(const orc_param* p) {
     q.wetdryfix = p->wetdryfix; q.nblk = p->nblk; q.nblkmem = p->nblkmem; q.initlevel = p->initlevel;
     q.minlevel = p->minlevel; q.maxlevel = p->maxlevel; q.nx = p->nx; q.ny = p->ny;
     q.rain_on = p->rain_on; q.rain = p->rain; q.nthreads = p->nthreads;
+    q.spherical = p->spherical; q.Radius = p->Radius;
     return q;
 }
 
@@ -1536,6 +1575,8 @@ float orc_kurgsolver_f(float g, float delta, float epsi, float CFL, float cm, fl
     return KurgSolver(g, delta, epsi, CFL, cm, fm, hp, hm, up, um, *fh, *fu);
 }
 double orc_calcres_d(double dx, int level) { return calcres(dx, level); }
+double orc_calccm_d(double Radius, double delta, double yo, int iy) { return calcCM(Radius, delta, yo, iy); }
+double orc_calcfm_d(double Radius, double delta, double yo, double iy) { return calcFM(Radius, delta, yo, iy); }
 int orc_ftoi_d(double v) { return ftoi(v); }
 int orc_memloc(int i, int j, int ib) { return memloc(i, j, ib); }
 double orc_timestep_d(double dtmin_cells, const orc_loop* loop) { return timestep_clamp(dtmin_cells, *loop); }

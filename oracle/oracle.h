@@ -33,6 +33,8 @@ typedef struct {
     float rain;               /* Rain.nowvalue in mm/h */
     int doubleprecision;      /* 1: T=double, 0: T=float */
     int nthreads;             /* host threads for per-variable halo/gradient work (reference: 4) */
+    int spherical;            /* lat/lon grid: dx in degrees, delta = dx*Radius*pi/180 */
+    double Radius;            /* sphere radius (Param.Radius) */
 } orc_param;
 
 /* Block description, BlockP<T> (src/Arrays.h:233-249); arrays of nblkmem ints/doubles. */
@@ -72,6 +74,9 @@ double orc_kurgsolver_d(double g, double delta, double epsi, double CFL, double 
 float  orc_kurgsolver_f(float g, float delta, float epsi, float CFL, float cm, float fm,
                         float hp, float hm, float up, float um, float* fh, float* fu);
 double orc_calcres_d(double dx, int level);
+/* Spherical metric terms (src/Spherical.cu:15-64). */
+double orc_calccm_d(double Radius, double delta, double yo, int iy);
+double orc_calcfm_d(double Radius, double delta, double yo, double iy);
 int    orc_ftoi_d(double v);
 int    orc_memloc(int i, int j, int ib);
 double orc_timestep_d(double dtmin_cells, const orc_loop* loop);

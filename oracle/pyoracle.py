@@ -33,6 +33,7 @@ class OrcParam(ctypes.Structure):
         ("nx", ctypes.c_int), ("ny", ctypes.c_int),
         ("rain_on", ctypes.c_int), ("rain", ctypes.c_float),
         ("doubleprecision", ctypes.c_int), ("nthreads", ctypes.c_int),
+        ("spherical", ctypes.c_int), ("Radius", ctypes.c_double),
     ]
 
 
@@ -88,6 +89,10 @@ def lib():
         L.orc_kurgsolver_f.argtypes = [ctypes.c_float] * 10 + [fp, fp]
         L.orc_calcres_d.restype = ctypes.c_double
         L.orc_calcres_d.argtypes = [ctypes.c_double, ctypes.c_int]
+        L.orc_calccm_d.restype = ctypes.c_double
+        L.orc_calccm_d.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.orc_calcfm_d.restype = ctypes.c_double
+        L.orc_calcfm_d.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.orc_ftoi_d.restype = ctypes.c_int
         L.orc_ftoi_d.argtypes = [ctypes.c_double]
         L.orc_memloc.restype = ctypes.c_int
@@ -114,7 +119,7 @@ def make_param(**kw):
                     lat=0.0, zsinit=0.0, zsoffset=0.0, dtinit=-1.0, mask=9999.0, frictionmodel=0,
                     aoibnd=0, ForceMassConserve=0, wetdryfix=1, nblk=0, nblkmem=0, initlevel=0,
                     minlevel=0, maxlevel=0, nx=0, ny=0, rain_on=0, rain=0.0, doubleprecision=1,
-                    nthreads=4)
+                    nthreads=4, spherical=0, Radius=6371220.0)
     defaults.update(kw)
     for k, v in defaults.items():
         setattr(p, k, v)

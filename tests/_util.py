@@ -12,6 +12,7 @@ def oracle_for(sc, precision=None, nthreads=4):
     p = sc["param"]
     prec = precision or p["precision"]
     kw = {k: p[k] for k in ORACLE_KEYS}
+    kw.update(spherical=p.get("spherical", 0), Radius=p.get("Radius", 6371220.0))
     kw.update(cf=p.get("cf", 0.0), nx=sc["nbx"] * 16, ny=sc["nby"] * 16, rain_on=p["rain_on"], rain=p["rain"],
               doubleprecision=1 if prec == "fp64" else 0, nthreads=nthreads, dtinit=p.get("dtinit", -1.0))
     op = po.make_param(**kw)

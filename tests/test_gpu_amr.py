@@ -124,3 +124,14 @@ def test_run_to_endtime_runs_the_last_corrector(amr):
     sa, sb = a.get_state(), b.get_state()
     for k in ("h", "u", "v"):
         assert np.array_equal(sa[k], sb[k]), k
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("maxlevel,aoibnd", [(0, 0), (0, 1), (2, 0)])
+def test_spherical_exact(prec, maxlevel, aoibnd):
+    """Spherical grids (metric terms per row, computed on the host with the reference's formulas)
+    on one level and on a quadtree."""
+    sc = scenarios.spherical_basin(lat0=25.0, lat_c=25.3, dx=0.01, precision=prec, aoibnd=aoibnd,
+                                   maxlevel=maxlevel)
+    d, o = run_both(sc, 30, prec)
+    assert_same(sc, d, o)
