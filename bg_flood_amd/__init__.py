@@ -331,7 +331,7 @@ class Batch:
         """Adds a member; returns its id."""
         nbm = int(param["nblkmem"])
         keep = {}
-        P = _param_struct(param)
+        P = _param_struct(dict(param, precision=self.precision))
         B = _blocks_struct(blocks, nbm, keep)
         arrs = [np.ascontiguousarray(x, dtype=self.dtype) for x in (zb, h, u, v, cf)]
         for a in arrs:
