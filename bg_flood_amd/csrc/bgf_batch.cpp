@@ -61,6 +61,11 @@ struct bgf_batch_s {
     int wg_pred = 0, wg_corr = 0;
     bool use_graph = true;
     hipGraphExec_t graph = nullptr;  // an even/odd pair of batch steps; rebuilt after membership changes
+    // the persistent kernels walk only the blocks of members still running:
+    // their processing order, launch parameters, and the members it was built for
+    bgf::KParams krun;
+    std::vector<int> run_members;
+    bool run_valid = false;
 };
 
 namespace {
@@ -102,6 +107,7 @@ bgf_status rebuild(bgf_batch b) {
         for (int k = 0; k < M.nblkmem; k++) sid[M.off + k] = m;
     }
     b->nactive_blocks = (int)active.size();
+    b->run_valid = false;
     if (b->graph) {
         (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
@@ -112,10 +118,32 @@ bgf_status rebuild(bgf_batch b) {
     }
     HIPCHK(hipMemcpy(b->d_sid, sid.data(), sizeof(int) * sid.size(), hipMemcpyHostToDevice));
     b->kp.nblk = b->nactive_blocks;
-    if (b->nactive_blocks > 0) {
-        b->kp.grid_pred = bgf::persistent_grid(b->prec, 0, b->nactive_blocks, b->wg_pred);
-        b->kp.grid_corr = bgf::persistent_grid(b->prec, 1, b->nactive_blocks, b->wg_corr);
-        if (b->kp.grid_pred <= 0 || b->kp.grid_corr <= 0) return fail(BGF_ERR_HIP, "occupancy query failed");
+    return BGF_OK;
+}
+
+// Processing order over the members that have not reached their end time,
+// rebuilt when that set changes (finished members cost nothing afterwards).
+bgf_status prepare_run(bgf_batch b) {
+    std::vector<int> running;
+    for (int m = 0; m < b->cap_members; m++)
+        if (b->members[m].used && !b->loops[m].done) running.push_back(m);
+    if (b->run_valid && running == b->run_members) return BGF_OK;
+    std::vector<int> order;
+    for (int m : running) order.insert(order.end(), b->members[m].order.begin(), b->members[m].order.end());
+    if (!order.empty())
+        HIPCHK(hipMemcpy(b->d_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
+    b->krun = b->kp;
+    b->krun.nblk = (int)order.size();
+    if (!order.empty()) {
+        b->krun.grid_pred = bgf::persistent_grid(b->prec, 0, b->krun.nblk, b->wg_pred);
+        b->krun.grid_corr = bgf::persistent_grid(b->prec, 1, b->krun.nblk, b->wg_corr);
+        if (b->krun.grid_pred <= 0 || b->krun.grid_corr <= 0) return fail(BGF_ERR_HIP, "occupancy query failed");
+    }
+    b->run_members = running;
+    b->run_valid = true;
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
     }
     return BGF_OK;
 }
@@ -158,7 +186,7 @@ void enqueue_steps(bgf_batch b, int nsteps) {
                   (float*)b->d_h[out], (float*)b->d_u[out], (float*)b->d_v[out], (const float*)b->d_zb,
                   (const float*)b->d_cf, (float*)b->d_dh, (float*)b->d_dhu, (float*)b->d_dhv, (float*)b->d_ch};
         }
-        bgf::launch_step(b->prec, b->kp, storage, b->d_loop, 1, b->stream, nullptr);
+        bgf::launch_step(b->prec, b->krun, storage, b->d_loop, 1, b->stream, nullptr);
     }
 }
 
@@ -371,9 +399,11 @@ bgf_status bgf_batch_run_async(bgf_batch b, int nsteps) {
             L.slot64[k] = 0x7ff0000000000000ull;
         }
     }
+    bgf_status s = prepare_run(b);
+    if (s != BGF_OK) return s;
     HIPCHK(hipMemcpyAsync(b->d_loop, b->loops.data(), sizeof(bgf::DevLoop) * b->cap_members, hipMemcpyHostToDevice,
                           b->stream));
-    if (b->nactive_blocks > 0) {
+    if (b->krun.nblk > 0) {
         if (b->use_graph && nsteps >= 4) {
             if (!b->graph) {
                 hipGraph_t g = nullptr;
