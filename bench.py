@@ -99,9 +99,10 @@ def cpu_baseline(d, sc, window=2048, nsteps=24):
                        % (window, window, nsteps, secs))
 
 
-def ensemble_bench(nmem=64, seed=7, horizon=7200.0, chunk=50, budget_s=90.0, plain_steps=8):
+def ensemble_bench(nmem=64, seed=7, horizon=7200.0, chunk=50, budget_s=90.0, plain_steps=8, prec="fp32_comp"):
     """The batched rain-on-terrain ensemble: nmem members (side log-uniform 256..2048 cells,
-    seeded bed 10-40 m, rain 10-100 mm/h, Manning 0.02-0.05), FP32 fast, each run to its own
+    seeded bed 10-40 m, rain 10-100 mm/h, Manning 0.02-0.05), FP32 with compensated depth updates
+    and the deviation-form reconstruction (fp32_comp), each run to its own
     horizon with its own dt in one batch.  Returns throughput, per-member completion latency and the
     plain one-domain-per-launch baseline on the same members."""
     import numpy as np
@@ -112,7 +113,7 @@ def ensemble_bench(nmem=64, seed=7, horizon=7200.0, chunk=50, budget_s=90.0, pla
     mem = [scenarios.ensemble_member(sp, precision="fp32") for sp in specs]
     log("ensemble: %d members, %d cells, built in %.1f s" % (nmem, sum(m["param"]["nblk"] for m in mem) * 256,
                                                              time.time() - t))
-    common = dict(mem[0]["param"], precision="fp32_fast")
+    common = dict(mem[0]["param"], precision=prec)
     B = bgf.Batch(common, max_blocks=sum(m["param"]["nblkmem"] for m in mem), max_members=nmem)
     ids = [B.add(m["param"], m["blocks"], m["zb"], m["h"], m["u"], m["v"], m["cf"], endtime=horizon) for m in mem]
     gpu_sync()
@@ -139,7 +140,7 @@ def ensemble_bench(nmem=64, seed=7, horizon=7200.0, chunk=50, budget_s=90.0, pla
     t_step = t_run = 0.0
     cu = 0
     for m in mem:
-        d = bgf.Domain(dict(m["param"], precision="fp32_fast"), m["blocks"])
+        d = bgf.Domain(dict(m["param"], precision=prec), m["blocks"])
         d.set_state(m["zb"], m["h"], m["u"], m["v"], m["cf"])
         L = bgf.Loop(nextoutputtime=horizon, dt=0, dtmax=0, totaltime=0)
         d.init_dt(L)
@@ -160,7 +161,7 @@ def ensemble_bench(nmem=64, seed=7, horizon=7200.0, chunk=50, budget_s=90.0, pla
     plain_run = cu / t_run / 1e9
     log("ensemble plain: bgf_step %.2f Gcell/s, bgf_run %.2f Gcell/s" % (plain_step, plain_run))
     pct = lambda q: round(float(np.percentile(done, q)), 3) if done else None
-    return dict(members=nmem, cells=sum(m["param"]["nblk"] for m in mem) * 256, horizon_s=horizon,
+    return dict(members=nmem, precision=prec, cells=sum(m["param"]["nblk"] for m in mem) * 256, horizon_s=horizon,
                 batch_gcups=round(gcups, 3), wall_s=round(wall, 2), batch_steps=steps, members_done=len(done),
                 latency_p50_s=pct(50), latency_p99_s=pct(99), min_sim_time_s=round(min(sim), 1),
                 plain_step_gcups=round(plain_step, 3), plain_run_gcups=round(plain_run, 3),

@@ -101,6 +101,18 @@ template <class T> __device__ __forceinline__ T minmod2c(T th, T half, T s0, T s
     return dmax(mn, T(0.0)) + dmin(mx, T(0.0));
 }
 
+// minmod2c of the sums z + h from the differences of each part: the slope of
+// zs = zb + h without forming zs, whose FP32 rounding (ulp of the bed
+// elevation) would swamp millimetre depths (deviation form).
+template <class T> __device__ __forceinline__ T minmod2s(T th, T half, T z0, T z1, T z2, T h0, T h1, T h2) {
+    T d1 = th * ((z1 - z0) + (h1 - h0));
+    T d2 = ((z2 - z0) + (h2 - h0)) * half;
+    T d3 = th * ((z2 - z1) + (h2 - h1));
+    T mn = dmin(dmin(d1, d2), d3);
+    T mx = dmax(dmax(d1, d2), d3);
+    return dmax(mn, T(0.0)) + dmin(mx, T(0.0));
+}
+
 // Constants of the step in T, computed once on the host (make_cst) and
 // passed by value, so kernels read them from scalar registers.
 template <class T> struct Cst {
@@ -140,13 +152,17 @@ template <class T, bool FAST> Cst<T> make_cst(const KParams& P) {
 // and 1865-1911.  Along Y the roles of u and v swap.  si/sn hold the cells'
 // limited slopes of (h, zs, un, ut): gradients in EXACT kernels, half-cell
 // increments in FAST ones.
-template <class T, bool FAST, bool WANT_DT>
+//
+// DEV (deviation form, FAST only): zsi/zsn carry the bed zb itself and the
+// interface depths use zl - zr = (zb_i - zb_n) - (zi - zl) - (zr - zn), so no
+// absolute elevation is rounded into a depth.
+template <class T, bool FAST, bool WANT_DT, bool DEV = false>
 __device__ __forceinline__ bool kurg_face(const Cst<T>& C, T hi, T hn, T zsi, T zsn, T ui, T un, T vi, T vn,
                                           const Q4<T>& si, const Q4<T>& sn, Q4<T>& F, T& dt) {
     typedef Ar<T, FAST> A;
     const bool wet = hi > C.eps_w || hn > C.eps_w;
-    T zi = zsi - hi;
-    T zn = zsn - hn;
+    T zi = DEV ? zsi : zsi - hi;
+    T zn = DEV ? zsn : zsn - hn;
     T zl, zr, hl, up, hr, um, dzi, dzn;
     if (FAST) {
         dzi = si.b - si.a;  // zi - zl
@@ -168,9 +184,16 @@ __device__ __forceinline__ bool kurg_face(const Cst<T>& C, T hi, T hn, T zsi, T 
         dzi = zi - zl;
         dzn = zr - zn;
     }
-    T zlr = dmax(zl, zr);
-    T hp = dmax(T(0.0), hl + zl - zlr);
-    T hm = dmax(T(0.0), hr + zr - zlr);
+    T hp, hm;
+    if (DEV) {
+        const T dlr = (zi - zn) - dzi - dzn;  // zl - zr
+        hp = dmax(T(0.0), hl + dmin(dlr, T(0.0)));
+        hm = dmax(T(0.0), hr + dmin(-dlr, T(0.0)));
+    } else {
+        T zlr = dmax(zl, zr);
+        hp = dmax(T(0.0), hl + zl - zlr);
+        hm = dmax(T(0.0), hr + zr - zlr);
+    }
     // KurgSolver
     T cp = A::sqrt_(C.g * hp);
     T cmo = A::sqrt_(C.g * hm);
@@ -250,13 +273,14 @@ __device__ __forceinline__ int strip_src(int k, int ib, int nL, int nR, int nB, 
     return memloc(j, sy, nb);
 }
 
-template <class T, bool FAST>
+template <class T, bool FAST, bool DEV = false>
 __device__ __forceinline__ Q4<T> slopes(const Cst<T>& C, const Q4<T>& m, const Q4<T>& c, const Q4<T>& p) {
     Q4<T> g;
     if (FAST) {
         // half-cell increments 0.5*minmod2: coefficients theta/2 and 1/4
         g.a = minmod2c(C.th_s, C.half_s, m.a, c.a, p.a);
-        g.b = minmod2c(C.th_s, C.half_s, m.b, c.b, p.b);
+        g.b = DEV ? minmod2s(C.th_s, C.half_s, m.b, c.b, p.b, m.a, c.a, p.a)
+                  : minmod2c(C.th_s, C.half_s, m.b, c.b, p.b);
         g.c = minmod2c(C.th_s, C.half_s, m.c, c.c, p.c);
         g.d = minmod2c(C.th_s, C.half_s, m.d, c.d, p.d);
     } else {
@@ -270,14 +294,15 @@ __device__ __forceinline__ Q4<T> slopes(const Cst<T>& C, const Q4<T>& m, const Q
 
 // Slopes of the tile: every thread its own cell along x and y, wave 0 the
 // halo ring (x slopes at x = -1, 16 and y slopes at y = -1, 16).
-template <class T, bool FAST> __device__ __forceinline__ void tile_gradients(Tile<T>& s, const Cst<T>& C, int t) {
+template <class T, bool FAST, bool DEV>
+__device__ __forceinline__ void tile_gradients(Tile<T>& s, const Cst<T>& C, int t) {
     {
         const int x = t & 15, y = t >> 4;
         const int c = tix(x, y);
         const Q4<T> q0 = s.q[c];
-        s.gx[gxi(x, y)] = slopes<T, FAST>(C, s.q[c - 1], q0, s.q[c + 1]);
+        s.gx[gxi(x, y)] = slopes<T, FAST, DEV>(C, s.q[c - 1], q0, s.q[c + 1]);
         Q4<T> m = s.q[c - TP], p = s.q[c + TP];
-        Q4<T> g = slopes<T, FAST>(C, m, q0, p);
+        Q4<T> g = slopes<T, FAST, DEV>(C, m, q0, p);
         Q4<T> gs;  // y slopes stored as (h, zs, v, u)
         gs.a = g.a; gs.b = g.b; gs.c = g.d; gs.d = g.c;
         s.gy[gyi(x, y)] = gs;
@@ -287,11 +312,11 @@ template <class T, bool FAST> __device__ __forceinline__ void tile_gradients(Til
         if (side < 2) {
             const int x = side == 0 ? -1 : BW;
             const int c = tix(x, j);
-            s.gx[gxi(x, j)] = slopes<T, FAST>(C, s.q[c - 1], s.q[c], s.q[c + 1]);
+            s.gx[gxi(x, j)] = slopes<T, FAST, DEV>(C, s.q[c - 1], s.q[c], s.q[c + 1]);
         } else {
             const int y = side == 2 ? -1 : BW;
             const int c = tix(j, y);
-            Q4<T> g = slopes<T, FAST>(C, s.q[c - TP], s.q[c], s.q[c + TP]);
+            Q4<T> g = slopes<T, FAST, DEV>(C, s.q[c - TP], s.q[c], s.q[c + TP]);
             Q4<T> gs;
             gs.a = g.a; gs.b = g.b; gs.c = g.d; gs.d = g.c;
             s.gy[gyi(j, y)] = gs;
@@ -299,25 +324,25 @@ template <class T, bool FAST> __device__ __forceinline__ void tile_gradients(Til
     }
 }
 
-template <class T, bool FAST, bool WANT_DT>
+template <class T, bool FAST, bool WANT_DT, bool DEV>
 __device__ __forceinline__ bool xface(Tile<T>& s, const Cst<T>& C, int xf, int y, T& dt) {
     const int ci = tix(xf, y);
     const Q4<T> qi = s.q[ci], qn = s.q[ci - 1];
     const int gi = gxi(xf, y);
     Q4<T> F;
-    bool wet = kurg_face<T, FAST, WANT_DT>(C, qi.a, qn.a, qi.b, qn.b, qi.c, qn.c, qi.d, qn.d, s.gx[gi], s.gx[gi - 1],
+    bool wet = kurg_face<T, FAST, WANT_DT, DEV>(C, qi.a, qn.a, qi.b, qn.b, qi.c, qn.c, qi.d, qn.d, s.gx[gi], s.gx[gi - 1],
                                            F, dt);
     s.fx[fxi(xf, y)] = F;
     return wet;
 }
 
-template <class T, bool FAST, bool WANT_DT>
+template <class T, bool FAST, bool WANT_DT, bool DEV>
 __device__ __forceinline__ bool yface(Tile<T>& s, const Cst<T>& C, int x, int yf, T& dt) {
     const int ci = tix(x, yf);
     const Q4<T> qi = s.q[ci], qn = s.q[ci - TP];
     const int gi = gyi(x, yf);
     Q4<T> F;
-    bool wet = kurg_face<T, FAST, WANT_DT>(C, qi.a, qn.a, qi.b, qn.b, qi.d, qn.d, qi.c, qn.c, s.gy[gi], s.gy[gi - GYP],
+    bool wet = kurg_face<T, FAST, WANT_DT, DEV>(C, qi.a, qn.a, qi.b, qn.b, qi.d, qn.d, qi.c, qn.c, s.gy[gi], s.gy[gi - GYP],
                                            F, dt);
     s.fy[fyi(x, yf)] = F;
     return wet;
@@ -331,17 +356,17 @@ __device__ __forceinline__ bool yface(Tile<T>& s, const Cst<T>& C, int x, int yf
 // cell's left-face Fq is replaced by its right-face S.  Returns the cell's
 // dtmax: its X face then its Y face, a dry face storing 1/epsi outright
 // (src/Kurganov.cu:102-105, 159, 1488-1491, 1537).
-template <class T, bool FAST, bool WANT_DT>
+template <class T, bool FAST, bool WANT_DT, bool DEV>
 __device__ __forceinline__ T tile_fluxes(Tile<T>& s, const Cst<T>& C, int t, bool wl, bool wr, bool wb, bool wt,
                                          T& dh, T& dhu, T& dhv) {
-    tile_gradients<T, FAST>(s, C, t);
+    tile_gradients<T, FAST, DEV>(s, C, t);
     __syncthreads();
     const int x = t & 15, y = t >> 4;
     T dtx = T(0.0), dty = T(0.0), dtd;
-    xface<T, FAST, WANT_DT>(s, C, x, y, dtx);
-    bool wy = yface<T, FAST, WANT_DT>(s, C, x, y, dty);
-    if (t < 16) xface<T, FAST, false>(s, C, BW, t, dtd);
-    else if (t < 32) yface<T, FAST, false>(s, C, t - 16, BW, dtd);
+    xface<T, FAST, WANT_DT, DEV>(s, C, x, y, dtx);
+    bool wy = yface<T, FAST, WANT_DT, DEV>(s, C, x, y, dty);
+    if (t < 16) xface<T, FAST, false, DEV>(s, C, BW, t, dtd);
+    else if (t < 32) yface<T, FAST, false, DEV>(s, C, t - 16, BW, dtd);
     T dtc = T(0.0);
     if (WANT_DT) dtc = wy ? (dty < dtx ? dty : dtx) : C.inv_epsi;
     __syncthreads();
@@ -452,7 +477,7 @@ __device__ __forceinline__ Sched schedule(int w, int nwg, int n) {
 // previous one.  One atomic min per workgroup (per member and workgroup in a
 // batch, whose processing order keeps each member's blocks together; blocks
 // of members that reached their end time are skipped).
-template <class T, bool FAST, bool BATCH>
+template <class T, bool FAST, bool BATCH, bool DEV>
 __global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuffers<T> B, DevLoop* L) {
     if (!BATCH && L->done) return;
     __shared__ Tile<T> s;
@@ -480,11 +505,11 @@ __global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuff
     for (int ibl = S.first; ibl < S.end; ibl += S.stride) {
         {
             Q4<T> q;
-            q.a = ph; q.b = pz + ph; q.c = pu; q.d = pv;
+            q.a = ph; q.b = DEV ? pz : pz + ph; q.c = pu; q.d = pv;
             s.q[tix(x, y)] = q;
             if (t < 128) {
                 Q4<T> r;
-                r.a = sh; r.b = sz + sh; r.c = su; r.d = sv;
+                r.a = sh; r.b = DEV ? sz : sz + sh; r.c = su; r.d = sv;
                 s.q[cs] = r;
             }
         }
@@ -503,7 +528,7 @@ __global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuff
             if (ldu(&L[n.sid].done, 0)) continue;
         }
         T dh, dhu, dhv;
-        T dtc = tile_fluxes<T, FAST, true>(s, C, t, wall && n.nL == n.ib, wall && n.nR == n.ib, wall && n.nB == n.ib,
+        T dtc = tile_fluxes<T, FAST, true, DEV>(s, C, t, wall && n.nL == n.ib, wall && n.nR == n.ib, wall && n.nB == n.ib,
                                            wall && n.nT == n.ib, dh, dhu, dhv);
         const int i = memloc(x, y, n.ib);
         B.dh[i] = dh;
@@ -577,12 +602,12 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
         {
             Q4<T> q;
             advance<T, FAST>(C, fmc, hdt, h0, u0, v0, pdh, pdhu, pdhv, q.a, q.c, q.d);
-            q.b = zb0 + q.a;
+            q.b = COMP ? zb0 : zb0 + q.a;
             s.q[tix(x, y)] = q;
             if (t < 128) {
                 Q4<T> r;
                 advance<T, FAST>(C, fmc, hdt, sh, su, sv, sdh, sdhu, sdhv, r.a, r.c, r.d);
-                r.b = sz + r.a;
+                r.b = COMP ? sz : sz + r.a;
                 s.q[cs] = r;
             }
         }
@@ -594,7 +619,7 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
         }
         const int ib = n.ib, nL = n.nL, nR = n.nR, nB = n.nB, nT = n.nT;
         T dh, dhu, dhv;
-        (void)tile_fluxes<T, FAST, false>(s, C, t, wall && nL == ib, wall && nR == ib, wall && nB == ib,
+        (void)tile_fluxes<T, FAST, false, COMP>(s, C, t, wall && nL == ib, wall && nR == ib, wall && nB == ib,
                                           wall && nT == ib, dh, dhu, dhv);
         T h1, u1, v1;
         const int i = memloc(x, y, ib);
@@ -779,7 +804,7 @@ void launch_step_t(const KParams& P, const StepBuffers<T>& B, DevLoop* L, int ad
                    hipEvent_t* ev) {
     if (ev) (void)hipEventRecord(ev[0], st);
     const Cst<T> C = make_cst<T, FAST>(P);
-    hipLaunchKernelGGL((k_predictor<T, FAST, BATCH>), dim3(P.grid_pred), dim3(256), 0, st, P, C, B, L);
+    hipLaunchKernelGGL((k_predictor<T, FAST, BATCH, COMP>), dim3(P.grid_pred), dim3(256), 0, st, P, C, B, L);
     if (ev) (void)hipEventRecord(ev[1], st);
     hipLaunchKernelGGL((k_reduce_dt<T>), dim3(BATCH ? P.nmember : 1), dim3(64), 0, st, L, advance_time);
     if (ev) (void)hipEventRecord(ev[2], st);
@@ -826,15 +851,15 @@ int persistent_grid(int precision, int corrector, int nblk, int wg_per_cu) {
     const int w = wg_per_cu;
     if (precision == 0)
         return corrector ? grid_for(k_corrector<double, false, false, false>, nblk, w)
-                         : grid_for(k_predictor<double, false, false>, nblk, w);
+                         : grid_for(k_predictor<double, false, false, false>, nblk, w);
     if (precision == 1)
         return corrector ? grid_for(k_corrector<float, false, false, false>, nblk, w)
-                         : grid_for(k_predictor<float, false, false>, nblk, w);
+                         : grid_for(k_predictor<float, false, false, false>, nblk, w);
     if (precision == 2)
         return corrector ? grid_for(k_corrector<float, true, false, false>, nblk, w)
-                         : grid_for(k_predictor<float, true, false>, nblk, w);
+                         : grid_for(k_predictor<float, true, false, false>, nblk, w);
     return corrector ? grid_for(k_corrector<float, true, false, true>, nblk, w)
-                     : grid_for(k_predictor<float, true, false>, nblk, w);
+                     : grid_for(k_predictor<float, true, false, true>, nblk, w);
 }
 
 void launch_init_dt(int precision, const KParams& P, const void* h, DevLoop* L, hipStream_t st) {
