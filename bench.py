@@ -17,6 +17,7 @@ runs rain cases) on a 2048 x 2048 window of the warmed state for 24 steps (about
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -171,6 +172,64 @@ def ensemble_bench(nmem=64, seed=7, horizon=7200.0, chunk=50, budget_s=90.0, pla
                      "checked every %d batch steps" % chunk)
 
 
+def serving_bench(lam=4.0, duration=20.0, seed=11, compress=1.0 / 60.0, lmin=256, lmax=4096, frac_amr=0.2,
+                  prec="fp32_comp", max_blocks=200000, max_members=128, chunk=40):
+    """Serving: a seeded Poisson stream of solve requests (rate lam per second of wall time for
+    `duration` seconds), 80 % rain on terrain (side log-uniform lmin..lmax cells, seeded bed crop,
+    rain and Manning n) and 20 % AMR tsunami (3 levels, seeded wave amplitude 0.5-3 m), horizons
+    U[0.5, 6] h of simulated time scaled by `compress` so that the stream drains within minutes.
+    Each result is a max-depth grid and 16 gauge series.  Reports latency percentiles (arrival
+    to result) and the cell-update rate of the whole stream."""
+    import numpy as np
+    from bg_flood_amd import scenarios, serving
+    rng = np.random.default_rng(seed)
+    t = time.time()
+    arr = []
+    a = 0.0
+    while True:
+        a += rng.exponential(1.0 / lam)
+        if a > duration:
+            break
+        arr.append(a)
+    base = scenarios.value_noise(lmax, lmax, seed)
+    reqs = []
+    for k in range(len(arr)):
+        horizon = rng.uniform(0.5, 6.0) * 3600.0 * compress
+        if rng.random() < frac_amr:
+            sc = scenarios.tsunami(amp=float(rng.uniform(0.5, 3.0)), seed=int(rng.integers(1 << 30)), precision="fp32")
+        else:
+            n = int(round(math.exp(rng.uniform(math.log(lmin), math.log(lmax))) / 16)) * 16
+            ox, oy = [int(v) for v in rng.integers(0, lmax - n + 1, 2)]
+            amp, rain, man = rng.uniform(10, 40), rng.uniform(10, 100), rng.uniform(0.02, 0.05)
+            p, blocks, nbx, nby = scenarios.uniform(n, n, 5.0, "fp32", aoibnd=1, frictionmodel=-1, cf=man,
+                                                    rain_on=1, rain=rain)
+            bed = amp * base[oy:oy + n, ox:ox + n] + 0.002 * ((np.arange(n) + 0.5) * 5.0)[None, :]
+            zb = scenarios.buq_from_grid(bed.astype(np.float32), p["nblkmem"], np.float32)
+            S = p["nblkmem"] * 324
+            sc = dict(param=p, blocks=blocks, nbx=nbx, nby=nby, zb=zb, h=np.zeros(S, np.float32),
+                      u=np.zeros(S, np.float32), v=np.zeros(S, np.float32), cf=np.full(S, np.float32(man)))
+        nb = sc["param"]["nblk"]
+        gb = rng.integers(0, nb, 16)
+        gi = rng.integers(0, 16, (16, 2))
+        cells = [int((gi[g, 0] + 1) + (gi[g, 1] + 1) * 18 + sc["blocks"]["active"][gb[g]] * 324) for g in range(16)]
+        reqs.append(serving.Request(k, sc, horizon, gauges=cells, gauge_every=10))
+    log("serving: %d requests generated in %.1f s" % (len(reqs), time.time() - t))
+    common = dict(next(r.sc["param"] for r in reqs if not r.multilevel), precision=prec, rain=0.0)
+    srv, wall = serving.serve(reqs, np.array(arr), common, max_blocks, max_members, chunk=chunk, time_limit=300.0)
+    lat = np.array([r.latency for r in reqs if r.latency is not None])
+    cu = srv.stats()["cell_updates"] + srv.cell_updates_amr
+    out = dict(requests=len(reqs), completed=len(lat), amr_requests=sum(r.multilevel for r in reqs), lam_per_s=lam,
+               arrival_window_s=duration, wall_s=round(wall, 2), horizon_scale=round(compress, 5),
+               latency_p50_s=round(float(np.percentile(lat, 50)), 3) if len(lat) else None,
+               latency_p99_s=round(float(np.percentile(lat, 99)), 3) if len(lat) else None,
+               gcups=round(cu / wall / 1e9, 3), precision=prec,
+               note="horizons U[0.5,6] h scaled by horizon_scale; rain sides %d-%d cells; batch capacity %d blocks"
+                    % (lmin, lmax, max_blocks))
+    srv.close()
+    log("serving: %s" % out)
+    return out
+
+
 def load_traffic(name):
     path = os.path.join(ROOT, "profiles", "traffic_%s.json" % name)
     if os.path.exists(path):
@@ -202,6 +261,8 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-only", action="store_true")
+    ap.add_argument("--serving-only", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (scale reference for HBM byte counters)")
     args = ap.parse_args()
@@ -215,6 +276,9 @@ def main():
 
     if args.ensemble_only:
         print(json.dumps(ensemble_bench()), flush=True)
+        return
+    if args.serving_only:
+        print(json.dumps(serving_bench()), flush=True)
         return
 
     from bg_flood_amd import scenarios
@@ -264,6 +328,12 @@ def main():
             ens = ensemble_bench()
         except Exception as e:  # report, never hide
             ens = dict(error=repr(e))
+    srv = None
+    if not args.no_serving:
+        try:
+            srv = serving_bench()
+        except Exception as e:  # report, never hide
+            srv = dict(error=repr(e))
 
     out = {
         "metric": "Gcell-updates/s + %HBM roofline; batched-ensemble p50/p99 scenario latency",
@@ -287,6 +357,7 @@ def main():
         "cpu_baseline": cpu,
         "variants": variants,
         "ensemble": ens,
+        "serving": srv,
     }
     print(json.dumps(out), flush=True)
 

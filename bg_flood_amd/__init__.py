@@ -93,6 +93,7 @@ def lib():
         L.bgf_run_async.argtypes = [vp, ctypes.POINTER(Loop), ctypes.c_int, ctypes.c_double]
         L.bgf_sync.argtypes = [vp, ctypes.POINTER(Loop)]
         L.bgf_volume.argtypes = [vp, dp]
+        L.bgf_last_run_steps.argtypes = [vp, ip]
         L.bgf_device_array.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp)]
         L.bgf_set_timing.argtypes = [vp, ctypes.c_int]
         L.bgf_kernel_times.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), dp, ip]
@@ -112,16 +113,19 @@ def lib():
         L.bgf_batch_volume.argtypes = [vp, ctypes.c_int, dp]
         L.bgf_batch_update_maxdepth.argtypes = [vp]
         L.bgf_batch_get_maxdepth.argtypes = [vp, ctypes.c_int, vp]
-        L.bgf_batch_set_gauges.argtypes = [vp, ctypes.c_int, ctypes.c_int, ip]
+        L.bgf_batch_set_gauges.argtypes = [vp, ctypes.c_int, ctypes.c_int, ip, ctypes.c_int]
+        L.bgf_batch_get_gauges.argtypes = [vp, ctypes.c_int, vp, vp, ip]
+        L.bgf_batch_track_maxdepth.argtypes = [vp, ctypes.c_int]
         L.bgf_batch_sample_gauges.argtypes = [vp, vp, vp]
         L.bgf_batch_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ip, ip]
         for name in ("bgf_create", "bgf_destroy", "bgf_set_state", "bgf_get_state", "bgf_init_dt", "bgf_step",
-                     "bgf_run", "bgf_run_async", "bgf_sync", "bgf_volume", "bgf_device_array", "bgf_set_timing",
+                     "bgf_run", "bgf_run_async", "bgf_sync", "bgf_volume", "bgf_last_run_steps", "bgf_device_array", "bgf_set_timing",
                      "bgf_kernel_times", "bgf_mesh_uniform", "bgf_mesh_from_leaves", "bgf_batch_create",
                      "bgf_batch_destroy", "bgf_batch_add", "bgf_batch_remove", "bgf_batch_run_async",
                      "bgf_batch_sync", "bgf_batch_loop", "bgf_batch_get_state", "bgf_batch_volume",
                      "bgf_batch_update_maxdepth", "bgf_batch_get_maxdepth", "bgf_batch_set_gauges",
-                     "bgf_batch_sample_gauges", "bgf_batch_stats"):
+                     "bgf_batch_sample_gauges", "bgf_batch_stats", "bgf_batch_get_gauges",
+                     "bgf_batch_track_maxdepth"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -288,6 +292,11 @@ class Domain:
         _check(lib().bgf_volume(self.h, ctypes.byref(v)))
         return v.value
 
+    def last_run_steps(self):
+        n = ctypes.c_int()
+        _check(lib().bgf_last_run_steps(self.h, ctypes.byref(n)))
+        return n.value
+
     def set_timing(self, on):
         _check(lib().bgf_set_timing(self.h, int(on)))
 
@@ -383,9 +392,23 @@ class Batch:
         _check(lib().bgf_batch_get_maxdepth(self.h, int(m), out.ctypes.data))
         return out
 
-    def set_gauges(self, m, cells):
+    NSAMP = 2048
+
+    def set_gauges(self, m, cells, every=0):
+        """Gauge cells of member m; every > 0 records them on the GPU every `every` member steps."""
         c = np.ascontiguousarray(np.asarray(cells, np.int32))
-        _check(lib().bgf_batch_set_gauges(self.h, int(m), len(c), _iptr(c)))
+        _check(lib().bgf_batch_set_gauges(self.h, int(m), len(c), _iptr(c), int(every)))
+
+    def gauges(self, m):
+        """Recorded gauge series of member m: (values[16, n], times[n])."""
+        vals = np.zeros((self.NGAUGE, self.NSAMP), np.float32)
+        times = np.zeros(self.NSAMP)
+        n = ctypes.c_int()
+        _check(lib().bgf_batch_get_gauges(self.h, int(m), vals.ctypes.data, times.ctypes.data, ctypes.byref(n)))
+        return vals[:, :n.value], times[:n.value]
+
+    def track_maxdepth(self, on=True):
+        _check(lib().bgf_batch_track_maxdepth(self.h, int(on)))
 
     def sample_gauges(self):
         vals = np.zeros((self.max_members, self.NGAUGE), np.float32)

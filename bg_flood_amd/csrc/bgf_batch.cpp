@@ -52,6 +52,10 @@ struct bgf_batch_s {
     double *d_volpart = nullptr, *d_volout = nullptr;
     int* d_gcell = nullptr;    // gauge cells, 16 per member slot
     float* d_gval = nullptr;   // sampled depths
+    float* d_ring = nullptr;   // per-step gauge series: NSAMP samples of 16 gauges per member slot
+    double* d_tring = nullptr; // their times
+    bool gauges_on = false;    // some member records gauge series
+    bool track_max = false;    // the corrector folds every step's depth into hmax
     bgf::DevLoop* d_loop = nullptr;
     bgf::KParams kp;
     hipStream_t stream = nullptr;
@@ -71,6 +75,7 @@ struct bgf_batch_s {
 namespace {
 
 constexpr int NGAUGE = 16;
+constexpr int NSAMP = 2048;  // gauge samples kept per member
 
 void free_batch(bgf_batch b) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
@@ -79,6 +84,7 @@ void free_batch(bgf_batch b) {
     for (int k = 0; k < 2; k++) { F(b->d_h[k]); F(b->d_u[k]); F(b->d_v[k]); }
     F(b->d_zb); F(b->d_cf); F(b->d_dh); F(b->d_dhu); F(b->d_dhv); F(b->d_hmax); F(b->d_ch);
     F(b->d_volpart); F(b->d_volout); F(b->d_gcell); F(b->d_gval); F(b->d_loop);
+    F(b->d_ring); F(b->d_tring);
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -134,6 +140,7 @@ bgf_status prepare_run(bgf_batch b) {
         HIPCHK(hipMemcpy(b->d_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
     b->krun = b->kp;
     b->krun.nblk = (int)order.size();
+    b->krun.hmax = b->track_max ? (float*)b->d_hmax : nullptr;
     if (!order.empty()) {
         b->krun.grid_pred = bgf::persistent_grid(b->prec, 0, b->krun.nblk, b->wg_pred);
         b->krun.grid_corr = bgf::persistent_grid(b->prec, 1, b->krun.nblk, b->wg_corr);
@@ -187,6 +194,9 @@ void enqueue_steps(bgf_batch b, int nsteps) {
                   (const float*)b->d_cf, (float*)b->d_dh, (float*)b->d_dhu, (float*)b->d_dhv, (float*)b->d_ch};
         }
         bgf::launch_step(b->prec, b->krun, storage, b->d_loop, 1, b->stream, nullptr);
+        if (b->gauges_on)
+            bgf::launch_gauge_step(b->prec, b->d_h[out], b->d_gcell, b->d_loop, b->cap_members, NSAMP, b->d_ring,
+                                   b->d_tring, b->stream);
     }
 }
 
@@ -231,6 +241,8 @@ bgf_status bgf_batch_create(const bgf_param* common, int max_blocks, int max_mem
          A((void**)&b->d_gval, sizeof(float) * NGAUGE * max_members);
     ok = ok && A((void**)&b->d_loop, sizeof(bgf::DevLoop) * max_members);
     ok = ok && hipMemset(b->d_gcell, 0xff, sizeof(int) * NGAUGE * max_members) == hipSuccess;  // no gauges: -1
+    ok = ok && A((void**)&b->d_ring, sizeof(float) * NGAUGE * NSAMP * max_members) &&
+         A((void**)&b->d_tring, sizeof(double) * NSAMP * max_members);
     if (!ok) {
         free_batch(b);
         (void)hipGetLastError();
@@ -372,6 +384,10 @@ bgf_status bgf_batch_add(bgf_batch b, const bgf_param* p, const bgf_blocks* blk,
     L.dtmax = L.dt;
     L.dt = 0.0;
     L.steps_done = 0;
+    L.nstep = 0;
+    L.gevery = 0;
+    std::vector<int> nog(NGAUGE, -1);
+    HIPCHK(hipMemcpy(b->d_gcell + NGAUGE * m, nog.data(), sizeof(int) * NGAUGE, hipMemcpyHostToDevice));
     L.done = L.totaltime >= L.endtime ? 1 : 0;
     b->loops[m] = L;
     *member = m;
@@ -535,11 +551,17 @@ bgf_status bgf_batch_get_maxdepth(bgf_batch b, int m, float* hmax) {
     return BGF_OK;
 }
 
-bgf_status bgf_batch_set_gauges(bgf_batch b, int m, int n, const int* cells) {
+bgf_status bgf_batch_set_gauges(bgf_batch b, int m, int n, const int* cells, int every) {
     bgf_status s = check_member(b, m);
     if (s != BGF_OK) return s;
     Member& M = b->members[m];
     if (n < 0 || n > NGAUGE || (n > 0 && !cells)) return fail(BGF_ERR_INVALID_ARG, "at most 16 gauges");
+    if (every < 0) return fail(BGF_ERR_INVALID_ARG, "bad sampling period");
+    b->loops[m].gevery = n > 0 ? every : 0;
+    if (n > 0 && every > 0 && !b->gauges_on) {
+        b->gauges_on = true;
+        b->run_valid = false;  // the captured step gains the sampling kernel
+    }
     std::vector<int> g(NGAUGE, -1);
     M.gauges.clear();
     for (int k = 0; k < n; k++) {
@@ -561,6 +583,29 @@ bgf_status bgf_batch_sample_gauges(bgf_batch b, float* values, double* times) {
     HIPCHK(hipStreamSynchronize(b->stream));
     if (times)
         for (int m = 0; m < b->cap_members; m++) times[m] = b->loops[m].totaltime;
+    return BGF_OK;
+}
+
+bgf_status bgf_batch_get_gauges(bgf_batch b, int m, float* values, double* times, int* nsamples) {
+    bgf_status s = check_member(b, m);
+    if (s != BGF_OK) return s;
+    const bgf::DevLoop& L = b->loops[m];
+    const int n = L.gevery > 0 ? std::min(NSAMP, L.nstep / L.gevery) : 0;
+    if (nsamples) *nsamples = n;
+    if (n > 0 && values)
+        for (int g = 0; g < NGAUGE; g++)
+            HIPCHK(hipMemcpy(values + (size_t)g * NSAMP, b->d_ring + ((size_t)m * NGAUGE + g) * NSAMP,
+                             sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (n > 0 && times)
+        HIPCHK(hipMemcpy(times, b->d_tring + (size_t)m * NSAMP, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return BGF_OK;
+}
+
+bgf_status bgf_batch_track_maxdepth(bgf_batch b, int on) {
+    if (!b) return fail(BGF_ERR_INVALID_ARG, "null batch");
+    if (b->pending) return fail(BGF_ERR_STATE, "bgf_batch_run_async pending: call bgf_batch_sync first");
+    b->track_max = on != 0;
+    b->run_valid = false;
     return BGF_OK;
 }
 

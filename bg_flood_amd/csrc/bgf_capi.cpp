@@ -52,6 +52,7 @@ struct bgf_handle_s {
     void* d_metric = nullptr;      // spherical metric terms per block
     bool use_graph = true;
     hipGraphExec_t graph = nullptr;  // one period of steps, captured on first use
+    int last_steps = 0;              // steps taken by the last bgf_run / bgf_sync
 };
 
 namespace {
@@ -492,6 +493,7 @@ bgf_status bgf_sync(bgf_handle h, bgf_loop* loop) {
     if (s != BGF_OK) return s;
     s = settle(h, D.steps_done);
     if (s != BGF_OK) return s;
+    h->last_steps = D.steps_done;
     if (loop) {
         loop->dt = D.dt;
         loop->dtmax = D.dtmax;
@@ -505,6 +507,12 @@ bgf_status bgf_run(bgf_handle h, bgf_loop* loop, int nsteps, double endtime) {
     bgf_status s = bgf_run_async(h, loop, nsteps, endtime);
     if (s != BGF_OK) return s;
     return bgf_sync(h, loop);
+}
+
+bgf_status bgf_last_run_steps(bgf_handle h, int* nsteps) {
+    if (!h || !nsteps) return fail(BGF_ERR_INVALID_ARG, "null argument");
+    *nsteps = h->last_steps;
+    return BGF_OK;
 }
 
 bgf_status bgf_volume(bgf_handle h, double* volume) {

@@ -19,6 +19,8 @@ struct DevLoop {
     int bad;         // a non-finite or non-positive dt was produced
     int steps_done;  // steps executed since the loop was uploaded
     int go;          // set by the dt reduction: the corrector half of this step runs
+    int nstep;       // steps since the domain started (gauge sampling counts these)
+    int gevery;      // gauge sampling period in steps (batch members)
     unsigned int slot32[NSLOT];
     unsigned long long slot64[NSLOT];
 };
@@ -51,6 +53,8 @@ struct KParams {
     // spherical grids: per-block rows of metric terms in T (METRIC_STRIDE per block)
     const void* metric;
     int spherical;
+    // running maximum depth of the interior cells, updated by every corrector (optional)
+    float* hmax;
 };
 
 // Layout of a block's metric terms (spherical grids, src/Spherical.cu:15-64):
@@ -88,6 +92,10 @@ void launch_init_dt_member(int precision, const KParams& P, const void* h, DevLo
 void launch_maxdepth(int precision, const KParams& P, const void* h, float* hmax, hipStream_t st);
 // val[k] = h[cell[k]] (NaN where cell[k] < 0), k < n.
 void launch_gauges(int precision, const void* h, const int* cell, float* val, int n, hipStream_t st);
+// After a batch step: members that stepped record h at their 16 gauges every L.gevery of their own
+// steps into ring[(m*16+g)*cap + k] and the time into tring[m*cap + k].
+void launch_gauge_step(int precision, const void* h, const int* cell, const DevLoop* L, int nmember, int cap,
+                       float* ring, double* tring, hipStream_t st);
 void launch_volume(int precision, const KParams& P, const void* h, double* part, double* out, hipStream_t st);
 // Copies h, u, v from src to dst for the blocks whose member ran an odd number of steps.
 void launch_settle(int precision, const KParams& P, const void* const* src, void* const* dst, const DevLoop* L,

@@ -677,6 +677,10 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
         B.uo[i] = u1;
         B.vo[i] = v1;
         if (COMP) B.ch[i] = comp;
+        if (P.hmax) {
+            const float hv = (float)h1;
+            if (hv > P.hmax[i]) P.hmax[i] = hv;
+        }
     }
 }
 
@@ -719,6 +723,7 @@ __global__ __launch_bounds__(64) void k_reduce_dt(DevLoop* L, int mode) {
         if (mode != 2) {
             L->go = 1;
             L->steps_done = L->steps_done + 1;
+            L->nstep = L->nstep + 1;
         }
         if (mode == 1) {
             L->totaltime = L->totaltime + L->dt;
@@ -752,6 +757,20 @@ __global__ __launch_bounds__(256) void k_maxdepth(KParams P, const T* __restrict
 template <class T> __global__ void k_gauges(const T* __restrict__ h, const int* __restrict__ cell, float* val, int n) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) val[k] = cell[k] >= 0 ? (float)h[cell[k]] : __builtin_nanf("");
+}
+
+template <class T>
+__global__ void k_gauge_step(const T* __restrict__ h, const int* __restrict__ cell, const DevLoop* L, int n, int cap,
+                             float* ring, double* tring) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int m = k >> 4;
+    const DevLoop& Lm = L[m];
+    if (!Lm.go || Lm.gevery <= 0 || Lm.nstep % Lm.gevery != 0) return;
+    const int s = Lm.nstep / Lm.gevery - 1;
+    if (s >= cap) return;
+    if (cell[k] >= 0) ring[(size_t)k * cap + s] = (float)h[cell[k]];
+    if ((k & 15) == 0) tring[(size_t)m * cap + s] = Lm.totaltime;
 }
 
 // Block partial sums of h*delta^2 in FP64 (deterministic two-level sum).
@@ -891,6 +910,14 @@ void launch_gauges(int precision, const void* h, const int* cell, float* val, in
     const dim3 g((n + 255) / 256), b(256);
     if (precision == 0) hipLaunchKernelGGL(k_gauges<double>, g, b, 0, st, (const double*)h, cell, val, n);
     else hipLaunchKernelGGL(k_gauges<float>, g, b, 0, st, (const float*)h, cell, val, n);
+}
+
+void launch_gauge_step(int precision, const void* h, const int* cell, const DevLoop* L, int nmember, int cap,
+                       float* ring, double* tring, hipStream_t st) {
+    const int n = 16 * nmember;
+    const dim3 g((n + 255) / 256), b(256);
+    if (precision == 0) hipLaunchKernelGGL(k_gauge_step<double>, g, b, 0, st, (const double*)h, cell, L, n, cap, ring, tring);
+    else hipLaunchKernelGGL(k_gauge_step<float>, g, b, 0, st, (const float*)h, cell, L, n, cap, ring, tring);
 }
 
 void launch_reduce_dt(int precision, DevLoop* L, int nloop, int advance_time, hipStream_t st) {

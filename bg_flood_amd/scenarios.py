@@ -337,3 +337,61 @@ def spherical_basin(nbx=4, nby=4, dx=0.01, lat0=-0.32, lon0=100.0, depth=10.0, a
     h[~inter.reshape(-1)] = 0
     z = np.zeros(S, T)
     return dict(param=p, blocks=blocks, nbx=nbx, nby=nby, zb=zb, h=h, u=z.copy(), v=z.copy(), cf=z.copy())
+
+
+def tsunami(nbx=32, nby=16, dx=40.0, maxlevel=2, amp=2.0, seed=5, precision="fp32", aoibnd=0, x0_frac=0.3,
+            refine_depth=5.0, refine_slope=1e-4):
+    """AMR tsunami: a 20 x 10 km domain (nbx x nby level-0 blocks of 16 cells of dx), a linear shelf
+    from -4000 m to a +20 m beach along x, a conical island (base radius 2 km, 50 m high) at a seeded
+    position, and a solitary wave of amplitude amp travelling towards the beach.  Blocks are refined
+    (up to maxlevel, so dx/2, dx/4) where any cell has |grad zs| > refine_slope or h < refine_depth."""
+    from . import mesh_from_leaves
+    rng = np.random.default_rng(seed)
+    Lx, Ly = nbx * BW * dx, nby * BW * dx
+    xi, yi = float(rng.uniform(0.45, 0.7) * Lx), float(rng.uniform(0.3, 0.7) * Ly)
+    x0 = x0_frac * Lx
+    d0 = 4000.0 - 4020.0 * x0_frac  # still-water depth under the crest
+    k = math.sqrt(3.0 * amp / (4.0 * d0 ** 3))
+
+    def bed(x, y):
+        shelf = -4000.0 + 4020.0 * x / Lx
+        cone = 50.0 - 50.0 * np.hypot(x - xi, y - yi) / 2000.0  # above the shelf within 2 km
+        return np.maximum(shelf, cone)
+
+    def eta(x):
+        return amp / np.cosh(k * (x - x0)) ** 2
+
+    def refine(l, bx, by):
+        s = dx / (1 << l)
+        xs = (bx * BW + np.arange(-1, BW + 1) + 0.5) * s
+        ys = (by * BW + np.arange(-1, BW + 1) + 0.5) * s
+        X, Y = np.meshgrid(xs, ys)
+        zb = bed(X, Y)
+        zs = np.maximum(eta(X), zb)
+        h = zs - zb
+        gx = np.abs(np.diff(zs, axis=1)) / s
+        gy = np.abs(np.diff(zs, axis=0)) / s
+        return bool((h[1:-1, 1:-1] < refine_depth).any() or (gx > refine_slope).any() or (gy > refine_slope).any())
+
+    leaves = quadtree_leaves(nbx, nby, maxlevel, refine)
+    blocks = mesh_from_leaves(leaves, nbx, nby, dx)
+    nblk = len(leaves)
+    levels = [b[0] for b in leaves]
+    p = default_param(dx=dx, delta=dx, nblk=nblk, nblkmem=nblkmem_for(nblk), precision=precision,
+                      minlevel=min(levels), maxlevel=max(levels), aoibnd=aoibnd, frictionmodel=-1, cf=0.025)
+    T = np.float64 if precision == "fp64" else np.float32
+    S = p["nblkmem"] * BS
+    X, Y = cell_coords_levels(blocks, p["nblkmem"], 0.0, 0.0, dx, np.float64)
+    zb64 = bed(X, Y)
+    e = eta(X)
+    h64 = np.maximum(np.maximum(e, zb64) - zb64, 0.0)
+    u64 = np.where(h64 > 1.0, e * np.sqrt(9.81 / np.maximum(h64, 1.0)), 0.0)
+    inter = np.zeros((p["nblkmem"], MW, MW), bool)
+    inter[:nblk, 1:MW - 1, 1:MW - 1] = True
+    m = inter.reshape(-1)
+    zb = np.where(m, zb64, 0.0).astype(T)
+    h = np.where(m, h64, 0.0).astype(T)
+    u = np.where(m, u64, 0.0).astype(T)
+    z = np.zeros(S, T)
+    return dict(param=p, blocks=blocks, nbx=nbx, nby=nby, zb=zb, h=h, u=u, v=z.copy(), cf=np.full(S, T(0.025), T),
+                leaves=leaves)

@@ -1,0 +1,181 @@
+"""Continuous batching of flood-solve requests on one GPU.
+
+A request is one scenario to run to its own simulated horizon; the result is its maximum-depth
+grid and the time series of up to 16 gauges.  One-level (Cartesian) requests join a shared
+bg_flood_amd.Batch as members: they are admitted while block slots, member slots and the HBM
+budget allow, advance together (each with its own dt) in chunks of steps, and leave the batch as
+soon as they reach their horizon, which frees room for the requests still waiting.  Multi-level
+(AMR) requests run on their own Domain handles, each on its own HIP stream, so their launches
+overlap the batch's on the device.  A member's arithmetic does not depend on the other members
+(same kernels, its own dt, its own gauge sampling counted in its own steps), so a result is the
+same whether the request ran alone or in the mix.
+
+There is no counterpart in the reference, which runs one scenario per process
+(src/Mainloop.cu:17-91); the outputs follow its max-depth and gauge outputs
+(src/Meanmax.cu, src/Mainloop.cu:359-500).
+"""
+import collections
+import time
+
+import numpy as np
+
+from . import BS, Batch, BgfError, Domain, Loop, interior_mask
+
+# HBM bytes of one block slot in the batch (h, u, v twice, zb, cf, dh, dhu, dhv, max depth,
+# compensation) and of one block of a multi-level domain (the reference's Model<T> arrays).
+BATCH_BYTES_PER_BLOCK = 13 * BS * 4
+AMR_BYTES_PER_BLOCK = 30 * BS * 4
+
+
+class Request:
+    """A solve request: scenario dict (param, blocks, nbx, nby, zb, h, u, v, cf), horizon in
+    simulated seconds and up to 16 gauge cells (memloc indices)."""
+
+    def __init__(self, rid, scenario, horizon, gauges=(), gauge_every=10):
+        self.rid = rid
+        self.sc = scenario
+        self.horizon = float(horizon)
+        self.gauges = list(gauges)[:16]
+        self.gauge_every = int(gauge_every)
+        self.multilevel = scenario["param"]["maxlevel"] > scenario["param"]["minlevel"]
+        self.submitted = None
+        self.finished = None
+        self.result = None
+        self.cells = scenario["param"]["nblk"] * 256
+        self.cell_updates = 0
+
+    @property
+    def latency(self):
+        return None if self.finished is None else self.finished - self.submitted
+
+
+class Server:
+    def __init__(self, common, max_blocks, max_members=128, hbm_budget=250e9, chunk=40):
+        self.batch = Batch(common, max_blocks, max_members)
+        self.batch.track_maxdepth(True)
+        self.precision = common["precision"]
+        self.max_blocks = max_blocks
+        self.budget = float(hbm_budget)
+        self.used = BATCH_BYTES_PER_BLOCK * max_blocks  # the batch is allocated up front
+        self.chunk = int(chunk)
+        self.waiting = collections.deque()
+        self.members = {}  # member id -> request
+        self.domains = []  # [request, Domain, Loop, hmax, gauge series, bytes]
+        self.done = []
+        self.cell_updates_amr = 0
+
+    # ------------------------------------------------------------ requests
+    def submit(self, req, now=None):
+        req.submitted = time.perf_counter() if now is None else now
+        self.waiting.append(req)
+
+    def _admit(self):
+        while self.waiting:
+            r = self.waiting[0]
+            p = r.sc["param"]
+            if r.multilevel:
+                need = AMR_BYTES_PER_BLOCK * p["nblkmem"]
+                if self.used + need > self.budget:
+                    return
+                d = Domain(dict(p, precision=self.precision), r.sc["blocks"])
+                d.set_state(r.sc["zb"], r.sc["h"], r.sc["u"], r.sc["v"], r.sc["cf"])
+                L = Loop(nextoutputtime=r.horizon, dt=0, dtmax=0, totaltime=0)
+                d.init_dt(L)
+                st = d.get_state()
+                self.domains.append([r, d, L, st["h"].astype(np.float32), ([], []), need])
+                self.used += need
+            else:
+                try:
+                    mid = self.batch.add(p, r.sc["blocks"], r.sc["zb"], r.sc["h"], r.sc["u"], r.sc["v"], r.sc["cf"],
+                                         endtime=r.horizon)
+                except BgfError:
+                    return  # no block or member slot free until a member leaves
+                if r.gauges:
+                    self.batch.set_gauges(mid, r.gauges, every=r.gauge_every)
+                self.members[mid] = r
+            self.waiting.popleft()
+
+    # ------------------------------------------------------------ serving
+    def poll(self):
+        """One scheduling round: admit, advance everything by one chunk, harvest finished requests."""
+        self._admit()
+        if self.members:
+            self.batch.run_async(self.chunk)
+        for rec in self.domains:
+            rec[1].run_async(rec[2], self.chunk, endtime=rec[0].horizon)
+        for rec in self.domains:
+            self._sync_domain(rec)
+        if self.members:
+            self.batch.sync()
+            now = time.perf_counter()
+            for mid in list(self.members):
+                L, done = self.batch.loop(mid)
+                if done:
+                    r = self.members.pop(mid)
+                    mask = interior_mask(self.batch.members[mid]["nblkmem"], self.batch.members[mid]["active"])
+                    hmax = np.where(mask, self.batch.maxdepth(mid), 0)
+                    gv, gt = self.batch.gauges(mid)
+                    r.result = dict(maxdepth=hmax, gauges=gv[:len(r.gauges)], times=gt, totaltime=L.totaltime)
+                    r.finished = now
+                    self.batch.remove(mid)
+                    self.done.append(r)
+        now = time.perf_counter()
+        for rec in list(self.domains):
+            r, d, L = rec[0], rec[1], rec[2]
+            if L.totaltime >= r.horizon:
+                p = r.sc["param"]
+                mask = interior_mask(p["nblkmem"], r.sc["blocks"]["active"][:p["nblk"]])
+                r.result = dict(maxdepth=np.where(mask, rec[3], 0), gauges=np.array(rec[4][0]).T if rec[4][0] else None,
+                                times=np.array(rec[4][1]), totaltime=L.totaltime)
+                r.finished = now
+                d.close()
+                self.used -= rec[5]
+                self.domains.remove(rec)
+                self.done.append(r)
+
+    def _sync_domain(self, rec):
+        r, d, L = rec[0], rec[1], rec[2]
+        d.sync(L)
+        st = d.get_state()
+        np.maximum(rec[3], st["h"].astype(np.float32), out=rec[3])
+        if r.gauges:
+            rec[4][0].append(st["h"][r.gauges].astype(np.float32))
+            rec[4][1].append(L.totaltime)
+        n = d.last_run_steps()
+        r.cell_updates += n * r.cells
+        self.cell_updates_amr += n * r.cells
+
+    def busy(self):
+        return bool(self.waiting or self.members or self.domains)
+
+    def stats(self):
+        return self.batch.stats()
+
+    def close(self):
+        for rec in self.domains:
+            rec[1].close()
+        self.domains = []
+        self.batch.close()
+
+
+def serve(requests, arrivals, common, max_blocks, max_members=128, hbm_budget=250e9, chunk=40, time_limit=600.0):
+    """Feeds requests to a Server at their arrival times (seconds from the start, real time) and
+    serves until all are done or time_limit passes.  Returns (server, wall seconds)."""
+    srv = Server(common, max_blocks, max_members, hbm_budget, chunk)
+    order = np.argsort(arrivals)
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        now = time.perf_counter() - t0
+        while k < len(order) and arrivals[order[k]] <= now:
+            srv.submit(requests[order[k]], now=t0 + arrivals[order[k]])
+            k += 1
+        if not srv.busy():
+            if k >= len(order):
+                break
+            time.sleep(min(0.002, max(0.0, arrivals[order[k]] - now)))
+            continue
+        srv.poll()
+        if time.perf_counter() - t0 > time_limit:
+            break
+    return srv, time.perf_counter() - t0

@@ -136,6 +136,10 @@ bgf_status bgf_run(bgf_handle h, bgf_loop* loop, int nsteps, double endtime);
 bgf_status bgf_run_async(bgf_handle h, const bgf_loop* loop, int nsteps, double endtime);
 bgf_status bgf_sync(bgf_handle h, bgf_loop* loop);
 
+/* Steps actually taken by the last bgf_run / bgf_run_async+bgf_sync (fewer
+ * than asked when endtime was reached). */
+bgf_status bgf_last_run_steps(bgf_handle h, int* nsteps);
+
 /* Sum of h * delta^2 over active cells, accumulated in FP64 on the GPU. */
 bgf_status bgf_volume(bgf_handle h, double* volume);
 
@@ -217,10 +221,19 @@ bgf_status bgf_batch_volume(bgf_batch b, int member, double* volume);
 bgf_status bgf_batch_update_maxdepth(bgf_batch b);
 bgf_status bgf_batch_get_maxdepth(bgf_batch b, int member, float* hmax);
 
-/* Gauges: up to 16 cells (memloc indices in the member's arrays) per member;
- * bgf_batch_sample_gauges writes 16*max_members depths (NaN for unused
- * gauges) and, if times is not NULL, each member's totaltime. */
-bgf_status bgf_batch_set_gauges(bgf_batch b, int member, int n, const int* cells);
+/* With tracking on, every step's corrector folds the depth into the running
+ * maximum (instead of the snapshots of bgf_batch_update_maxdepth). */
+bgf_status bgf_batch_track_maxdepth(bgf_batch b, int on);
+
+/* Gauges: up to 16 cells (memloc indices in the member's arrays) per member,
+ * recorded on the GPU after every `every`-th step of the member itself (so a
+ * member's series do not depend on the rest of the batch), up to 2048
+ * samples.  bgf_batch_get_gauges copies them: values[g*2048 + k] is gauge g
+ * at sample k, times[k] its totaltime.  bgf_batch_sample_gauges reads the
+ * current depth at every member's gauges now: 16*max_members values (NaN for
+ * unused gauges) and, if times is not NULL, each member's totaltime. */
+bgf_status bgf_batch_set_gauges(bgf_batch b, int member, int n, const int* cells, int every);
+bgf_status bgf_batch_get_gauges(bgf_batch b, int member, float* values, double* times, int* nsamples);
 bgf_status bgf_batch_sample_gauges(bgf_batch b, float* values, double* times);
 
 /* Cell-updates performed so far, members and active blocks now. */
