@@ -93,6 +93,9 @@ def lib():
         L.bgf_run_async.argtypes = [vp, ctypes.POINTER(Loop), ctypes.c_int, ctypes.c_double]
         L.bgf_sync.argtypes = [vp, ctypes.POINTER(Loop)]
         L.bgf_volume.argtypes = [vp, dp]
+        L.bgf_adapt_refine.argtypes = [ctypes.POINTER(BgfParam), ctypes.POINTER(BgfBlocks), vp, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, ip, ip, ip, ip, dp, dp] + [ip] * 8
+        L.bgf_refine_flags.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, vp]
         L.bgf_last_run_steps.argtypes = [vp, ip]
         L.bgf_device_array.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp)]
         L.bgf_set_timing.argtypes = [vp, ctypes.c_int]
@@ -119,7 +122,8 @@ def lib():
         L.bgf_batch_sample_gauges.argtypes = [vp, vp, vp]
         L.bgf_batch_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ip, ip]
         for name in ("bgf_create", "bgf_destroy", "bgf_set_state", "bgf_get_state", "bgf_init_dt", "bgf_step",
-                     "bgf_run", "bgf_run_async", "bgf_sync", "bgf_volume", "bgf_last_run_steps", "bgf_device_array", "bgf_set_timing",
+                     "bgf_run", "bgf_run_async", "bgf_sync", "bgf_volume", "bgf_last_run_steps", "bgf_adapt_refine",
+                     "bgf_refine_flags", "bgf_device_array", "bgf_set_timing",
                      "bgf_kernel_times", "bgf_mesh_uniform", "bgf_mesh_from_leaves", "bgf_batch_create",
                      "bgf_batch_destroy", "bgf_batch_add", "bgf_batch_remove", "bgf_batch_run_async",
                      "bgf_batch_sync", "bgf_batch_loop", "bgf_batch_get_state", "bgf_batch_volume",
@@ -171,6 +175,30 @@ def mesh_from_leaves(leaves, nbx, nby, dx, base=0):
                                       _iptr(out["TopLeft"]), _iptr(out["TopRight"]), _iptr(out["BotLeft"]),
                                       _iptr(out["BotRight"])))
     return out
+
+
+def adapt_refine(param, blocks, flags, nbx, nby, capacity=None):
+    """One refinement pass with the reference's block numbering (bgf_adapt_refine): blocks whose
+    flags[ib] is set split in four.  Returns (blocks, nblk, nblkmem) of the new mesh."""
+    nbm = int(param["nblkmem"])
+    keep = {}
+    P = _param_struct(param)
+    B = _blocks_struct(blocks, nbm, keep)
+    f = np.zeros(nbm, np.uint8)
+    f[:len(flags)] = np.asarray(flags, np.uint8)
+    cap = int(capacity or (4 * nbm + 16))
+    out = {k: np.zeros(cap, np.int32) for k in BLOCK_KEYS}
+    out["xo"] = np.zeros(cap)
+    out["yo"] = np.zeros(cap)
+    nb, nm = ctypes.c_int(), ctypes.c_int()
+    _check(lib().bgf_adapt_refine(ctypes.byref(P), ctypes.byref(B), f.ctypes.data, int(nbx), int(nby), cap,
+                                  ctypes.byref(nb), ctypes.byref(nm), _iptr(out["active"]), _iptr(out["level"]),
+                                  _dptr(out["xo"]), _dptr(out["yo"]), _iptr(out["LeftBot"]), _iptr(out["LeftTop"]),
+                                  _iptr(out["RightBot"]), _iptr(out["RightTop"]), _iptr(out["TopLeft"]),
+                                  _iptr(out["TopRight"]), _iptr(out["BotLeft"]), _iptr(out["BotRight"])))
+    res = {k: v[:nm.value].copy() for k, v in out.items()}
+    res["active"] = res["active"][:nb.value]
+    return res, nb.value, nm.value
 
 
 def default_param(**kw):
@@ -291,6 +319,13 @@ class Domain:
         v = ctypes.c_double()
         _check(lib().bgf_volume(self.h, ctypes.byref(v)))
         return v.value
+
+    def refine_flags(self, field="h", mode=0, a=0.0, b=0.0):
+        """Refinement flags of the Threshold (mode 0: any cell > a) or Inrange (mode 1: a <= z <= b)
+        criterion on the current state, per block (nblkmem entries)."""
+        f = np.zeros(self.nblkmem, np.uint8)
+        _check(lib().bgf_refine_flags(self.h, field.encode(), int(mode), float(a), float(b), f.ctypes.data))
+        return f
 
     def last_run_steps(self):
         n = ctypes.c_int()

@@ -509,6 +509,32 @@ bgf_status bgf_run(bgf_handle h, bgf_loop* loop, int nsteps, double endtime) {
     return bgf_sync(h, loop);
 }
 
+bgf_status bgf_refine_flags(bgf_handle h, const char* field, int mode, double a, double b, unsigned char* flags) {
+    if (!h || !field || !flags || (mode != 0 && mode != 1)) return fail(BGF_ERR_INVALID_ARG, "bad argument");
+    if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
+    const std::string f(field);
+    const void *x = nullptr, *y = nullptr;
+    if (f == "h") x = h->d_h[0];
+    else if (f == "u") x = h->d_u[0];
+    else if (f == "v") x = h->d_v[0];
+    else if (f == "zb") x = h->d_zb;
+    else if (f == "zs") { x = h->d_h[0]; y = h->d_zb; }
+    else return fail(BGF_ERR_INVALID_ARG, "unknown field " + f);
+    unsigned char* d = nullptr;
+    HIPCHK(hipMalloc((void**)&d, h->nblkmem));
+    if (hipMemsetAsync(d, 0, h->nblkmem, h->stream) != hipSuccess) {
+        (void)hipFree(d);
+        return fail(BGF_ERR_HIP, "refine flags: memset failed");
+    }
+    bgf::launch_refine_flags(h->prec, h->kp, x, y, mode, a, b, d, h->stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(flags, d, h->nblkmem, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(BGF_ERR_HIP, std::string("refine flags: ") + hipGetErrorString(e));
+    return BGF_OK;
+}
+
 bgf_status bgf_last_run_steps(bgf_handle h, int* nsteps) {
     if (!h || !nsteps) return fail(BGF_ERR_INVALID_ARG, "null argument");
     *nsteps = h->last_steps;
@@ -612,58 +638,22 @@ bgf_status bgf_mesh_from_leaves(int nleaf, const int* lev, const int* ix, const 
                                 int* BotLeft, int* BotRight) {
     if (nleaf <= 0 || nbx <= 0 || nby <= 0 || !lev || !ix || !iy || !active || !level || !xo || !yo)
         return fail(BGF_ERR_INVALID_ARG, "bad mesh arguments");
-    auto key = [base](int l, long long x, long long y) {
-        return ((unsigned long long)(l - base) << 58) | ((unsigned long long)x << 29) | (unsigned long long)y;
-    };
-    std::unordered_map<unsigned long long, int> at;
-    at.reserve((size_t)nleaf * 2);
-    double area = 0.0;
+    std::vector<int> id(nleaf);
+    std::vector<long long> x(nleaf), y(nleaf);
     for (int k = 0; k < nleaf; k++) {
-        const int d = lev[k] - base;
-        if (d < 0 || d > 28) return fail(BGF_ERR_INVALID_ARG, "leaf level out of range");
-        const long long w = (long long)nbx << d, hgt = (long long)nby << d;
-        if (ix[k] < 0 || iy[k] < 0 || ix[k] >= w || iy[k] >= hgt) return fail(BGF_ERR_INVALID_ARG, "leaf outside the domain");
-        if (!at.emplace(key(lev[k], ix[k], iy[k]), k).second) return fail(BGF_ERR_INVALID_ARG, "duplicate leaf");
-        area += std::ldexp(1.0, -2 * d);
+        id[k] = k;
+        x[k] = ix[k];
+        y[k] = iy[k];
     }
-    for (int k = 0; k < nleaf; k++)  // no leaf may contain another
-        for (int l = lev[k] - 1, x = ix[k] >> 1, y = iy[k] >> 1; l >= base; l--, x >>= 1, y >>= 1)
-            if (at.count(key(l, x, y))) return fail(BGF_ERR_INVALID_ARG, "overlapping leaves");
-    if (area != (double)nbx * (double)nby) return fail(BGF_ERR_INVALID_ARG, "leaves do not cover the domain");
-    auto find = [&](int l, long long x, long long y) -> int {
-        auto it = at.find(key(l, x, y));
-        return it == at.end() ? -1 : it->second;
-    };
-    // neighbour(s) of block k across the side with offset (ox, oy); (f0, f1)
-    // are the positions of the two finer blocks relative to 2*(x+ox, y+oy)
-    auto side = [&](int k, int ox, int oy, int fx0, int fy0, int fx1, int fy1, int* first, int* second) -> bool {
-        const int l = lev[k], d = l - base;
-        const long long x = (long long)ix[k] + ox, y = (long long)iy[k] + oy;
-        if (x < 0 || y < 0 || x >= ((long long)nbx << d) || y >= ((long long)nby << d)) {
-            first[k] = second[k] = k;
-            return true;
-        }
-        int n = find(l, x, y);
-        if (n < 0 && l > base) n = find(l - 1, x >> 1, y >> 1);
-        if (n >= 0) {
-            first[k] = second[k] = n;
-            return true;
-        }
-        const int a = find(l + 1, 2 * x + fx0, 2 * y + fy0), c = find(l + 1, 2 * x + fx1, 2 * y + fy1);
-        if (a < 0 || c < 0) return false;
-        first[k] = a;
-        second[k] = c;
-        return true;
-    };
+    bgf_status s = quadtree_neighbours(nleaf, id.data(), lev, x.data(), y.data(), nbx, nby, base, LeftBot, LeftTop,
+                                       RightBot, RightTop, TopLeft, TopRight, BotLeft, BotRight);
+    if (s != BGF_OK) return s;
     for (int k = 0; k < nleaf; k++) {
         active[k] = k;
         level[k] = lev[k];
-        const double ldx = lev[k] < 0 ? dx * (1 << -lev[k]) : dx / (1 << lev[k]);
+        const double ldx = level_dx(dx, lev[k]);
         xo[k] = ix[k] * (double)BW * ldx + 0.5 * ldx;
         yo[k] = iy[k] * (double)BW * ldx + 0.5 * ldx;
-        bool ok = side(k, -1, 0, 1, 0, 1, 1, LeftBot, LeftTop) && side(k, 1, 0, 0, 0, 0, 1, RightBot, RightTop) &&
-                  side(k, 0, -1, 0, 1, 1, 1, BotLeft, BotRight) && side(k, 0, 1, 0, 0, 1, 0, TopLeft, TopRight);
-        if (!ok) return fail(BGF_ERR_INVALID_ARG, "mesh is not 2:1 balanced");
     }
     return BGF_OK;
 }

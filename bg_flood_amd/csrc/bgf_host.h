@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/bgflood_amd.h"
@@ -140,6 +141,69 @@ inline std::vector<int> processing_order(const bgf_blocks* b, int nblk, double d
     std::stable_sort(idx.begin(), idx.end(), [&](int a, int c) { return key[a] < key[c]; });
     for (int k = 0; k < nblk; k++) order[k] = b->active[idx[k]];
     return order;
+}
+
+// Neighbours of quadtree blocks from their integer positions: block k of the
+// list has id[k], level lev[k] and position (ix[k], iy[k]) among the blocks
+// of its level; the domain is nbx x nby blocks of level `base`.  A side shared
+// with one block of the same or a coarser level points both halves at it, a
+// side shared with two finer blocks points the first half (bottom or left)
+// and the second half at them, a side on the domain edge points at the block
+// itself (the convention of src/Mesh.cu:351-433 and src/Adaptation.cu:1017-1795).
+// Outputs are indexed by block id.  Fails unless the blocks tile the domain
+// with neighbouring levels within one of each other.
+inline bgf_status quadtree_neighbours(int n, const int* id, const int* lev, const long long* ix, const long long* iy,
+                                      int nbx, int nby, int base, int* LeftBot, int* LeftTop, int* RightBot,
+                                      int* RightTop, int* TopLeft, int* TopRight, int* BotLeft, int* BotRight) {
+    auto key = [base](int l, long long x, long long y) {
+        return ((unsigned long long)(l - base) << 58) | ((unsigned long long)x << 29) | (unsigned long long)y;
+    };
+    std::unordered_map<unsigned long long, int> at;
+    at.reserve((size_t)n * 2);
+    double area = 0.0;
+    for (int k = 0; k < n; k++) {
+        const int d = lev[k] - base;
+        if (d < 0 || d > 28) return fail(BGF_ERR_INVALID_ARG, "block level out of range");
+        const long long w = (long long)nbx << d, hgt = (long long)nby << d;
+        if (ix[k] < 0 || iy[k] < 0 || ix[k] >= w || iy[k] >= hgt) return fail(BGF_ERR_INVALID_ARG, "block outside the domain");
+        if (!at.emplace(key(lev[k], ix[k], iy[k]), id[k]).second) return fail(BGF_ERR_INVALID_ARG, "duplicate block");
+        area += std::ldexp(1.0, -2 * d);
+    }
+    for (int k = 0; k < n; k++)  // no block may contain another
+        for (long long l = lev[k] - 1, x = ix[k] >> 1, y = iy[k] >> 1; l >= base; l--, x >>= 1, y >>= 1)
+            if (at.count(key((int)l, x, y))) return fail(BGF_ERR_INVALID_ARG, "overlapping blocks");
+    if (area != (double)nbx * (double)nby) return fail(BGF_ERR_INVALID_ARG, "blocks do not cover the domain");
+    auto find = [&](int l, long long x, long long y) -> int {
+        auto it = at.find(key(l, x, y));
+        return it == at.end() ? -1 : it->second;
+    };
+    // neighbour(s) across the side with offset (ox, oy); (f0, f1) are the
+    // positions of the two finer blocks relative to 2*(x+ox, y+oy)
+    auto side = [&](int k, int ox, int oy, int fx0, int fy0, int fx1, int fy1, int* first, int* second) -> bool {
+        const int l = lev[k], d = l - base, me = id[k];
+        const long long x = ix[k] + ox, y = iy[k] + oy;
+        if (x < 0 || y < 0 || x >= ((long long)nbx << d) || y >= ((long long)nby << d)) {
+            first[me] = second[me] = me;
+            return true;
+        }
+        int nb = find(l, x, y);
+        if (nb < 0 && l > base) nb = find(l - 1, x >> 1, y >> 1);
+        if (nb >= 0) {
+            first[me] = second[me] = nb;
+            return true;
+        }
+        const int a = find(l + 1, 2 * x + fx0, 2 * y + fy0), c = find(l + 1, 2 * x + fx1, 2 * y + fy1);
+        if (a < 0 || c < 0) return false;
+        first[me] = a;
+        second[me] = c;
+        return true;
+    };
+    for (int k = 0; k < n; k++) {
+        bool ok = side(k, -1, 0, 1, 0, 1, 1, LeftBot, LeftTop) && side(k, 1, 0, 0, 0, 0, 1, RightBot, RightTop) &&
+                  side(k, 0, -1, 0, 1, 1, 1, BotLeft, BotRight) && side(k, 0, 1, 0, 0, 1, 0, TopLeft, TopRight);
+        if (!ok) return fail(BGF_ERR_INVALID_ARG, "mesh is not 2:1 balanced");
+    }
+    return BGF_OK;
 }
 
 }  // namespace bgf_host

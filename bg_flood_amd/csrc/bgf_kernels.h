@@ -101,6 +101,12 @@ void launch_volume(int precision, const KParams& P, const void* h, double* part,
 void launch_settle(int precision, const KParams& P, const void* const* src, void* const* dst, const DevLoop* L,
                    hipStream_t st);
 
+// Refinement criteria (src/AdaptCriteria.cu:62-160) per active block of P:
+// flags[ib] = any interior cell of a (plus b when b is not null: zs = zb + h)
+// exceeds lo (mode 0) or lies in [lo, hi] (mode 1).
+void launch_refine_flags(int precision, const KParams& P, const void* a, const void* b, int mode, double lo,
+                         double hi, unsigned char* flags, hipStream_t st);
+
 // dt reduction of the block-minimum slots, clamp and output landing (one wave).
 void launch_reduce_dt(int precision, DevLoop* L, int nloop, int advance_time, hipStream_t st);
 

@@ -773,6 +773,18 @@ __global__ void k_gauge_step(const T* __restrict__ h, const int* __restrict__ ce
     if ((k & 15) == 0) tring[(size_t)m * cap + s] = Lm.totaltime;
 }
 
+template <class T>
+__global__ __launch_bounds__(256) void k_refine_flags(KParams P, const T* __restrict__ a, const T* __restrict__ b,
+                                                      int mode, T lo, T hi, unsigned char* flags) {
+    const int t = threadIdx.x;
+    const int ib = P.active[blockIdx.x];
+    const int i = memloc(t & 15, t >> 4, ib);
+    const T z = b ? b[i] + a[i] : a[i];
+    const int hit = mode == 0 ? z > lo : (z >= lo && z <= hi);
+    const int any = __syncthreads_or(hit);
+    if (t == 0) flags[ib] = any ? 1 : 0;
+}
+
 // Block partial sums of h*delta^2 in FP64 (deterministic two-level sum).
 template <class T>
 __global__ __launch_bounds__(256) void k_volume(KParams P, const T* __restrict__ h, double* __restrict__ part) {
@@ -918,6 +930,16 @@ void launch_gauge_step(int precision, const void* h, const int* cell, const DevL
     const dim3 g((n + 255) / 256), b(256);
     if (precision == 0) hipLaunchKernelGGL(k_gauge_step<double>, g, b, 0, st, (const double*)h, cell, L, n, cap, ring, tring);
     else hipLaunchKernelGGL(k_gauge_step<float>, g, b, 0, st, (const float*)h, cell, L, n, cap, ring, tring);
+}
+
+void launch_refine_flags(int precision, const KParams& P, const void* a, const void* b, int mode, double lo,
+                         double hi, unsigned char* flags, hipStream_t st) {
+    if (precision == 0)
+        hipLaunchKernelGGL(k_refine_flags<double>, dim3(P.nblk), dim3(256), 0, st, P, (const double*)a,
+                           (const double*)b, mode, lo, hi, flags);
+    else
+        hipLaunchKernelGGL(k_refine_flags<float>, dim3(P.nblk), dim3(256), 0, st, P, (const float*)a,
+                           (const float*)b, mode, (float)lo, (float)hi, flags);
 }
 
 void launch_reduce_dt(int precision, DevLoop* L, int nloop, int advance_time, hipStream_t st) {

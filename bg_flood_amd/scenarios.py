@@ -395,3 +395,32 @@ def tsunami(nbx=32, nby=16, dx=40.0, maxlevel=2, amp=2.0, seed=5, precision="fp3
     z = np.zeros(S, T)
     return dict(param=p, blocks=blocks, nbx=nbx, nby=nby, zb=zb, h=h, u=u, v=z.copy(), cf=np.full(S, T(0.025), T),
                 leaves=leaves)
+
+
+def block_any(blocks, nblk, nblkmem, dx, fn, precision="fp64"):
+    """Per block (nblkmem entries): whether fn(x, y) holds at any interior cell centre."""
+    T = np.float64 if precision == "fp64" else np.float32
+    X, Y = cell_coords_levels(blocks, nblkmem, 0.0, 0.0, dx, T)
+    hit = np.asarray(fn(X.astype(np.float64), Y.astype(np.float64)), bool).reshape(nblkmem, MW, MW)
+    inter = np.zeros((nblkmem, MW, MW), bool)
+    inter[np.asarray(blocks["active"][:nblk]), 1:MW - 1, 1:MW - 1] = True
+    return (hit & inter).any(axis=(1, 2))
+
+
+def adapt_mesh(nbx, nby, dx, maxlevel, fn, precision="fp64", maxiter=None):
+    """The reference's set-up adaptation loop (src/Adaptation.cu:17-121): starting from the uniform
+    level-0 mesh, refine every block where fn(x, y) holds at some cell, with the reference's sanity
+    cascade and block numbering (bgf_adapt_refine), until no block is added.  Returns
+    (blocks, nblk, nblkmem)."""
+    from . import adapt_refine
+    blocks = mesh_uniform(nbx, nby, dx, 0, precision == "fp64")
+    nblk = nbx * nby
+    nblkmem = nblkmem_for(nblk)
+    for _ in range(maxiter or maxlevel + 1):
+        p = default_param(dx=dx, delta=dx, nblk=nblk, nblkmem=nblkmem, maxlevel=maxlevel, precision=precision)
+        flags = block_any(blocks, nblk, nblkmem, dx, fn, precision)
+        nb2, n2, m2 = adapt_refine(p, blocks, flags, nbx, nby)
+        if n2 == nblk:
+            break
+        blocks, nblk, nblkmem = nb2, n2, m2
+    return blocks, nblk, nblkmem

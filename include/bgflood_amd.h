@@ -177,6 +177,29 @@ bgf_status bgf_mesh_from_leaves(int nleaf, const int* lev, const int* ix, const 
                                 int* LeftTop, int* RightBot, int* RightTop, int* TopLeft, int* TopRight,
                                 int* BotLeft, int* BotRight);
 
+/* One refinement pass of the quadtree with the reference's numbering
+ * (src/Adaptation.cu:157-325 sanity cascade, 435-664 CalcAvailblk/AddBlocks,
+ * 1017-1879 refine and cleanup; the reference cancels coarsening): blocks
+ * whose refine_flags[ib] is set (and the coarser neighbours the 2:1 balance
+ * forces) split in four; the block keeps its index as the bottom-left child
+ * and the other three take free block slots in ascending order; new blocks
+ * are appended to the active list.  p gives nblk, nblkmem, maxlevel, dx and
+ * the precision the origins are stored in; b the current mesh (nbx x nby
+ * level-0 blocks).  Outputs have `capacity` entries; *nblkmem_out may grow
+ * (ceil((nblk + 3*nrefine) * 1.05)), BGF_ERR_OOM if it exceeds capacity.
+ * Entries of unused slots: active -1, neighbours the slot itself. */
+bgf_status bgf_adapt_refine(const bgf_param* p, const bgf_blocks* b, const unsigned char* refine_flags, int nbx,
+                            int nby, int capacity, int* nblk_out, int* nblkmem_out, int* active, int* level,
+                            double* xo, double* yo, int* LeftBot, int* LeftTop, int* RightBot, int* RightTop,
+                            int* TopLeft, int* TopRight, int* BotLeft, int* BotRight);
+
+/* Refinement flags of the reference's criteria on the GPU state
+ * (src/AdaptCriteria.cu:62-160): per active block, mode 0 (Threshold) sets
+ * flags[ib] if any interior cell of the field exceeds a; mode 1 (Inrange) if
+ * any lies in [a, b].  field: "h", "zs", "zb", "u" or "v".  flags has
+ * nblkmem entries (0 for inactive blocks). */
+bgf_status bgf_refine_flags(bgf_handle h, const char* field, int mode, double a, double b, unsigned char* flags);
+
 /* ------------------------------------------------------------------ batches
  * A batch advances many independent one-level domains ("members") with one
  * set of launches per step: ensembles, and continuous batching of solve

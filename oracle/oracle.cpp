@@ -1583,3 +1583,117 @@ double orc_timestep_d(double dtmin_cells, const orc_loop* loop) { return timeste
 float orc_timestep_f(float dtmin_cells, const orc_loop* loop) { return timestep_clamp(dtmin_cells, *loop); }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ adaptation
+// One pass of src/Adaptation.cu restricted to refinement (the reference
+// cancels all coarsening, :170-175): refinesanitycheck (:157-325),
+// CalcAvailblk and AddBlocks (:535-662), refine (:1017-1795, block indices,
+// levels, origins and activation order) and Adaptationcleanup (:1807-1876).
+// The reference updates neighbours incrementally; here they are found by a
+// brute-force geometric search over the active blocks (a point just across
+// each end of each side, as InitBlockneighbours does by coordinates,
+// src/Mesh.cu:351-433).
+namespace {
+template <class T>
+int adapt_refine_t(const orc_param* p, const orc_blocks* b, const unsigned char* flags, int capacity, int* nblk_out,
+                   int* nblkmem_out, int* active, int* level, double* xo, double* yo, int* nb[8]) {
+    const int nblk = p->nblk, nbm = p->nblkmem;
+    std::vector<int> lev(b->level, b->level + nbm), isact(nbm, 0);
+    std::vector<bool> refine(nbm, false);
+    for (int ibl = 0; ibl < nblk; ibl++) { int ib = b->active[ibl]; isact[ib] = 1; refine[ib] = flags[ib] != 0; }
+    for (int ibl = 0; ibl < nblk; ibl++) {
+        int ib = b->active[ibl];
+        if (refine[ib] && lev[ib] == p->maxlevel) refine[ib] = false;
+    }
+    int iter = 1;
+    while (iter > 0) {
+        iter = 0;
+        for (int ibl = 0; ibl < nblk; ibl++) {
+            int ib = b->active[ibl];
+            if (refine[ib]) {
+                const int nbs[4] = {b->TopLeft[ib], b->BotLeft[ib], b->LeftBot[ib], b->RightBot[ib]};
+                for (int n : nbs)
+                    if (refine[n] == false && lev[n] < lev[ib]) { refine[n] = true; iter++; }
+            }
+        }
+    }
+    // CalcAvailblk
+    std::vector<int> invactive(nbm, -1), csumblk(nbm, 0);
+    int csum = -3, nrefineblk = 0;
+    for (int ibl = 0; ibl < nblk; ibl++) {
+        int ib = b->active[ibl];
+        invactive[ib] = ibl;
+        if (refine[ib]) { nrefineblk++; csum += 3; }
+        csumblk[ib] = csum;
+    }
+    std::vector<int> availblk;
+    for (int ibl = 0; ibl < nbm; ibl++) if (invactive[ibl] == -1) availblk.push_back(ibl);
+    int nnewblk = 3 * nrefineblk;
+    int nblkmem = nbm;
+    if (nnewblk > (int)availblk.size()) {  // AddBlocks
+        nblkmem = (int)std::ceil((nblk + nnewblk) * 1.05);
+        invactive.resize(nblkmem, -1);
+        availblk.clear();
+        for (int ibl = 0; ibl < nblkmem; ibl++) if (invactive[ibl] == -1) availblk.push_back(ibl);
+    }
+    if (nblkmem > capacity) return -1;
+    std::vector<T> bxo(nblkmem, T(0)), byo(nblkmem, T(0));
+    std::vector<int> newlevel(nblkmem, 0);
+    for (int i = 0; i < nbm; i++) { bxo[i] = T(b->xo[i]); byo[i] = T(b->yo[i]); newlevel[i] = lev[i]; }
+    std::vector<int> act(b->active, b->active + nblk);
+    for (int ibl = 0; ibl < nblk; ibl++) {
+        int ib = b->active[ibl];
+        if (!refine[ib]) continue;
+        double delx = calcres(p->dx, lev[ib] + 1);
+        double xoblk = bxo[ib] - 0.5 * delx;
+        double yoblk = byo[ib] - 0.5 * delx;
+        int ibr = availblk[csumblk[ib]], ibtl = availblk[csumblk[ib] + 1], ibtr = availblk[csumblk[ib] + 2];
+        newlevel[ib] = newlevel[ibr] = newlevel[ibtl] = newlevel[ibtr] = lev[ib] + 1;
+        bxo[ib] = T(xoblk); byo[ib] = T(yoblk);
+        bxo[ibr] = T(xoblk + BW * delx); byo[ibr] = T(yoblk);
+        bxo[ibtl] = T(xoblk); byo[ibtl] = T(yoblk + BW * delx);
+        bxo[ibtr] = T(xoblk + BW * delx); byo[ibtr] = T(yoblk + BW * delx);
+    }
+    for (int ibl = 0; ibl < nblk; ibl++) {
+        int ib = b->active[ibl];
+        if (refine[ib]) for (int j = 0; j < 3; j++) act.push_back(availblk[csumblk[ib] + j]);
+    }
+    const int n2 = (int)act.size();
+    // neighbours by coordinates: the block containing a point a quarter cell
+    // across each side, a quarter cell in from each end of it
+    for (int i = 0; i < nblkmem; i++) {
+        for (int k = 0; k < 8; k++) nb[k][i] = i;
+        active[i] = -1; level[i] = newlevel[i]; xo[i] = (double)bxo[i]; yo[i] = (double)byo[i];
+    }
+    auto contains = [&](int j, double px, double py) {
+        double d = calcres(p->dx, newlevel[j]);
+        double x0 = (double)bxo[j] - 0.5 * d, y0 = (double)byo[j] - 0.5 * d;
+        return px > x0 && px < x0 + BW * d && py > y0 && py < y0 + BW * d;
+    };
+    const double W = p->nx * p->dx, H = p->ny * p->dx;  // domain of the level-0 mesh
+    for (int k = 0; k < n2; k++) {
+        int ib = act[k];
+        double d = calcres(p->dx, newlevel[ib]);
+        double x0 = (double)bxo[ib] - 0.5 * d, y0 = (double)byo[ib] - 0.5 * d, L = BW * d, q = 0.25 * d;
+        const double px[8] = {x0 - q, x0 - q, x0 + L + q, x0 + L + q, x0 + q, x0 + L - q, x0 + q, x0 + L - q};
+        const double py[8] = {y0 + q, y0 + L - q, y0 + q, y0 + L - q, y0 + L + q, y0 + L + q, y0 - q, y0 - q};
+        for (int s = 0; s < 8; s++) {
+            if (px[s] < 0 || py[s] < 0 || px[s] > W || py[s] > H) continue;
+            for (int m = 0; m < n2; m++)
+                if (contains(act[m], px[s], py[s])) { nb[s][ib] = act[m]; break; }
+        }
+    }
+    for (int k = 0; k < n2; k++) active[k] = act[k];
+    *nblk_out = n2;
+    *nblkmem_out = nblkmem;
+    return 0;
+}
+}  // namespace
+
+extern "C" int orc_adapt_refine(const orc_param* p, const orc_blocks* b, const unsigned char* flags, int capacity,
+                                int* nblk_out, int* nblkmem_out, int* active, int* level, double* xo, double* yo,
+                                int* LB, int* LT, int* RB, int* RT, int* TL, int* TR, int* BL, int* BR) {
+    int* nb[8] = {LB, LT, RB, RT, TL, TR, BL, BR};
+    if (p->doubleprecision) return adapt_refine_t<double>(p, b, flags, capacity, nblk_out, nblkmem_out, active, level, xo, yo, nb);
+    return adapt_refine_t<float>(p, b, flags, capacity, nblk_out, nblkmem_out, active, level, xo, yo, nb);
+}

@@ -74,6 +74,13 @@ double orc_kurgsolver_d(double g, double delta, double epsi, double CFL, double 
 float  orc_kurgsolver_f(float g, float delta, float epsi, float CFL, float cm, float fm,
                         float hp, float hm, float up, float um, float* fh, float* fu);
 double orc_calcres_d(double dx, int level);
+/* One refinement pass of the adaptation (src/Adaptation.cu; coarsening is
+ * cancelled by the reference): returns 0, or -1 if nblkmem would exceed
+ * capacity.  Outputs have capacity entries; neighbour order LB LT RB RT TL
+ * TR BL BR.  The level-0 domain is p->nx x p->ny cells of p->dx. */
+int orc_adapt_refine(const orc_param* p, const orc_blocks* b, const unsigned char* flags, int capacity,
+                     int* nblk_out, int* nblkmem_out, int* active, int* level, double* xo, double* yo,
+                     int* LB, int* LT, int* RB, int* RT, int* TL, int* TR, int* BL, int* BR);
 /* Spherical metric terms (src/Spherical.cu:15-64). */
 double orc_calccm_d(double Radius, double delta, double yo, int iy);
 double orc_calcfm_d(double Radius, double delta, double yo, double iy);

@@ -89,6 +89,9 @@ def lib():
         L.orc_kurgsolver_f.argtypes = [ctypes.c_float] * 10 + [fp, fp]
         L.orc_calcres_d.restype = ctypes.c_double
         L.orc_calcres_d.argtypes = [ctypes.c_double, ctypes.c_int]
+        L.orc_adapt_refine.restype = ctypes.c_int
+        L.orc_adapt_refine.argtypes = [ctypes.POINTER(OrcParam), ctypes.POINTER(OrcBlocks), ctypes.c_void_p,
+                                       ctypes.c_int, ip, ip, ip, ip, dp, dp] + [ip] * 8
         L.orc_calccm_d.restype = ctypes.c_double
         L.orc_calccm_d.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.orc_calcfm_d.restype = ctypes.c_double
@@ -212,3 +215,42 @@ class Oracle:
 
     def volume(self):
         return lib().orc_volume(self.h)
+
+
+def adapt_refine(p, blocks, flags, capacity):
+    """One refinement pass of the oracle (orc_adapt_refine): (blocks, nblk, nblkmem)."""
+    nbm = p.nblkmem
+    keep = {}
+    b = OrcBlocks()
+    for k in ("active", "level", "LeftBot", "LeftTop", "RightBot", "RightTop", "TopLeft", "TopRight",
+              "BotLeft", "BotRight"):
+        a = np.zeros(nbm, np.int32)
+        src = np.asarray(blocks[k], np.int32)
+        a[:len(src)] = src
+        if k == "active":
+            a[len(src):] = -1
+        elif k != "level":
+            a[len(src):] = np.arange(len(src), nbm)
+        keep[k] = a
+        setattr(b, k, _iptr(a))
+    for k in ("xo", "yo"):
+        a = np.zeros(nbm)
+        a[:len(blocks[k])] = blocks[k]
+        keep[k] = a
+        setattr(b, k, _dptr(a))
+    f = np.zeros(nbm, np.uint8)
+    f[:len(flags)] = np.asarray(flags, np.uint8)
+    out = {k: np.zeros(capacity, np.int32) for k in ("active", "level", "LeftBot", "LeftTop", "RightBot",
+                                                     "RightTop", "TopLeft", "TopRight", "BotLeft", "BotRight")}
+    out["xo"] = np.zeros(capacity)
+    out["yo"] = np.zeros(capacity)
+    nb, nm = ctypes.c_int(), ctypes.c_int()
+    rc = lib().orc_adapt_refine(ctypes.byref(p), ctypes.byref(b), f.ctypes.data, capacity, ctypes.byref(nb),
+                                ctypes.byref(nm), _iptr(out["active"]), _iptr(out["level"]), _dptr(out["xo"]),
+                                _dptr(out["yo"]), _iptr(out["LeftBot"]), _iptr(out["LeftTop"]),
+                                _iptr(out["RightBot"]), _iptr(out["RightTop"]), _iptr(out["TopLeft"]),
+                                _iptr(out["TopRight"]), _iptr(out["BotLeft"]), _iptr(out["BotRight"]))
+    assert rc == 0, "capacity exceeded"
+    res = {k: v[:nm.value].copy() for k, v in out.items()}
+    res["active"] = res["active"][:nb.value]
+    return res, nb.value, nm.value

@@ -135,3 +135,18 @@ def test_spherical_exact(prec, maxlevel, aoibnd):
                                    maxlevel=maxlevel)
     d, o = run_both(sc, 30, prec)
     assert_same(sc, d, o)
+
+
+@pytest.mark.parametrize("field,mode,a,b", [("h", 0, 10.2, 0), ("zs", 1, 0.1, 0.3), ("zb", 0, -5.0, 0)])
+def test_refine_flags_match_the_criteria(field, mode, a, b):
+    """Threshold and Inrange criteria (src/AdaptCriteria.cu:62-160) evaluated on the GPU state."""
+    sc = scenarios.amr_basin(precision="fp64", island=True)
+    d = domain_for(sc)
+    f = d.refine_flags(field, mode, a, b)
+    st = d.get_state()
+    z = {"h": st["h"], "zs": st["h"] + sc["zb"], "zb": sc["zb"]}[field].reshape(-1, 18, 18)
+    p = sc["param"]
+    for ib in sc["blocks"]["active"][:p["nblk"]]:
+        c = z[ib, 1:17, 1:17]
+        want = (c > a).any() if mode == 0 else ((c >= a) & (c <= b)).any()
+        assert bool(f[ib]) == bool(want), ib

@@ -1,5 +1,7 @@
 """Pins of the oracle's whole step (CPU): the reference's stored Kurganov transect,
 symmetry, lake at rest, mass and rain volume balances."""
+import math
+
 import numpy as np
 import pytest
 
@@ -106,3 +108,53 @@ def test_rain_volume_closed_basin():
     area = (32 * 5.0) ** 2
     added = 36.0 / 1000.0 / 3600.0 * loop.totaltime * area
     assert abs((o.volume() - v0) - added) / added < 1e-9
+
+
+def _uniform_flow(frictionmodel, cf, lat=0.0, u0=0.8, v0=-0.3, h0=2.0, n=32):
+    """Uniform depth and velocity over a flat bed with Neumann edges: every face carries the same
+    flux, so the flux divergence is exactly zero and one step isolates the source terms."""
+    p, blocks, nbx, nby = scenarios.uniform(n, n, 10.0, "fp64", aoibnd=1, frictionmodel=frictionmodel, cf=cf,
+                                            lat=lat)
+    S = p["nblkmem"] * 324
+    m = np.zeros(S, bool)
+    m.reshape(-1, 18, 18)[:p["nblk"], 1:17, 1:17] = True
+    sc = dict(param=p, blocks=blocks, nbx=nbx, nby=nby, zb=np.where(m, -h0, 0.0), h=np.where(m, h0, 0.0),
+              u=np.where(m, u0, 0.0), v=np.where(m, v0, 0.0), cf=np.full(S, cf))
+    o = oracle_for(sc)
+    L = po.OrcLoop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+    o.init_loop(L)
+    o.step(L)
+    return o, L, m
+
+
+@pytest.mark.parametrize("model,cf", [(0, 0.003), (1, 0.001), (-1, 0.03)])
+def test_bottom_friction_closed_forms(model, cf):
+    """u1 = u/(1 + cf_eff |u| dt / h) with cf_eff = cf (quadratic), Smart's law, or Manning g n^2/h^(1/3)
+    (src/Friction.cu:89-148, 317-351)."""
+    u0, v0, h0 = 0.8, -0.3, 2.0
+    o, L, m = _uniform_flow(model, cf, u0=u0, v0=v0, h0=h0)
+    dt = L.dt
+    if model == 0:
+        c = cf
+    elif model == 1:
+        hbar = h0 / cf
+        c = (1.0 / (0.46 * hbar) if hbar <= 2.71828182845905 else 1.0 / (2.5 * (math.log(hbar) - 1.0 + 1.359 / hbar))) ** 2
+    else:
+        c = 9.81 * cf * cf / h0 ** (1.0 / 3.0)
+    tb = c * math.hypot(u0, v0) / h0 * dt
+    np.testing.assert_allclose(o.get("u")[m], u0 / (1.0 + tb), rtol=1e-13)
+    np.testing.assert_allclose(o.get("v")[m], v0 / (1.0 + tb), rtol=1e-13)
+    np.testing.assert_allclose(o.get("h")[m], h0, rtol=0, atol=0)
+
+
+def test_coriolis_rotation_closed_form():
+    """du/dt = f v, dv/dt = -f u with f = sin(lat) pi / 21600 (src/Advection.cu:187-250), integrated by
+    the predictor-corrector: u* = u + dt/2 f v, v* = v - dt/2 f u; u1 = u + dt f v*, v1 = v - dt f u*."""
+    u0, v0, lat = 0.8, -0.3, 45.0
+    o, L, m = _uniform_flow(0, 0.0, lat=lat, u0=u0, v0=v0)
+    pi = 3.14159265
+    f = math.sin(lat * pi / 180.0) * pi / 21600.0
+    dt = L.dt
+    us, vs = u0 + 0.5 * dt * f * v0, v0 - 0.5 * dt * f * u0
+    np.testing.assert_allclose(o.get("u")[m], u0 + dt * f * vs, rtol=1e-13)
+    np.testing.assert_allclose(o.get("v")[m], v0 - dt * f * us, rtol=1e-13)
