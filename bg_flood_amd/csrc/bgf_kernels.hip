@@ -292,8 +292,14 @@ __device__ __forceinline__ Q4<T> slopes(const Cst<T>& C, const Q4<T>& m, const Q
     return g;
 }
 
-// Slopes of the tile: every thread its own cell along x and y, wave 0 the
-// halo ring (x slopes at x = -1, 16 and y slopes at y = -1, 16).
+// Extra work of a tile beyond one cell per thread goes to distinct waves, so
+// that no wave runs more than one extra section between two barriers: the x
+// halo-ring slopes to wave 0, the y halo-ring slopes to wave 1, the 17th X
+// faces to wave 2 and the 17th Y faces to wave 3.
+__device__ __forceinline__ int ring_slot(int t) { return t < 32 ? t : (t >= 64 && t < 96 ? t - 32 : -1); }
+
+// Slopes of the tile: every thread its own cell along x and y; the halo ring
+// (x slopes at x = -1, 16 and y slopes at y = -1, 16) on waves 0 and 1.
 template <class T, bool FAST, bool DEV>
 __device__ __forceinline__ void tile_gradients(Tile<T>& s, const Cst<T>& C, int t) {
     {
@@ -307,8 +313,9 @@ __device__ __forceinline__ void tile_gradients(Tile<T>& s, const Cst<T>& C, int 
         gs.a = g.a; gs.b = g.b; gs.c = g.d; gs.d = g.c;
         s.gy[gyi(x, y)] = gs;
     }
-    if (t < 64) {
-        const int j = t & 15, side = t >> 4;
+    const int r = ring_slot(t);
+    if (r >= 0) {
+        const int j = r & 15, side = r >> 4;
         if (side < 2) {
             const int x = side == 0 ? -1 : BW;
             const int c = tix(x, j);
@@ -365,8 +372,8 @@ __device__ __forceinline__ T tile_fluxes(Tile<T>& s, const Cst<T>& C, int t, boo
     T dtx = T(0.0), dty = T(0.0), dtd;
     xface<T, FAST, WANT_DT, DEV>(s, C, x, y, dtx);
     bool wy = yface<T, FAST, WANT_DT, DEV>(s, C, x, y, dty);
-    if (t < 16) xface<T, FAST, false, DEV>(s, C, BW, t, dtd);
-    else if (t < 32) yface<T, FAST, false, DEV>(s, C, t - 16, BW, dtd);
+    if (t >= 128 && t < 144) xface<T, FAST, false, DEV>(s, C, BW, t - 128, dtd);
+    else if (t >= 192 && t < 208) yface<T, FAST, false, DEV>(s, C, t - 192, BW, dtd);
     T dtc = T(0.0);
     if (WANT_DT) dtc = wy ? (dty < dtx ? dty : dtx) : C.inv_epsi;
     __syncthreads();
