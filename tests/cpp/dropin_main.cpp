@@ -1,0 +1,78 @@
+// Drives bgflood_amd::Flow<T> the way BG_Flood's MainLoop drives FlowGPU<T>, with stand-in
+// types that carry the field names of the reference's Param, Loop<T>, BlockP<T>, EvolvingP<T>
+// and Model<T>.  Gaussian hump on 32x32 cells (4 blocks), walls, 5 steps.  Prints "no device"
+// and exits 3 without a GPU; otherwise prints the dt of each step and the final h of cell
+// (5, 7) of block 0, one value per line, in full precision.
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "bgflood_amd.hpp"
+
+struct Param {
+    double g = 9.81, eps = 1e-4, CFL = 0.5, theta = 1.3, dx = 10.0, delta = 10.0, xo = 0, yo = 0, lat = 0,
+           mask = 9999.0, Radius = 6371220.0;
+    int frictionmodel = 0, engine = 2, aoibnd = 0, nblk = 4, nblkmem = 5, minlevel = 0, maxlevel = 0;
+    bool ForceMassConserve = false, wetdryfix = true, spherical = false;
+};
+template <class T> struct Loop { double nextoutputtime = 1e9, dt = 0, dtmax = 0, totaltime = 0; };
+template <class T> struct BlockP {
+    int *active, *level, *LeftBot, *LeftTop, *RightBot, *RightTop, *TopLeft, *TopRight, *BotLeft, *BotRight;
+    T *xo, *yo;
+};
+template <class T> struct EvolvingP { T *h, *zs, *u, *v; };
+template <class T> struct Model {
+    BlockP<T> blocks;
+    EvolvingP<T> evolv;
+    T *zb, *cf;
+};
+
+int main() {
+    typedef double T;
+    Param XParam;
+    const int nbm = XParam.nblkmem, S = nbm * 324;
+    std::vector<int> iarr[10];
+    for (auto& a : iarr) a.assign(nbm, 0);
+    std::vector<double> dxo(nbm), dyo(nbm);
+    if (bgf_mesh_uniform(2, 2, XParam.dx, 0, 1, iarr[0].data(), iarr[1].data(), dxo.data(), dyo.data(),
+                         iarr[2].data(), iarr[3].data(), iarr[4].data(), iarr[5].data(), iarr[6].data(),
+                         iarr[7].data(), iarr[8].data(), iarr[9].data()) != BGF_OK)
+        return 2;
+    for (int k = 2; k < 10; k++) iarr[k][4] = 4;  // the spare block points at itself
+    std::vector<T> xo(dxo.begin(), dxo.end()), yo(dyo.begin(), dyo.end());
+    std::vector<T> h(S, 0), zs(S, 0), u(S, 0), v(S, 0), zb(S, -10.0), cf(S, 0);
+    for (int ib = 0; ib < 4; ib++)
+        for (int j = 0; j < 16; j++)
+            for (int i = 0; i < 16; i++) {
+                const int n = (i + 1) + (j + 1) * 18 + ib * 324;
+                const double x = xo[ib] + i * XParam.dx - 160.0, y = yo[ib] + j * XParam.dx - 160.0;
+                h[n] = 10.0 + std::exp(-(x * x + y * y) / (2.0 * 60.0 * 60.0));
+            }
+    Model<T> XModel;
+    XModel.blocks = {iarr[0].data(), iarr[1].data(), iarr[2].data(), iarr[3].data(), iarr[4].data(),
+                     iarr[5].data(), iarr[6].data(), iarr[7].data(), iarr[8].data(), iarr[9].data(), xo.data(),
+                     yo.data()};
+    XModel.evolv = {h.data(), zs.data(), u.data(), v.data()};
+    XModel.zb = zb.data();
+    XModel.cf = cf.data();
+    bgflood_amd::Flow<T> flow;
+    bgf_status s = flow.init(XParam, XModel);
+    if (s == BGF_ERR_NO_DEVICE) {
+        std::printf("no device\n");
+        return 3;
+    }
+    if (s != BGF_OK) {
+        std::printf("init failed: %s\n", bgf_last_error());
+        return 1;
+    }
+    Loop<T> XLoop;
+    if (flow.init_dt(XLoop) != BGF_OK) return 1;
+    for (int k = 0; k < 5; k++) {
+        if (bgflood_amd::FlowGPU(flow, XLoop) != BGF_OK) return 1;
+        XLoop.totaltime += XLoop.dt;
+        std::printf("%.17g\n", XLoop.dt);
+    }
+    if (flow.download(XModel) != BGF_OK) return 1;
+    std::printf("%.17g\n", h[(5 + 1) + (7 + 1) * 18]);
+    return 0;
+}
