@@ -121,6 +121,10 @@ def lib():
         L.bgf_batch_track_maxdepth.argtypes = [vp, ctypes.c_int]
         L.bgf_batch_sample_gauges.argtypes = [vp, vp, vp]
         L.bgf_batch_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ip, ip]
+        L.bgf_set_schedule.argtypes = [vp] + [ctypes.c_int] * 4
+        L.bgf_get_schedule.argtypes = [vp] + [ip] * 6
+        L.bgf_batch_set_schedule.argtypes = [vp] + [ctypes.c_int] * 4
+        L.bgf_batch_get_schedule.argtypes = [vp] + [ip] * 4
         for name in ("bgf_create", "bgf_destroy", "bgf_set_state", "bgf_get_state", "bgf_init_dt", "bgf_step",
                      "bgf_run", "bgf_run_async", "bgf_sync", "bgf_volume", "bgf_last_run_steps", "bgf_adapt_refine",
                      "bgf_refine_flags", "bgf_device_array", "bgf_set_timing",
@@ -129,7 +133,8 @@ def lib():
                      "bgf_batch_sync", "bgf_batch_loop", "bgf_batch_get_state", "bgf_batch_volume",
                      "bgf_batch_update_maxdepth", "bgf_batch_get_maxdepth", "bgf_batch_set_gauges",
                      "bgf_batch_sample_gauges", "bgf_batch_stats", "bgf_batch_get_gauges",
-                     "bgf_batch_track_maxdepth"):
+                     "bgf_batch_track_maxdepth", "bgf_set_schedule", "bgf_get_schedule",
+                     "bgf_batch_set_schedule", "bgf_batch_get_schedule"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -342,6 +347,15 @@ class Domain:
         _check(lib().bgf_kernel_times(self.h, 8, names, ms, ctypes.byref(n)))
         return {names[i].decode(): ms[i] for i in range(n.value)}
 
+    def set_schedule(self, tile_x=0, tile_y=0, wg_pred=0, wg_corr=0):
+        """Launch schedule of the persistent kernels (results do not depend on it)."""
+        _check(lib().bgf_set_schedule(self.h, int(tile_x), int(tile_y), int(wg_pred), int(wg_corr)))
+
+    def schedule(self):
+        v = [ctypes.c_int() for _ in range(6)]
+        _check(lib().bgf_get_schedule(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("tile_x", "tile_y", "wg_pred", "wg_corr", "grid_pred", "grid_corr"), [x.value for x in v]))
+
 
 class Batch:
     """Many one-level domains advanced together on the GPU (a bgf_batch): members are added with
@@ -457,6 +471,14 @@ class Batch:
         nb = ctypes.c_int()
         _check(lib().bgf_batch_stats(self.h, ctypes.byref(cu), ctypes.byref(nm), ctypes.byref(nb)))
         return dict(cell_updates=cu.value, members=nm.value, blocks=nb.value)
+
+    def set_schedule(self, tile_x=0, tile_y=0, wg_pred=0, wg_corr=0):
+        _check(lib().bgf_batch_set_schedule(self.h, int(tile_x), int(tile_y), int(wg_pred), int(wg_corr)))
+
+    def schedule(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        _check(lib().bgf_batch_get_schedule(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("tile_x", "tile_y", "wg_pred", "wg_corr"), [x.value for x in v]))
 
 
 def interior_mask(nblkmem, active):

@@ -29,6 +29,7 @@ struct Member {
     int nblkmem = 0;  // block slots
     std::vector<int> active;  // global block ids, in the member's own active order
     std::vector<int> order;   // global block ids, tiled processing order
+    BlockXY xy;               // block coordinates (global ids), for re-tiling
     double rain = 0.0;
     std::vector<int> gauges;  // global cell indices
 };
@@ -63,6 +64,7 @@ struct bgf_batch_s {
     long long cell_updates = 0;
     int nactive_blocks = 0;
     int wg_pred = 0, wg_corr = 0;
+    int tile_x = TILE_X, tile_y = TILE_Y;
     bool use_graph = true;
     hipGraphExec_t graph = nullptr;  // an even/odd pair of batch steps; rebuilt after membership changes
     // the persistent kernels walk only the blocks of members still running:
@@ -336,7 +338,9 @@ bgf_status bgf_batch_add(bgf_batch b, const bgf_param* p, const bgf_blocks* blk,
     M.nblkmem = nbm;
     M.rain = (double)(float)p->rain;
     for (int k = 0; k < nblk; k++) M.active.push_back(off + blk->active[k]);
-    for (int ib : processing_order(blk, nblk, p->dx)) M.order.push_back(off + ib);
+    M.xy = block_xy(blk, nblk, p->dx);
+    for (int& ib : M.xy.id) ib += off;
+    M.order = processing_order(M.xy, b->tile_x, b->tile_y);
     for (int k = 0; k < nbm; k++) b->slot_owner[off + k] = m;
     auto undo = [&](bgf_status e) {
         for (int k = 0; k < nbm; k++) b->slot_owner[off + k] = -1;
@@ -616,6 +620,34 @@ bgf_status bgf_batch_stats(bgf_batch b, long long* cell_updates, int* nmembers, 
     if (cell_updates) *cell_updates = b->cell_updates;
     if (nmembers) *nmembers = n;
     if (nblocks) *nblocks = b->nactive_blocks;
+    return BGF_OK;
+}
+
+bgf_status bgf_batch_set_schedule(bgf_batch b, int tile_x, int tile_y, int wg_pred, int wg_corr) {
+    if (!b) return fail(BGF_ERR_INVALID_ARG, "null batch");
+    if (b->pending) return fail(BGF_ERR_STATE, "bgf_batch_run_async pending: call bgf_batch_sync first");
+    if (tile_x < 0 || tile_y < 0 || tile_x > 4096 || tile_y > 4096 || wg_pred < 0 || wg_corr < 0 || wg_pred > 64 ||
+        wg_corr > 64)
+        return fail(BGF_ERR_INVALID_ARG, "schedule out of range");
+    const int tx = tile_x ? tile_x : b->tile_x, ty = tile_y ? tile_y : b->tile_y;
+    if (tx != b->tile_x || ty != b->tile_y) {
+        for (auto& M : b->members)
+            if (M.used) M.order = processing_order(M.xy, tx, ty);
+        b->tile_x = tx;
+        b->tile_y = ty;
+    }
+    b->wg_pred = wg_pred;
+    b->wg_corr = wg_corr;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return rebuild(b);  // re-uploads the order; the run set, grids and graph are redone on the next run
+}
+
+bgf_status bgf_batch_get_schedule(bgf_batch b, int* tile_x, int* tile_y, int* wg_pred, int* wg_corr) {
+    if (!b) return fail(BGF_ERR_INVALID_ARG, "null batch");
+    if (tile_x) *tile_x = b->tile_x;
+    if (tile_y) *tile_y = b->tile_y;
+    if (wg_pred) *wg_pred = b->wg_pred;
+    if (wg_corr) *wg_corr = b->wg_corr;
     return BGF_OK;
 }
 

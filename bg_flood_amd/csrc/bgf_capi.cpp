@@ -53,6 +53,9 @@ struct bgf_handle_s {
     bool use_graph = true;
     hipGraphExec_t graph = nullptr;  // one period of steps, captured on first use
     int last_steps = 0;              // steps taken by the last bgf_run / bgf_sync
+    // launch schedule of the persistent one-level kernels
+    BlockXY xy;
+    int tile_x = TILE_X, tile_y = TILE_Y, wg_pred = 0, wg_corr = 0;
 };
 
 namespace {
@@ -304,7 +307,8 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
         return fail(BGF_ERR_OOM, "device allocation failed");
     }
     const int* nbs[8] = {b->LeftBot, b->LeftTop, b->RightBot, b->RightTop, b->TopLeft, b->TopRight, b->BotLeft, b->BotRight};
-    std::vector<int> order = processing_order(b, nblk, p->dx);
+    h->xy = block_xy(b, nblk, p->dx);
+    std::vector<int> order = processing_order(h->xy);
     bool cp = hipMemcpy(h->d_active, b->active, sizeof(int) * nblk, hipMemcpyHostToDevice) == hipSuccess;
     cp = cp && hipMemcpy(h->d_order, order.data(), sizeof(int) * nblk, hipMemcpyHostToDevice) == hipSuccess;
     cp = cp && hipMemcpy(h->d_level, b->level, sizeof(int) * nbm, hipMemcpyHostToDevice) == hipSuccess;
@@ -344,11 +348,10 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
     kp.maxlevel = maxlev;
     // BGF_WG_PER_CU="p,c" caps the persistent workgroups per CU of the
     // predictor and the corrector (a tuning override; default: occupancy).
-    int wp = 0, wc = 0;
-    if (const char* e = std::getenv("BGF_WG_PER_CU")) std::sscanf(e, "%d,%d", &wp, &wc);
+    if (const char* e = std::getenv("BGF_WG_PER_CU")) std::sscanf(e, "%d,%d", &h->wg_pred, &h->wg_corr);
     if (const char* e = std::getenv("BGF_GRAPH")) h->use_graph = std::atoi(e) != 0;
-    kp.grid_pred = bgf::persistent_grid(h->prec, 0, nblk, wp);
-    kp.grid_corr = bgf::persistent_grid(h->prec, 1, nblk, wc);
+    kp.grid_pred = bgf::persistent_grid(h->prec, 0, nblk, h->wg_pred);
+    kp.grid_corr = bgf::persistent_grid(h->prec, 1, nblk, h->wg_corr);
     if (kp.grid_pred <= 0 || kp.grid_corr <= 0) {
         free_handle(h);
         return fail(BGF_ERR_HIP, "occupancy query failed");
@@ -568,6 +571,47 @@ bgf_status bgf_device_array(bgf_handle h, const char* name, void** devptr) {
     else if (n == "dhv") p = h->d_dhv;
     else return fail(BGF_ERR_INVALID_ARG, "unknown array " + n);
     *devptr = p;
+    return BGF_OK;
+}
+
+bgf_status bgf_set_schedule(bgf_handle h, int tile_x, int tile_y, int wg_pred, int wg_corr) {
+    if (!h) return fail(BGF_ERR_INVALID_ARG, "null handle");
+    if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
+    if (tile_x < 0 || tile_y < 0 || tile_x > 4096 || tile_y > 4096 || wg_pred < 0 || wg_corr < 0 || wg_pred > 64 ||
+        wg_corr > 64)
+        return fail(BGF_ERR_INVALID_ARG, "schedule out of range");
+    const int tx = tile_x ? tile_x : h->tile_x, ty = tile_y ? tile_y : h->tile_y;
+    if (tx != h->tile_x || ty != h->tile_y) {
+        std::vector<int> order = processing_order(h->xy, tx, ty);
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipMemcpy(h->d_order, order.data(), sizeof(int) * h->nblk, hipMemcpyHostToDevice));
+        h->tile_x = tx;
+        h->tile_y = ty;
+    }
+    const int gp = bgf::persistent_grid(h->prec, 0, h->nblk, wg_pred);
+    const int gc = bgf::persistent_grid(h->prec, 1, h->nblk, wg_corr);
+    if (gp <= 0 || gc <= 0) return fail(BGF_ERR_HIP, "occupancy query failed");
+    h->wg_pred = wg_pred;
+    h->wg_corr = wg_corr;
+    h->kp.grid_pred = gp;
+    h->kp.grid_corr = gc;
+    if (h->graph) {
+        (void)hipGraphExecDestroy(h->graph);
+        h->graph = nullptr;
+    }
+    g_err.clear();
+    return BGF_OK;
+}
+
+bgf_status bgf_get_schedule(bgf_handle h, int* tile_x, int* tile_y, int* wg_pred, int* wg_corr, int* grid_pred,
+                            int* grid_corr) {
+    if (!h) return fail(BGF_ERR_INVALID_ARG, "null handle");
+    if (tile_x) *tile_x = h->tile_x;
+    if (tile_y) *tile_y = h->tile_y;
+    if (wg_pred) *wg_pred = h->wg_pred;
+    if (wg_corr) *wg_corr = h->wg_corr;
+    if (grid_pred) *grid_pred = h->kp.grid_pred;
+    if (grid_corr) *grid_corr = h->kp.grid_corr;
     return BGF_OK;
 }
 

@@ -112,34 +112,50 @@ template <class T> inline double coriolis(double lat) {
     return (double)T(std::sin(T(lat * pi / 180.0)) * pi / T(21600.0));
 }
 
-// Processing order of the persistent kernels: the active blocks grouped in
-// tiles of TILE_X x TILE_Y blocks (row-major tiles, row-major blocks inside a
-// tile), by their integer block coordinates.  Each XCD processes consecutive
-// positions of this order, so the blocks it has in flight at once form a
-// compact patch whose neighbours are mostly in its own L2.
-constexpr int TILE_X = 16, TILE_Y = 12;
-inline std::vector<int> processing_order(const bgf_blocks* b, int nblk, double dx) {
-    std::vector<int> order(b->active, b->active + nblk);
-    std::vector<long long> key(nblk);
-    long long maxbx = 1;
-    std::vector<long long> bx(nblk), by(nblk);
+// Integer block coordinates of the active blocks (position among the blocks
+// of their level), in active-list order.
+struct BlockXY {
+    std::vector<int> id;
+    std::vector<long long> bx, by;
+};
+inline BlockXY block_xy(const bgf_blocks* b, int nblk, double dx) {
+    BlockXY xy;
+    xy.id.assign(b->active, b->active + nblk);
+    xy.bx.resize(nblk);
+    xy.by.resize(nblk);
     for (int k = 0; k < nblk; k++) {
         const int ib = b->active[k];
         const int lev = b->level[ib];
         const double levdx = lev < 0 ? dx * (1 << -lev) : dx / (1 << lev);
-        bx[k] = (long long)std::floor(b->xo[ib] / (BW * levdx));
-        by[k] = (long long)std::floor(b->yo[ib] / (BW * levdx));
-        if (bx[k] + 1 > maxbx) maxbx = bx[k] + 1;
+        xy.bx[k] = (long long)std::floor(b->xo[ib] / (BW * levdx));
+        xy.by[k] = (long long)std::floor(b->yo[ib] / (BW * levdx));
     }
-    const long long ntx = (maxbx + TILE_X - 1) / TILE_X;
+    return xy;
+}
+
+// Processing order of the persistent kernels: the active blocks grouped in
+// tiles of tile_x x tile_y blocks (row-major tiles, row-major blocks inside a
+// tile), by their integer block coordinates.  Each XCD processes consecutive
+// positions of this order, so the blocks it has in flight at once form a
+// compact patch whose neighbours are mostly in its own L2.  The tile is part
+// of the launch schedule (bgf_set_schedule); TILE_X x TILE_Y is the default.
+constexpr int TILE_X = 16, TILE_Y = 12;
+inline std::vector<int> processing_order(const BlockXY& xy, int tile_x = TILE_X, int tile_y = TILE_Y) {
+    const int nblk = (int)xy.id.size();
+    long long maxbx = 1;
+    for (int k = 0; k < nblk; k++)
+        if (xy.bx[k] + 1 > maxbx) maxbx = xy.bx[k] + 1;
+    const long long ntx = (maxbx + tile_x - 1) / tile_x;
+    std::vector<long long> key(nblk);
     for (int k = 0; k < nblk; k++) {
-        const long long tx = bx[k] / TILE_X, ty = by[k] / TILE_Y;
-        key[k] = ((ty * ntx + tx) * TILE_Y + by[k] % TILE_Y) * TILE_X + bx[k] % TILE_X;
+        const long long tx = xy.bx[k] / tile_x, ty = xy.by[k] / tile_y;
+        key[k] = ((ty * ntx + tx) * tile_y + xy.by[k] % tile_y) * tile_x + xy.bx[k] % tile_x;
     }
     std::vector<int> idx(nblk);
     for (int k = 0; k < nblk; k++) idx[k] = k;
     std::stable_sort(idx.begin(), idx.end(), [&](int a, int c) { return key[a] < key[c]; });
-    for (int k = 0; k < nblk; k++) order[k] = b->active[idx[k]];
+    std::vector<int> order(nblk);
+    for (int k = 0; k < nblk; k++) order[k] = xy.id[idx[k]];
     return order;
 }
 

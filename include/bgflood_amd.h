@@ -147,6 +147,18 @@ bgf_status bgf_volume(bgf_handle h, double* volume);
  * "cf", "dh", "dhu", "dhv"), for callers that keep Model<T> views. */
 bgf_status bgf_device_array(bgf_handle h, const char* name, void** devptr);
 
+/* Launch schedule of the persistent one-level kernels (no counterpart in the
+ * reference, whose launches are one block per workgroup, src/FlowGPU.cu:14-315):
+ * the active blocks are processed in tiles of tile_x x tile_y blocks, and the
+ * predictor and corrector run wg_pred / wg_corr persistent workgroups per CU
+ * (0 = as many as occupancy allows).  tile_x or tile_y 0 keeps the current
+ * tile.  Results do not depend on the schedule; bg_flood_amd.autotune picks
+ * one by timing.  Defaults: 16 x 12 blocks, occupancy; the environment
+ * variable BGF_WG_PER_CU="p,c" sets the workgroups at creation. */
+bgf_status bgf_set_schedule(bgf_handle h, int tile_x, int tile_y, int wg_pred, int wg_corr);
+bgf_status bgf_get_schedule(bgf_handle h, int* tile_x, int* tile_y, int* wg_pred, int* wg_corr, int* grid_pred,
+                            int* grid_corr);
+
 /* Average duration, in milliseconds, of each kernel of the last bgf_run
  * call, measured with HIP events on the handle's stream when timing was
  * enabled with bgf_set_timing(h, 1).  names/ms receive up to n entries;
@@ -261,6 +273,11 @@ bgf_status bgf_batch_sample_gauges(bgf_batch b, float* values, double* times);
 
 /* Cell-updates performed so far, members and active blocks now. */
 bgf_status bgf_batch_stats(bgf_batch b, long long* cell_updates, int* nmembers, int* nblocks);
+
+/* Launch schedule of the batch's step, as bgf_set_schedule; a member's blocks
+ * stay together and are tiled inside the member. */
+bgf_status bgf_batch_set_schedule(bgf_batch b, int tile_x, int tile_y, int wg_pred, int wg_corr);
+bgf_status bgf_batch_get_schedule(bgf_batch b, int* tile_x, int* tile_y, int* wg_pred, int* wg_corr);
 
 #ifdef __cplusplus
 }
