@@ -50,6 +50,13 @@ class BgfBlocks(ctypes.Structure):
                [("xo", ctypes.POINTER(ctypes.c_double)), ("yo", ctypes.POINTER(ctypes.c_double))]
 
 
+class BgfBlocksOut(ctypes.Structure):
+    _fields_ = [(k, ctypes.POINTER(ctypes.c_int)) for k in ("active", "level", "LeftBot", "LeftTop", "RightBot",
+                                                             "RightTop", "TopLeft", "TopRight", "BotLeft",
+                                                             "BotRight")] + \
+               [("xo", ctypes.POINTER(ctypes.c_double)), ("yo", ctypes.POINTER(ctypes.c_double))]
+
+
 class Loop(ctypes.Structure):
     """Loop<T> time keeping: nextoutputtime, dt, dtmax, totaltime."""
     _fields_ = [("nextoutputtime", ctypes.c_double), ("dt", ctypes.c_double),
@@ -121,6 +128,11 @@ def lib():
         L.bgf_batch_track_maxdepth.argtypes = [vp, ctypes.c_int]
         L.bgf_batch_sample_gauges.argtypes = [vp, vp, vp]
         L.bgf_batch_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ip, ip]
+        L.bgf_adapt_mesh.argtypes = [ctypes.POINTER(BgfParam), ctypes.POINTER(BgfBlocks), vp, vp, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, ip, ip, ctypes.POINTER(BgfBlocksOut), ip]
+        L.bgf_remesh.argtypes = [vp, ctypes.POINTER(BgfParam), ctypes.POINTER(BgfBlocks), ip]
+        L.bgf_set_bed_source.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, vp]
         L.bgf_set_schedule.argtypes = [vp] + [ctypes.c_int] * 4
         L.bgf_get_schedule.argtypes = [vp] + [ip] * 6
         L.bgf_batch_set_schedule.argtypes = [vp] + [ctypes.c_int] * 4
@@ -134,7 +146,8 @@ def lib():
                      "bgf_batch_update_maxdepth", "bgf_batch_get_maxdepth", "bgf_batch_set_gauges",
                      "bgf_batch_sample_gauges", "bgf_batch_stats", "bgf_batch_get_gauges",
                      "bgf_batch_track_maxdepth", "bgf_set_schedule", "bgf_get_schedule",
-                     "bgf_batch_set_schedule", "bgf_batch_get_schedule"):
+                     "bgf_batch_set_schedule", "bgf_batch_get_schedule", "bgf_adapt_mesh", "bgf_remesh",
+                     "bgf_set_bed_source"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -206,6 +219,39 @@ def adapt_refine(param, blocks, flags, nbx, nby, capacity=None):
     return res, nb.value, nm.value
 
 
+def adapt_mesh(param, blocks, refine, coarsen, nbx, nby, capacity=None):
+    """One adaptation pass with the reference's rules and block numbering (bgf_adapt_mesh): refine[ib]
+    asks block ib to split, coarsen[ib] (or coarsen=None: never, as the reference) to merge with its
+    siblings.  Returns (blocks, nblk, nblkmem, remap) of the new mesh; remap is (nblkmem, 5):
+    kind (-1 inactive, 0 kept, 1 child, 2 coarsened) and the source blocks."""
+    nbm = int(param["nblkmem"])
+    keep = {}
+    P = _param_struct(param)
+    B = _blocks_struct(blocks, nbm, keep)
+    f = np.zeros(nbm, np.uint8)
+    f[:len(refine)] = np.asarray(refine, np.uint8)
+    c = None
+    if coarsen is not None:
+        c = np.zeros(nbm, np.uint8)
+        c[:len(coarsen)] = np.asarray(coarsen, np.uint8)
+    cap = int(capacity or (4 * nbm + 16))
+    out = {k: np.zeros(cap, np.int32) for k in BLOCK_KEYS}
+    out["xo"] = np.zeros(cap)
+    out["yo"] = np.zeros(cap)
+    O = BgfBlocksOut()
+    for k in BLOCK_KEYS:
+        setattr(O, k, _iptr(out[k]))
+    O.xo, O.yo = _dptr(out["xo"]), _dptr(out["yo"])
+    remap = np.zeros((cap, 5), np.int32)
+    nb, nm = ctypes.c_int(), ctypes.c_int()
+    _check(lib().bgf_adapt_mesh(ctypes.byref(P), ctypes.byref(B), f.ctypes.data,
+                                None if c is None else c.ctypes.data, int(nbx), int(nby), cap, ctypes.byref(nb),
+                                ctypes.byref(nm), ctypes.byref(O), _iptr(remap)))
+    res = {k: v[:nm.value].copy() for k, v in out.items()}
+    res["active"] = res["active"][:nb.value]
+    return res, nb.value, nm.value, remap[:nm.value].copy()
+
+
 def default_param(**kw):
     """Param defaults of src/Param.h:15-97 for the keys the step reads."""
     p = dict(g=9.81, eps=1e-4, CFL=0.5, theta=1.3, dx=1.0, delta=None, xo=0.0, yo=0.0, lat=0.0, mask=9999.0,
@@ -268,6 +314,7 @@ class Domain:
         self._keep = {}
         B = _blocks_struct(blocks, nbm, self._keep)
         self.active = np.asarray(blocks["active"][:nblk], np.int64)
+        self.blocks = {k: np.array(blocks[k]) for k in BLOCK_KEYS + ("xo", "yo")}
         h = ctypes.c_void_p()
         _check(lib().bgf_create(ctypes.byref(P), ctypes.byref(B), ctypes.byref(h)))
         self.h = h
@@ -331,6 +378,52 @@ class Domain:
         f = np.zeros(self.nblkmem, np.uint8)
         _check(lib().bgf_refine_flags(self.h, field.encode(), int(mode), float(a), float(b), f.ctypes.data))
         return f
+
+    def set_bed_source(self, values, xo, yo, dx):
+        """Bathymetry raster (ny, nx) that remesh samples for new cells; None removes it."""
+        if values is None:
+            _check(lib().bgf_set_bed_source(self.h, 0, 0, 0.0, 0.0, 0.0, None))
+            return
+        a = np.ascontiguousarray(values, np.float32)
+        ny, nx = a.shape
+        _check(lib().bgf_set_bed_source(self.h, nx, ny, float(xo), float(yo), float(dx), a.ctypes.data))
+
+    def remesh(self, blocks, nblk, nblkmem, remap):
+        """Moves the state onto a mesh from adapt_mesh (bgf_remesh)."""
+        p = dict(self.param, nblk=int(nblk), nblkmem=int(nblkmem))
+        P = _param_struct(p)
+        keep = {}
+        B = _blocks_struct(blocks, int(nblkmem), keep)
+        r = np.ascontiguousarray(remap, np.int32).reshape(-1, 5)
+        if r.shape[0] < nblkmem:
+            raise BgfError("remap has %d rows, expected nblkmem = %d" % (r.shape[0], nblkmem))
+        _check(lib().bgf_remesh(self.h, ctypes.byref(P), ctypes.byref(B), _iptr(r)))
+        self.param = p
+        self.nblk, self.nblkmem = int(nblk), int(nblkmem)
+        self.size = self.nblkmem * BS
+        self.active = np.asarray(blocks["active"][:nblk], np.int64)
+        self.blocks = {k: np.array(blocks[k]) for k in BLOCK_KEYS + ("xo", "yo")}
+
+    def adapt(self, criteria, nbx, nby, coarsen=True):
+        """One adaptation of the running domain: the refinement flags are the union of the criteria
+        (each (field, mode, a, b) for refine_flags) on the GPU state, the coarsening flags their
+        complement on the active blocks (as the reference's criteria, src/AdaptCriteria.cu:62-160) when
+        coarsen is set; then adapt_mesh and remesh.  Returns (blocks before, after, refined, coarsened)."""
+        ref = np.zeros(self.nblkmem, np.uint8)
+        for field, mode, a, b in criteria:
+            ref |= self.refine_flags(field, mode, a, b)
+        co = None
+        if coarsen:
+            co = np.zeros(self.nblkmem, np.uint8)
+            co[self.active] = 1 - ref[self.active]
+        blocks, nb, nm, remap = adapt_mesh(self.param, self.blocks, ref, co, nbx, nby)
+        n0 = self.nblk
+        act = remap[blocks["active"][:nb]]
+        nref = int(np.sum(act[:, 0] == 1)) // 4
+        nco = int(np.sum(act[:, 0] == 2))
+        if nref or nco:
+            self.remesh(blocks, nb, nm, remap)
+        return n0, self.nblk, nref, nco
 
     def last_run_steps(self):
         n = ctypes.c_int()

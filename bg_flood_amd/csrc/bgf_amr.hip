@@ -786,4 +786,14 @@ void launch_amr_fill(int precision, const KParams& P, void* z, hipStream_t st) {
     else hipLaunchKernelGGL(k_amr_fill<float>, quad, tpb, 0, st, P, (float*)z);
 }
 
+void launch_amr_halo(int precision, const KParams& P, void* h, void* u, void* v, const void* zb, hipStream_t st) {
+    const dim3 quad((P.nblk + 3) / 4), tpb(256);
+    if (precision == 0)
+        hipLaunchKernelGGL(k_amr_halo<double>, quad, tpb, 0, st, P, (double*)h, (double*)u, (double*)v,
+                           (const double*)zb, (DevLoop*)nullptr, 0);
+    else
+        hipLaunchKernelGGL(k_amr_halo<float>, quad, tpb, 0, st, P, (float*)h, (float*)u, (float*)v,
+                           (const float*)zb, (DevLoop*)nullptr, 0);
+}
+
 }  // namespace bgf

@@ -56,6 +56,10 @@ struct bgf_handle_s {
     // launch schedule of the persistent one-level kernels
     BlockXY xy;
     int tile_x = TILE_X, tile_y = TILE_Y, wg_pred = 0, wg_corr = 0;
+    // bed source raster sampled by bgf_remesh for new cells
+    float* d_bed = nullptr;
+    int bed_nx = 0, bed_ny = 0;
+    double bed_xo = 0, bed_yo = 0, bed_dx = 0;
 };
 
 namespace {
@@ -70,6 +74,7 @@ void free_handle(bgf_handle h) {
     F(h->d_inv);
     F(h->d_ch);
     F(h->d_metric);
+    F(h->d_bed);
     for (int k = 0; k < 17; k++) F(h->d_amr[k]);
     for (int k = 0; k < 4; k++) if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
     if (h->graph) (void)hipGraphExecDestroy(h->graph);
@@ -513,7 +518,7 @@ bgf_status bgf_run(bgf_handle h, bgf_loop* loop, int nsteps, double endtime) {
 }
 
 bgf_status bgf_refine_flags(bgf_handle h, const char* field, int mode, double a, double b, unsigned char* flags) {
-    if (!h || !field || !flags || (mode != 0 && mode != 1)) return fail(BGF_ERR_INVALID_ARG, "bad argument");
+    if (!h || !field || !flags || mode < 0 || mode > 2) return fail(BGF_ERR_INVALID_ARG, "bad argument");
     if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
     const std::string f(field);
     const void *x = nullptr, *y = nullptr;
@@ -536,6 +541,100 @@ bgf_status bgf_refine_flags(bgf_handle h, const char* field, int mode, double a,
     (void)hipFree(d);
     if (e != hipSuccess) return fail(BGF_ERR_HIP, std::string("refine flags: ") + hipGetErrorString(e));
     return BGF_OK;
+}
+
+bgf_status bgf_set_bed_source(bgf_handle h, int nx, int ny, double xo, double yo, double dx, const float* values) {
+    if (!h) return fail(BGF_ERR_INVALID_ARG, "null handle");
+    if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
+    if (h->d_bed) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        (void)hipFree(h->d_bed);
+        h->d_bed = nullptr;
+    }
+    if (!values) return BGF_OK;
+    if (nx < 2 || ny < 2 || !(dx > 0)) return fail(BGF_ERR_INVALID_ARG, "bed source needs nx, ny >= 2 and dx > 0");
+    const size_t bytes = sizeof(float) * (size_t)nx * ny;
+    HIPCHK(hipMalloc((void**)&h->d_bed, bytes));
+    HIPCHK(hipMemcpy(h->d_bed, values, bytes, hipMemcpyHostToDevice));
+    h->bed_nx = nx;
+    h->bed_ny = ny;
+    h->bed_xo = xo;
+    h->bed_yo = yo;
+    h->bed_dx = dx;
+    return BGF_OK;
+}
+
+bgf_status bgf_remesh(bgf_handle h, const bgf_param* p, const bgf_blocks* b, const int* remap) {
+    if (!h || !p || !b || !remap) return fail(BGF_ERR_INVALID_ARG, "null argument");
+    if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
+    if (!h->state_set) return fail(BGF_ERR_STATE, "state not set");
+    if (!h->amr)
+        return fail(BGF_ERR_UNSUPPORTED, "bgf_remesh needs a multi-level handle (minlevel < maxlevel at bgf_create)");
+    bgf_param q = h->p;
+    q.nblk = p->nblk;
+    q.nblkmem = p->nblkmem;
+    const int nbo = h->nblkmem, nbm = q.nblkmem;
+    if (q.nblk <= 0 || nbm < q.nblk) return fail(BGF_ERR_INVALID_ARG, "bad sizes");
+    std::vector<char> was(nbo, 0);
+    for (int ib : h->active) was[ib] = 1;
+    auto old_ok = [&](int i) { return i >= 0 && i < nbo && was[i]; };
+    for (int k = 0; k < q.nblk; k++) {
+        const int ib = b->active[k];
+        if (ib < 0 || ib >= nbm) return fail(BGF_ERR_INVALID_ARG, "active block index out of range");
+        const int* r = remap + 5 * ib;
+        bool ok = false;
+        if (r[0] == 0) ok = r[1] == ib && old_ok(ib);
+        else if (r[0] == 1) ok = old_ok(r[1]) && r[2] >= 0 && r[2] < 4;
+        else if (r[0] == 2) ok = old_ok(r[1]) && old_ok(r[2]) && old_ok(r[3]) && old_ok(r[4]);
+        if (!ok) return fail(BGF_ERR_INVALID_ARG, "remap entry of block " + std::to_string(ib) + " is not valid");
+    }
+    bgf_handle nh = nullptr;
+    bgf_status s = bgf_create(&q, b, &nh);
+    if (s != BGF_OK) return s;
+    int* d_remap = nullptr;
+    double *d_xo = nullptr, *d_yo = nullptr;
+    auto cleanup = [&](bgf_status e) {
+        (void)hipFree(d_remap);
+        (void)hipFree(d_xo);
+        (void)hipFree(d_yo);
+        if (nh) bgf_destroy(nh);
+        return e;
+    };
+    bool ok = hipMalloc((void**)&d_remap, sizeof(int) * 5 * (size_t)nbm) == hipSuccess &&
+              hipMalloc((void**)&d_xo, sizeof(double) * nbm) == hipSuccess &&
+              hipMalloc((void**)&d_yo, sizeof(double) * nbm) == hipSuccess;
+    ok = ok && hipMemcpy(d_remap, remap, sizeof(int) * 5 * (size_t)nbm, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(d_xo, b->xo, sizeof(double) * nbm, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(d_yo, b->yo, sizeof(double) * nbm, hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) return cleanup(fail(BGF_ERR_HIP, "remesh: upload failed"));
+    const int prec = h->prec == 0 ? 0 : 1;
+    // the old state's halo, as the step fills it, then the new cells
+    bgf::launch_amr_halo(prec, h->kp, h->d_h[0], h->d_u[0], h->d_v[0], h->d_zb, h->stream);
+    bgf::RemeshLaunch R;
+    R.nblk = q.nblk;
+    R.h = h->d_h[0]; R.u = h->d_u[0]; R.v = h->d_v[0]; R.zb = h->d_zb; R.cf = h->d_cf;
+    R.active = nh->d_active; R.level = nh->d_level; R.xo = d_xo; R.yo = d_yo; R.remap = d_remap;
+    R.nh = nh->d_h[0]; R.nu = nh->d_u[0]; R.nv = nh->d_v[0]; R.nzb = nh->d_zb; R.ncf = nh->d_cf;
+    R.bed = h->d_bed; R.bnx = h->bed_nx; R.bny = h->bed_ny; R.bxo = h->bed_xo; R.byo = h->bed_yo; R.bdx = h->bed_dx;
+    R.pxo = q.xo; R.pyo = q.yo; R.dx = q.dx; R.eps = q.eps;
+    bgf::launch_remesh(prec, h->kp, R, h->stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return cleanup(fail(BGF_ERR_HIP, std::string("remesh: ") + hipGetErrorString(e)));
+    // the bed's halo on the new mesh (src/InitialConditions.cu:135-147)
+    bgf::launch_amr_fill(prec, nh->kp, nh->d_zb, nh->stream);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(nh->stream);
+    if (e != hipSuccess) return cleanup(fail(BGF_ERR_HIP, std::string("remesh: ") + hipGetErrorString(e)));
+    nh->state_set = true;
+    nh->timing = h->timing;
+    nh->use_graph = h->use_graph;
+    std::swap(nh->d_bed, h->d_bed);
+    nh->bed_nx = h->bed_nx; nh->bed_ny = h->bed_ny;
+    nh->bed_xo = h->bed_xo; nh->bed_yo = h->bed_yo; nh->bed_dx = h->bed_dx;
+    std::swap(*h, *nh);  // h takes the new mesh; nh (the old one) is freed
+    g_err.clear();
+    return cleanup(BGF_OK);
 }
 
 bgf_status bgf_last_run_steps(bgf_handle h, int* nsteps) {

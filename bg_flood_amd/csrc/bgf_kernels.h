@@ -125,5 +125,22 @@ void launch_amr_step(int precision, const KParams& P, const void* buffers, DevLo
                      hipStream_t st, hipEvent_t* ev);
 // Halo of a static field (the bed) on a multi-level mesh, src/InitialConditions.cu:135-147.
 void launch_amr_fill(int precision, const KParams& P, void* z, hipStream_t st);
+// The step's halo fill of h, u, v on a multi-level mesh (src/Halo.cu:586-616) by itself.
+void launch_amr_halo(int precision, const KParams& P, void* h, void* u, void* v, const void* zb, hipStream_t st);
+
+// State of a multi-level mesh moved onto an adapted mesh (bgf_remesh.hip).
+struct RemeshLaunch {
+    int nblk;                             // active blocks of the new mesh
+    const void *h, *u, *v, *zb, *cf;      // old state
+    const int* active;                    // new mesh
+    const int* level;
+    const double *xo, *yo;
+    const int* remap;                     // kind, src0..src3 per new slot
+    void *nh, *nu, *nv, *nzb, *ncf;       // new state
+    const float* bed;                     // bed source raster (may be null)
+    int bnx, bny;
+    double bxo, byo, bdx, pxo, pyo, dx, eps;
+};
+void launch_remesh(int precision, const KParams& Po, const RemeshLaunch& R, hipStream_t st);
 
 }  // namespace bgf

@@ -785,9 +785,20 @@ __global__ __launch_bounds__(256) void k_refine_flags(KParams P, const T* __rest
                                                       int mode, T lo, T hi, unsigned char* flags) {
     const int t = threadIdx.x;
     const int ib = P.active[blockIdx.x];
-    const int i = memloc(t & 15, t >> 4, ib);
-    const T z = b ? b[i] + a[i] : a[i];
-    const int hit = mode == 0 ? z > lo : (z >= lo && z <= hi);
+    const int x = t & 15, y = t >> 4;
+    const int i = memloc(x, y, ib);
+    auto val = [&](int c) { return b ? b[c] + a[c] : a[c]; };
+    const T z = val(i);
+    int hit;
+    if (mode == 0) hit = z > lo;
+    else if (mode == 1) hit = z >= lo && z <= hi;
+    else {
+        // steepness: a difference to the next interior cell over the cell size
+        const T d = T(calcres(P.delta0, P.level[ib]));
+        const T gx = x < BW - 1 ? fabs(val(memloc(x + 1, y, ib)) - z) / d : T(0);
+        const T gy = y < BW - 1 ? fabs(val(memloc(x, y + 1, ib)) - z) / d : T(0);
+        hit = gx > lo || gy > lo;
+    }
     const int any = __syncthreads_or(hit);
     if (t == 0) flags[ib] = any ? 1 : 0;
 }

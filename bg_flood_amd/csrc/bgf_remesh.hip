@@ -1,0 +1,219 @@
+// bgf_remesh.hip -- moves the state of a multi-level mesh onto the mesh of an
+// adaptation pass, on the GPU.
+//
+// One workgroup per active block of the new mesh, one thread per cell.  The
+// remap table (5 ints per block slot, from bgf_adapt_mesh) says where the
+// block's cells come from:
+//   kept      the same cells of the same block index;
+//   child     refine (src/Adaptation.cu:1032-1102): the bilinear interpolation
+//             of the parent's h, zs, u, v at the child cell centre, on the
+//             parent's 18x18 tile; the tile's halo is the step's halo fill
+//             (run on the old mesh just before), its corners follow
+//             fillCorners (src/Halo.cu:5277-5409);
+//   coarsened coarsen (src/Adaptation.cu:700-760): the 2x2 means of h, zs, u,
+//             v over the four old children.
+// The bed of a new cell is sampled from the bed source raster when there is
+// one (interp2BUQ, src/GridManip.cu:333-560: bilinear where the cell is no
+// coarser than the raster, else the mean of the raster cells it covers),
+// otherwise interpolated like the state; then h = max(eps, zs - zb) where
+// h > eps (src/Adaptation.cu:493-512; elsewhere the reference sets zs = zb,
+// which holds here since zs is always zb + h).  Roughness: a child cell keeps
+// its parent cell's value, a coarsened cell takes the mean.
+//
+// fillCorners as restated here reads, for the top-right and bottom-right
+// corners beside a coarser diagonal block, that diagonal block; the reference
+// reads the top-left diagonal there (src/Halo.cu:5368-5374, 5399-5405).  The
+// reference fills corners only during its set-up adaptation, whose state it
+// then overwrites with the initial conditions.
+#include <hip/hip_runtime.h>
+
+#include "bgf_common.h"
+
+namespace bgf {
+namespace {
+
+template <class T> __device__ __forceinline__ T smax(T a, T b) { return (a < b) ? b : a; }
+
+template <class T> struct RemeshArgs {
+    // old mesh state (the halo of h, u, v filled, the bed's halo filled)
+    const T *h, *u, *v, *zb, *cf;
+    // new mesh
+    const int* active;
+    const int* level;
+    const double *xo, *yo;
+    const int* remap;  // kind, src0..src3 per slot
+    T *nh, *nu, *nv, *nzb, *ncf;
+    // bed source raster (may be null)
+    const float* bed;
+    int bnx, bny;
+    double bxo, byo, bdx, pxo, pyo;
+    double dx;  // Param.dx
+    T eps;
+};
+
+// A field on a tile: the array, or zs = zb + h.
+template <class T> struct Field {
+    const T* a;
+    const T* zb;
+    __device__ __forceinline__ T operator()(int c) const { return zb ? zb[c] + a[c] : a[c]; }
+};
+
+// Corner value of block ib (cx, cy in {-1, 16}) following fillCorners
+// (src/Halo.cu:5277-5409): no diagonal block -> mean of the two adjacent halo
+// cells; same level -> its nearest cell; finer -> mean of its 2x2 corner
+// cells; coarser -> half its corner cell plus a quarter of each adjacent halo
+// cell.
+template <class T, class F> __device__ T corner_value(const KParams& P, int ib, int cx, int cy, const F& z) {
+    const bool right = cx >= 0, top = cy >= 0;
+    // the diagonal block, through the bottom or top neighbour (src/Halo.cu:5290, 5317, 5349, 5380)
+    int vert, n;
+    if (!right && !top) { vert = P.BotLeft[ib]; n = P.LeftTop[vert]; }
+    else if (!right && top) { vert = P.TopLeft[ib]; n = P.LeftBot[vert]; }
+    else if (right && top) { vert = P.TopRight[ib]; n = P.RightBot[vert]; }
+    else { vert = P.BotRight[ib]; n = P.RightBot[vert]; }
+    // the diagonal block's cell nearest the corner, and its inner neighbours
+    const int ex = right ? 0 : BW - 1, ey = top ? 0 : BW - 1;
+    const int sx = right ? 1 : -1, sy = top ? 1 : -1;
+    // the block's own halo cells beside the corner
+    const int hx = memloc(cx, cy - sy, ib), hy = memloc(cx - sx, cy, ib);
+    if (n == vert) return T(0.5) * (z(hx) + z(hy));
+    const int ln = P.level[n], lb = P.level[ib];
+    if (ln == lb) return z(memloc(ex, ey, n));
+    if (ln > lb) {
+        const int ii = memloc(ex, ey, n), ir = memloc(ex, ey + sy, n), it = memloc(ex + sx, ey, n),
+                  itr = memloc(ex + sx, ey + sy, n);
+        return T(0.25) * (z(ii) + z(ir) + z(it) + z(itr));
+    }
+    return T(0.5) * z(memloc(ex, ey, n)) + T(0.25) * (z(hx) + z(hy));
+}
+
+// Value of a field at tile cell (x, y) of block ib, x, y in [-1, 16].
+template <class T, class F> __device__ __forceinline__ T tile(const KParams& P, int ib, int x, int y, const F& z) {
+    const bool cx = x < 0 || x >= BW, cy = y < 0 || y >= BW;
+    if (cx && cy) return corner_value<T>(P, ib, x, y, z);
+    return z(memloc(x, y, ib));
+}
+
+// BilinearInterpolation (src/Util_CPU.cu:129-144).
+template <class T> __device__ __forceinline__ T bilinear(T q11, T q12, T q21, T q22, T x1, T x2, T y1, T y2, T x, T y) {
+    const T x2x1 = x2 - x1, y2y1 = y2 - y1, x2x = x2 - x, y2y = y2 - y, yy1 = y - y1, xx1 = x - x1;
+    return T(1.0) / (x2x1 * y2y1) * (q11 * x2x * y2y + q21 * xx1 * y2y + q12 * x2x * yy1 + q22 * xx1 * yy1);
+}
+
+// src/Util_CPU.cu:308-311
+__device__ __forceinline__ int ftoi(double v) { return v >= 0.0 ? (int)(v + 0.5) : (int)(v - 0.5); }
+
+// interp2BUQ(x, y, dx, forcing) (src/GridManip.cu:456-560) on a float raster.
+template <class T> __device__ T bed_sample(const RemeshArgs<T>& A, T x, T y, T dx) {
+    const double fdx = A.bdx;
+    if (dx <= T(fdx)) {
+        const double xmax = A.bxo + (A.bnx - 1) * fdx, ymax = A.byo + (A.bny - 1) * fdx;
+        const double xi = fmax(fmin((double)x, xmax), A.bxo);
+        const double yi = fmax(fmin((double)y, ymax), A.byo);
+        int cfi = (int)floor((xi - A.bxo) / fdx);
+        cfi = min(max(cfi, 0), A.bnx - 2);
+        int cfj = (int)floor((yi - A.byo) / fdx);
+        cfj = min(max(cfj, 0), A.bny - 2);
+        const int cfip = cfi + 1, cfjp = cfj + 1;
+        const double x1 = A.bxo + fdx * cfi, x2 = A.bxo + fdx * cfip;
+        const double y1 = A.byo + fdx * cfj, y2 = A.byo + fdx * cfjp;
+        const double q11 = A.bed[cfi + cfj * A.bnx], q12 = A.bed[cfi + cfjp * A.bnx];
+        const double q21 = A.bed[cfip + cfj * A.bnx], q22 = A.bed[cfip + cfjp * A.bnx];
+        return T(bilinear<double>(q11, q12, q21, q22, x1, x2, y1, y2, xi, yi));
+    }
+    const double xmin = x - dx * 0.5, xmx = x + dx * 0.5, ymin = y - dx * 0.5, ymx = y + dx * 0.5;
+    const int imin = max(ftoi(floor((xmin - A.bxo) / fdx)), 0);
+    const int imax = min(ftoi(floor((xmx - A.bxo) / fdx)), A.bnx - 1);
+    const int jmin = max(ftoi(floor((ymin - A.byo) / fdx)), 0);
+    const int jmax = min(ftoi(floor((ymx - A.byo) / fdx)), A.bny - 1);
+    const int ni = max(imax - imin + 1, 1), nj = max(jmax - jmin + 1, 1);
+    T z = T(0.0);
+    for (int i = 0; i < ni; i++)
+        for (int j = 0; j < nj; j++) {
+            const int cfi = min(imin + i, A.bnx - 1), cfj = min(jmin + j, A.bny - 1);
+            z = z + A.bed[cfi + cfj * A.bnx];
+        }
+    return z / T(ni * nj);
+}
+
+template <class T> __global__ __launch_bounds__(256) void k_remesh(KParams Po, RemeshArgs<T> A) {
+    const int t = threadIdx.x;
+    const int ib = A.active[blockIdx.x];
+    const int x = t & 15, y = t >> 4;
+    const int o = memloc(x, y, ib);
+    const int kind = A.remap[5 * ib];
+    const int* s = A.remap + 5 * ib + 1;
+    if (kind == 0) {
+        A.nh[o] = A.h[o];
+        A.nu[o] = A.u[o];
+        A.nv[o] = A.v[o];
+        A.nzb[o] = A.zb[o];
+        A.ncf[o] = A.cf[o];
+        return;
+    }
+    T h, zs, u, v, zb, cf;
+    const Field<T> fh{A.h, nullptr}, fzs{A.h, A.zb}, fu{A.u, nullptr}, fv{A.v, nullptr}, fzb{A.zb, nullptr};
+    if (kind == 1) {
+        const int p = s[0], q = s[1];
+        const int kx = (q & 1) * (BW / 2), ky = (q >> 1) * (BW / 2);
+        const T lx = T(x) * T(0.5) - T(0.25), ly = T(y) * T(0.5) - T(0.25);
+        const int fx = (int)floor(lx) + kx, cx = (int)ceil(lx) + kx;
+        const int fy = (int)floor(ly) + ky, cy = (int)ceil(ly) + ky;
+        const T rx = lx + T(kx), ry = ly + T(ky);
+        auto interp = [&](const Field<T>& f) {
+            return bilinear<T>(tile<T>(Po, p, fx, fy, f), tile<T>(Po, p, fx, cy, f), tile<T>(Po, p, cx, fy, f),
+                               tile<T>(Po, p, cx, cy, f), T(fx), T(cx), T(fy), T(cy), rx, ry);
+        };
+        h = interp(fh);
+        zs = interp(fzs);
+        u = interp(fu);
+        v = interp(fv);
+        zb = interp(fzb);
+        cf = A.cf[memloc(kx + x / 2, ky + y / 2, p)];
+    } else {
+        const int qx = x >= BW / 2, qy = y >= BW / 2;
+        const int c = s[qx + 2 * qy];
+        const int i0 = (x - qx * (BW / 2)) * 2, j0 = (y - qy * (BW / 2)) * 2;
+        const int ii = memloc(i0, j0, c), ir = memloc(i0 + 1, j0, c), it = memloc(i0, j0 + 1, c),
+                  itr = memloc(i0 + 1, j0 + 1, c);
+        auto mean = [&](const Field<T>& f) { return T(0.25) * (f(ii) + f(ir) + f(it) + f(itr)); };
+        h = mean(fh);
+        zs = mean(fzs);
+        u = mean(fu);
+        v = mean(fv);
+        zb = mean(fzb);
+        cf = T(0.25) * (A.cf[ii] + A.cf[ir] + A.cf[it] + A.cf[itr]);
+    }
+    if (A.bed) {
+        const double blkdx = calcres(A.dx, A.level[ib]);
+        const T px = T(A.pxo + A.xo[ib] + x * blkdx), py = T(A.pyo + A.yo[ib] + y * blkdx);
+        zb = bed_sample(A, px, py, T(blkdx));
+    }
+    if (h > A.eps) h = smax(A.eps, zs - zb);
+    A.nh[o] = h;
+    A.nu[o] = u;
+    A.nv[o] = v;
+    A.nzb[o] = zb;
+    A.ncf[o] = cf;
+}
+
+}  // namespace
+
+void launch_remesh(int precision, const KParams& Po, const RemeshLaunch& R, hipStream_t st) {
+    if (R.nblk <= 0) return;
+    if (precision == 0) {
+        RemeshArgs<double> A{(const double*)R.h, (const double*)R.u, (const double*)R.v, (const double*)R.zb,
+                             (const double*)R.cf, R.active, R.level, R.xo, R.yo, R.remap,
+                             (double*)R.nh, (double*)R.nu, (double*)R.nv, (double*)R.nzb, (double*)R.ncf,
+                             R.bed, R.bnx, R.bny, R.bxo, R.byo, R.bdx, R.pxo, R.pyo, R.dx, R.eps};
+        hipLaunchKernelGGL(k_remesh<double>, dim3(R.nblk), dim3(256), 0, st, Po, A);
+    } else {
+        RemeshArgs<float> A{(const float*)R.h, (const float*)R.u, (const float*)R.v, (const float*)R.zb,
+                            (const float*)R.cf, R.active, R.level, R.xo, R.yo, R.remap,
+                            (float*)R.nh, (float*)R.nu, (float*)R.nv, (float*)R.nzb, (float*)R.ncf,
+                            R.bed, R.bnx, R.bny, R.bxo, R.byo, R.bdx, R.pxo, R.pyo, R.dx, (float)R.eps};
+        hipLaunchKernelGGL(k_remesh<float>, dim3(R.nblk), dim3(256), 0, st, Po, A);
+    }
+}
+
+}  // namespace bgf

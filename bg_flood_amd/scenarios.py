@@ -51,7 +51,7 @@ def cell_coords(blocks, nblkmem, xo, yo, delta, dtype):
 def cell_coords_levels(blocks, nblkmem, xo, yo, dx, dtype):
     """Cell centres of a multi-level block list: x = xo + blk.xo + ix*calcres(dx, level), in T."""
     T = dtype
-    nb = len(blocks["active"])
+    nb = min(len(blocks["level"]), nblkmem)  # every slot (active or not) has a level and an origin
     lev = np.asarray(blocks["level"][:nb])
     d = np.where(lev < 0, dx * np.exp2(-lev.astype(np.float64)), dx / np.exp2(lev.astype(np.float64))).astype(T)
     ix = np.arange(BW, dtype=T)

@@ -205,11 +205,64 @@ bgf_status bgf_adapt_refine(const bgf_param* p, const bgf_blocks* b, const unsig
                             double* xo, double* yo, int* LeftBot, int* LeftTop, int* RightBot, int* RightTop,
                             int* TopLeft, int* TopRight, int* BotLeft, int* BotRight);
 
+/* Block description written by bgf_adapt_mesh (arrays of `capacity` entries). */
+typedef struct {
+    int* active;
+    int* level;
+    int *LeftBot, *LeftTop, *RightBot, *RightTop;
+    int *TopLeft, *TopRight, *BotLeft, *BotRight;
+    double *xo, *yo;
+} bgf_blocks_out;
+
+/* One adaptation pass with refinement and coarsening: Adapt's
+ * refinesanitycheck, CalcAvailblk, AddBlocks, coarsen, refine and
+ * Adaptationcleanup (src/Adaptation.cu:157-1876), with the reference's block
+ * numbering.  coarsen_flags may be NULL: no coarsening, as the reference
+ * itself (it cancels every coarsening request, :170-175).  With coarsen flags
+ * the reference's coarsening rules apply: a block coarsens only as the
+ * bottom-left of four same-level siblings that all request it, at a level
+ * above minlevel, beside no finer block and no refining block.  remap (5 ints
+ * per output slot, may be NULL) says where each slot's state comes from:
+ * kind (-1 inactive, 0 kept, 1 child of a refined block, 2 coarsened parent)
+ * then, for a child, its old parent and quadrant (0 BL, 1 BR, 2 TL, 3 TR),
+ * for a coarsened parent its old BL, BR, TL, TR blocks, for a kept block its
+ * own index.  Inputs and outputs as bgf_adapt_refine. */
+bgf_status bgf_adapt_mesh(const bgf_param* p, const bgf_blocks* b, const unsigned char* refine_flags,
+                          const unsigned char* coarsen_flags, int nbx, int nby, int capacity, int* nblk_out,
+                          int* nblkmem_out, const bgf_blocks_out* out, int* remap);
+
+/* Moves a multi-level handle's state onto the mesh of bgf_adapt_mesh, on the
+ * GPU (the reference adapts on the CPU at set-up only, src/Adaptation.cu:17-
+ * 155, and re-runs its initial conditions afterwards; this is the in-run
+ * form).  p gives the new nblk and nblkmem (every other field as at
+ * bgf_create).  Kept blocks keep their cells.  A child takes the bilinear
+ * interpolation of its parent's h, zs, u, v (refine, :1032-1102) on the
+ * parent's tile with its halo filled by the step's halo fill and corners by
+ * fillCorners (src/Halo.cu:5277-5409); a coarsened parent the 2x2 means of
+ * its children (coarsen, :700-760).  The bed of new cells comes from the bed
+ * source of bgf_set_bed_source (interp2BUQ, src/GridManip.cu:333-560) when
+ * one is set, else from the same interpolation of the old bed; then, on new
+ * cells, h = max(eps, zs - zb) where h > eps and zs = zb elsewhere
+ * (src/Adaptation.cu:493-512).  Roughness: a child cell takes its parent
+ * cell's, a coarsened cell the mean.  The loop state (time, dt) is kept; the
+ * next bgf_step or bgf_run continues on the new mesh. */
+bgf_status bgf_remesh(bgf_handle h, const bgf_param* p, const bgf_blocks* b, const int* remap);
+
+/* Bathymetry raster that bgf_remesh samples for new cells, as the reference
+ * re-interpolates its bathymetry forcing after adaptation
+ * (StaticForcingP<float>, src/Forcing.h): nx * ny values, row-major from
+ * (xo, yo), spacing dx (and dy = dx), in the coordinates of Param.xo/yo.
+ * NULL values removes it. */
+bgf_status bgf_set_bed_source(bgf_handle h, int nx, int ny, double xo, double yo, double dx, const float* values);
+
 /* Refinement flags of the reference's criteria on the GPU state
  * (src/AdaptCriteria.cu:62-160): per active block, mode 0 (Threshold) sets
  * flags[ib] if any interior cell of the field exceeds a; mode 1 (Inrange) if
- * any lies in [a, b].  field: "h", "zs", "zb", "u" or "v".  flags has
- * nblkmem entries (0 for inactive blocks). */
+ * any lies in [a, b]; mode 2 (steepness, no counterpart in the reference) if
+ * the difference between two adjacent interior cells of the field, over the
+ * block's cell size, exceeds a.  field: "h", "zs", "zb", "u" or "v".  flags
+ * has nblkmem entries (0 for inactive blocks).  The reference's coarsening
+ * flags are the complement of its refinement flags on the active blocks. */
 bgf_status bgf_refine_flags(bgf_handle h, const char* field, int mode, double a, double b, unsigned char* flags);
 
 /* ------------------------------------------------------------------ batches

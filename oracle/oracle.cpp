@@ -1594,30 +1594,88 @@ float orc_timestep_f(float dtmin_cells, const orc_loop* loop) { return timestep_
 // each end of each side, as InitBlockneighbours does by coordinates,
 // src/Mesh.cu:351-433).
 namespace {
+// One adaptation pass, Adapt's sequence (src/Adaptation.cu:157-1876): the
+// sanity rules on the refine/coarsen flags, CalcAvailblk, AddBlocks, coarsen,
+// refine, Adaptationcleanup.  coarsen == nullptr reproduces the reference's
+// cancellation of every coarsening (:170-175).  remap (5 per slot) records
+// where each slot's state comes from (kind, sources).
 template <class T>
-int adapt_refine_t(const orc_param* p, const orc_blocks* b, const unsigned char* flags, int capacity, int* nblk_out,
-                   int* nblkmem_out, int* active, int* level, double* xo, double* yo, int* nb[8]) {
+int adapt_mesh_t(const orc_param* p, const orc_blocks* b, const unsigned char* flags, const unsigned char* cflags,
+                 int capacity, int* nblk_out, int* nblkmem_out, int* active, int* level, double* xo, double* yo,
+                 int* nb[8], int* remap) {
     const int nblk = p->nblk, nbm = p->nblkmem;
     std::vector<int> lev(b->level, b->level + nbm), isact(nbm, 0);
-    std::vector<bool> refine(nbm, false);
-    for (int ibl = 0; ibl < nblk; ibl++) { int ib = b->active[ibl]; isact[ib] = 1; refine[ib] = flags[ib] != 0; }
+    std::vector<bool> refine(nbm, false), coarsen(nbm, false);
+    for (int ibl = 0; ibl < nblk; ibl++) {
+        int ib = b->active[ibl];
+        isact[ib] = 1;
+        refine[ib] = flags[ib] != 0;
+        coarsen[ib] = cflags ? cflags[ib] != 0 : false;
+    }
+    // refinesanitycheck (src/Adaptation.cu:157-296)
     for (int ibl = 0; ibl < nblk; ibl++) {
         int ib = b->active[ibl];
         if (refine[ib] && lev[ib] == p->maxlevel) refine[ib] = false;
+        if (coarsen[ib] && lev[ib] == p->minlevel) coarsen[ib] = false;
     }
+    for (int ibl = 0; ibl < nblk; ibl++) {
+        int ib = b->active[ibl];
+        if (refine[ib]) {
+            coarsen[b->RightBot[ib]] = false; coarsen[b->RightTop[ib]] = false;
+            coarsen[b->LeftBot[ib]] = false; coarsen[b->LeftTop[ib]] = false;
+            coarsen[b->TopLeft[ib]] = false; coarsen[b->TopRight[ib]] = false;
+            coarsen[b->BotLeft[ib]] = false; coarsen[b->BotRight[ib]] = false;
+        }
+    }
+    for (int ibl = 0; ibl < nblk; ibl++) {
+        int ib = b->active[ibl];
+        if (coarsen[ib]) {
+            int levi = lev[ib];
+            if (levi < lev[b->LeftBot[ib]] || levi < lev[b->RightBot[ib]] || levi < lev[b->TopLeft[ib]] ||
+                levi < lev[b->BotLeft[ib]])
+                coarsen[ib] = false;
+        }
+    }
+    // checkneighbourrefine (:298-325)
+    auto check = [&](int n, int levelib, int leveln) {
+        int it = 0;
+        if (refine[n] == false && leveln < levelib) { refine[n] = true; coarsen[n] = false; it++; }
+        if (leveln == levelib) coarsen[n] = false;
+        return it;
+    };
     int iter = 1;
     while (iter > 0) {
         iter = 0;
         for (int ibl = 0; ibl < nblk; ibl++) {
             int ib = b->active[ibl];
             if (refine[ib]) {
-                const int nbs[4] = {b->TopLeft[ib], b->BotLeft[ib], b->LeftBot[ib], b->RightBot[ib]};
-                for (int n : nbs)
-                    if (refine[n] == false && lev[n] < lev[ib]) { refine[n] = true; iter++; }
+                iter += check(b->TopLeft[ib], lev[ib], lev[b->TopLeft[ib]]);
+                iter += check(b->BotLeft[ib], lev[ib], lev[b->BotLeft[ib]]);
+                iter += check(b->LeftBot[ib], lev[ib], lev[b->LeftBot[ib]]);
+                iter += check(b->RightBot[ib], lev[ib], lev[b->RightBot[ib]]);
             }
         }
     }
-    // CalcAvailblk
+    for (int ibl = 0; ibl < nblk; ibl++) {
+        int ib = b->active[ibl];
+        double dxfac = calcres(p->dx, lev[ib]);
+        if (coarsen[ib]) {
+            T bx = T(b->xo[ib]), by = T(b->yo[ib]);
+            if ((int(bx / dxfac / BW) % 2) == 0 && (int(by / dxfac / BW) % 2) == 0 && b->RightBot[ib] != ib &&
+                b->TopLeft[ib] != ib && b->RightBot[b->TopRight[ib]] != b->TopRight[ib]) {
+                if (lev[ib] == lev[b->RightBot[ib]] && lev[ib] == lev[b->TopLeft[ib]] &&
+                    lev[ib] == lev[b->RightBot[b->TopRight[ib]]]) {
+                    if (!(coarsen[b->RightBot[ib]] && coarsen[b->TopLeft[ib]] && coarsen[b->RightBot[b->TopRight[ib]]]))
+                        coarsen[ib] = false;
+                } else {
+                    coarsen[ib] = false;
+                }
+            } else {
+                coarsen[ib] = false;
+            }
+        }
+    }
+    // CalcAvailblk (:535-598)
     std::vector<int> invactive(nbm, -1), csumblk(nbm, 0);
     int csum = -3, nrefineblk = 0;
     for (int ibl = 0; ibl < nblk; ibl++) {
@@ -1630,7 +1688,7 @@ int adapt_refine_t(const orc_param* p, const orc_blocks* b, const unsigned char*
     for (int ibl = 0; ibl < nbm; ibl++) if (invactive[ibl] == -1) availblk.push_back(ibl);
     int nnewblk = 3 * nrefineblk;
     int nblkmem = nbm;
-    if (nnewblk > (int)availblk.size()) {  // AddBlocks
+    if (nnewblk > (int)availblk.size()) {  // AddBlocks (:608-662)
         nblkmem = (int)std::ceil((nblk + nnewblk) * 1.05);
         invactive.resize(nblkmem, -1);
         availblk.clear();
@@ -1638,26 +1696,44 @@ int adapt_refine_t(const orc_param* p, const orc_blocks* b, const unsigned char*
     }
     if (nblkmem > capacity) return -1;
     std::vector<T> bxo(nblkmem, T(0)), byo(nblkmem, T(0));
-    std::vector<int> newlevel(nblkmem, 0);
+    std::vector<int> newlevel(nblkmem, 0), kind(nblkmem, -1), src(4 * (size_t)nblkmem, -1);
     for (int i = 0; i < nbm; i++) { bxo[i] = T(b->xo[i]); byo[i] = T(b->yo[i]); newlevel[i] = lev[i]; }
-    std::vector<int> act(b->active, b->active + nblk);
+    std::vector<int> act(b->active, b->active + nblk);  // the active array, -1 where a block leaves
+    // coarsen (:677-1004): BL becomes the parent, the three others leave
+    for (int ibl = 0; ibl < nblk; ibl++) {
+        int ib = b->active[ibl];
+        if (!coarsen[ib]) continue;
+        int ibr = b->RightBot[ib], ibtl = b->TopLeft[ib], ibtr = b->TopLeft[b->RightBot[ib]];
+        newlevel[ib] = lev[ib] - 1;
+        act[invactive[ibr]] = -1; act[invactive[ibtl]] = -1; act[invactive[ibtr]] = -1;
+        bxo[ib] = bxo[ib] + T(calcres(p->dx, lev[ib] + 1));
+        byo[ib] = byo[ib] + T(calcres(p->dx, lev[ib] + 1));
+        kind[ib] = 2;
+        src[4 * ib] = ib; src[4 * ib + 1] = ibr; src[4 * ib + 2] = ibtl; src[4 * ib + 3] = ibtr;
+    }
+    // refine (:1017-1795)
     for (int ibl = 0; ibl < nblk; ibl++) {
         int ib = b->active[ibl];
         if (!refine[ib]) continue;
         double delx = calcres(p->dx, lev[ib] + 1);
         double xoblk = bxo[ib] - 0.5 * delx;
         double yoblk = byo[ib] - 0.5 * delx;
-        int ibr = availblk[csumblk[ib]], ibtl = availblk[csumblk[ib] + 1], ibtr = availblk[csumblk[ib] + 2];
-        newlevel[ib] = newlevel[ibr] = newlevel[ibtl] = newlevel[ibtr] = lev[ib] + 1;
-        bxo[ib] = T(xoblk); byo[ib] = T(yoblk);
-        bxo[ibr] = T(xoblk + BW * delx); byo[ibr] = T(yoblk);
-        bxo[ibtl] = T(xoblk); byo[ibtl] = T(yoblk + BW * delx);
-        bxo[ibtr] = T(xoblk + BW * delx); byo[ibtr] = T(yoblk + BW * delx);
+        int kb[4] = {ib, availblk[csumblk[ib]], availblk[csumblk[ib] + 1], availblk[csumblk[ib] + 2]};
+        for (int q = 0; q < 4; q++) { newlevel[kb[q]] = lev[ib] + 1; kind[kb[q]] = 1; src[4 * kb[q]] = ib; src[4 * kb[q] + 1] = q; }
+        bxo[kb[0]] = T(xoblk); byo[kb[0]] = T(yoblk);
+        bxo[kb[1]] = T(xoblk + BW * delx); byo[kb[1]] = T(yoblk);
+        bxo[kb[2]] = T(xoblk); byo[kb[2]] = T(yoblk + BW * delx);
+        bxo[kb[3]] = T(xoblk + BW * delx); byo[kb[3]] = T(yoblk + BW * delx);
     }
     for (int ibl = 0; ibl < nblk; ibl++) {
         int ib = b->active[ibl];
         if (refine[ib]) for (int j = 0; j < 3; j++) act.push_back(availblk[csumblk[ib] + j]);
     }
+    // Adaptationcleanup (:1807-1876): compact the active list in order
+    std::vector<int> cact;
+    for (int ib : act) if (ib != -1) cact.push_back(ib);
+    act.swap(cact);
+    for (int ib : act) if (kind[ib] < 0) { kind[ib] = 0; src[4 * ib] = ib; }
     const int n2 = (int)act.size();
     // neighbours by coordinates: the block containing a point a quarter cell
     // across each side, a quarter cell in from each end of it
@@ -1684,9 +1760,173 @@ int adapt_refine_t(const orc_param* p, const orc_blocks* b, const unsigned char*
         }
     }
     for (int k = 0; k < n2; k++) active[k] = act[k];
+    if (remap)
+        for (int i = 0; i < nblkmem; i++) {
+            remap[5 * i] = kind[i];
+            for (int q = 0; q < 4; q++) remap[5 * i + 1 + q] = src[4 * i + q];
+        }
     *nblk_out = n2;
     *nblkmem_out = nblkmem;
     return 0;
+}
+
+// ------------------------------------------------------- state on a new mesh
+// fillCorners (src/Halo.cu:5277-5409) for one block; the top-right and
+// bottom-right corners beside a coarser diagonal block read that block (the
+// reference reads the top-left diagonal there, :5368-5374 and :5399-5405).
+template <class T> void fillCorners(const Blk<T>& B, int ib, T* z) {
+    T zz;
+    int n;
+    // bottom left
+    n = B.LeftTop[B.BotLeft[ib]];
+    if (n == B.BotLeft[ib]) zz = T(0.5) * (z[memloc(-1, 0, ib)] + z[memloc(0, -1, ib)]);
+    else if (B.level[n] == B.level[ib]) zz = z[memloc(BW - 1, BW - 1, n)];
+    else if (B.level[n] > B.level[ib])
+        zz = T(0.25) * (z[memloc(BW - 1, BW - 1, n)] + z[memloc(BW - 1, BW - 2, n)] + z[memloc(BW - 2, BW - 1, n)] + z[memloc(BW - 2, BW - 2, n)]);
+    else zz = T(0.5) * z[memloc(BW - 1, BW - 1, n)] + T(0.25) * (z[memloc(-1, 0, ib)] + z[memloc(0, -1, ib)]);
+    z[memloc(-1, -1, ib)] = zz;
+    // top left
+    n = B.LeftBot[B.TopLeft[ib]];
+    if (n == B.TopLeft[ib]) zz = T(0.5) * (z[memloc(-1, BW - 1, ib)] + z[memloc(0, BW, ib)]);
+    else if (B.level[n] == B.level[ib]) zz = z[memloc(BW - 1, 0, n)];
+    else if (B.level[n] > B.level[ib])
+        zz = T(0.25) * (z[memloc(BW - 1, 0, n)] + z[memloc(BW - 1, 1, n)] + z[memloc(BW - 2, 0, n)] + z[memloc(BW - 2, 1, n)]);
+    else zz = T(0.5) * z[memloc(BW - 1, 0, n)] + T(0.25) * (z[memloc(-1, BW - 1, ib)] + z[memloc(0, BW, ib)]);
+    z[memloc(-1, BW, ib)] = zz;
+    // top right
+    n = B.RightBot[B.TopRight[ib]];
+    if (n == B.TopRight[ib]) zz = T(0.5) * (z[memloc(BW, BW - 1, ib)] + z[memloc(BW - 1, BW, ib)]);
+    else if (B.level[n] == B.level[ib]) zz = z[memloc(0, 0, n)];
+    else if (B.level[n] > B.level[ib])
+        zz = T(0.25) * (z[memloc(0, 0, n)] + z[memloc(0, 1, n)] + z[memloc(1, 0, n)] + z[memloc(1, 1, n)]);
+    else zz = T(0.5) * z[memloc(0, 0, n)] + T(0.25) * (z[memloc(BW, BW - 1, ib)] + z[memloc(BW - 1, BW, ib)]);
+    z[memloc(BW, BW, ib)] = zz;
+    // bottom right
+    n = B.RightBot[B.BotRight[ib]];
+    if (n == B.BotRight[ib]) zz = T(0.5) * (z[memloc(BW - 1, -1, ib)] + z[memloc(BW, 0, ib)]);
+    else if (B.level[n] == B.level[ib]) zz = z[memloc(0, BW - 1, n)];
+    else if (B.level[n] > B.level[ib])
+        zz = T(0.25) * (z[memloc(0, BW - 1, n)] + z[memloc(0, BW - 2, n)] + z[memloc(1, BW - 1, n)] + z[memloc(1, BW - 2, n)]);
+    else zz = T(0.5) * z[memloc(0, BW - 1, n)] + T(0.25) * (z[memloc(BW - 1, -1, ib)] + z[memloc(BW, 0, ib)]);
+    z[memloc(BW, -1, ib)] = zz;
+}
+
+struct BedSource { int nx, ny; double xo, yo, dx; const float* val; };
+
+// interp2BUQ(x, y, forcing) (src/GridManip.cu:520-560)
+template <class T> T bed_bilinear(T x, T y, const BedSource& f) {
+    double xmax = f.xo + (f.nx - 1) * f.dx, ymax = f.yo + (f.ny - 1) * f.dx;
+    double xi = umax(umin(double(x), xmax), f.xo);
+    double yi = umax(umin(double(y), ymax), f.yo);
+    int cfi = std::min(std::max((int)std::floor((xi - f.xo) / f.dx), 0), f.nx - 2);
+    int cfip = cfi + 1;
+    double x1 = f.xo + f.dx * cfi, x2 = f.xo + f.dx * cfip;
+    int cfj = std::min(std::max((int)std::floor((yi - f.yo) / f.dx), 0), f.ny - 2);
+    int cfjp = cfj + 1;
+    double y1 = f.yo + f.dx * cfj, y2 = f.yo + f.dx * cfjp;
+    double q11 = f.val[cfi + cfj * f.nx], q12 = f.val[cfi + cfjp * f.nx];
+    double q21 = f.val[cfip + cfj * f.nx], q22 = f.val[cfip + cfjp * f.nx];
+    return T(BilinearInterpolation(q11, q12, q21, q22, x1, x2, y1, y2, xi, yi));
+}
+
+// blockmean (src/GridManip.cu:475-510)
+template <class T> T bed_blockmean(T x, T y, T dx, const BedSource& f) {
+    double xmin = x - dx * 0.5, xmax = x + dx * 0.5, ymin = y - dx * 0.5, ymax = y + dx * 0.5;
+    int imin = std::max(ftoi(std::floor((xmin - f.xo) / f.dx)), 0);
+    int imax = std::min(ftoi(std::floor((xmax - f.xo) / f.dx)), f.nx - 1);
+    int jmin = std::max(ftoi(std::floor((ymin - f.yo) / f.dx)), 0);
+    int jmax = std::min(ftoi(std::floor((ymax - f.yo) / f.dx)), f.ny - 1);
+    int ni = std::max(imax - imin + 1, 1), nj = std::max(jmax - jmin + 1, 1);
+    T z = T(0.0);
+    for (int i = 0; i < ni; i++)
+        for (int j = 0; j < nj; j++) {
+            int cfi = std::min(imin + i, f.nx - 1), cfj = std::min(jmin + j, f.ny - 1);
+            z = z + f.val[cfi + cfj * f.nx];
+        }
+    z = z / (ni * nj);
+    return z;
+}
+
+// interp2BUQ(x, y, dx, forcing) (src/GridManip.cu:456-466)
+template <class T> T bed_interp(T x, T y, T dx, const BedSource& f) {
+    if (dx <= T(f.dx)) return bed_bilinear(x, y, f);
+    return bed_blockmean(x, y, dx, f);
+}
+
+// The state of M on the mesh (p, b) with the remap table of adapt_mesh_t:
+// the old tiles' halos by the step's fill (fillHaloEv) and corners
+// (fillCorners); kept cells copied; children by refine's bilinear
+// interpolation (src/Adaptation.cu:1032-1102); coarsened cells by coarsen's
+// 2x2 means (:700-760); bed from the source (interp2BUQ) or interpolated like
+// the state; then h = max(eps, zs - zb) where h > eps (:493-512) on new cells;
+// roughness of the parent cell / the mean.
+template <class T> Mdl<T>* remesh_t(Mdl<T>& M, const orc_param* p, const orc_blocks* b, const int* remap,
+                                    const BedSource* bed) {
+    fillHaloEv(M, M.ev);
+    for (int ibl = 0; ibl < M.p.nblk; ibl++) {
+        int ib = M.b.active[ibl];
+        fillCorners(M.b, ib, M.ev.h.data());
+        fillCorners(M.b, ib, M.ev.zs.data());
+        fillCorners(M.b, ib, M.ev.u.data());
+        fillCorners(M.b, ib, M.ev.v.data());
+        fillCorners(M.b, ib, M.zb.data());
+    }
+    Mdl<T>* N = create<T>(p, b);
+    const T eps = T(M.p.eps);
+    const Ev<T>& E = M.ev;
+    for (int ibl = 0; ibl < p->nblk; ibl++) {
+        int ib = b->active[ibl];
+        const int* r = remap + 5 * ib;
+        double blkdx = calcres(p->dx, b->level[ib]);
+        for (int iy = 0; iy < BW; iy++)
+            for (int ix = 0; ix < BW; ix++) {
+                int o = memloc(ix, iy, ib);
+                T h, zs, u, v, zb, cf;
+                if (r[0] == 0) {
+                    N->ev.h[o] = E.h[o]; N->ev.u[o] = E.u[o]; N->ev.v[o] = E.v[o];
+                    N->zb[o] = M.zb[o]; N->cf[o] = M.cf[o];
+                    continue;
+                }
+                if (r[0] == 1) {
+                    int par = r[1], kk = r[2];
+                    int kx[] = {0, BW / 2, 0, BW / 2};
+                    int ky[] = {0, 0, BW / 2, BW / 2};
+                    T lx = ix * T(0.5) - T(0.25);
+                    T ly = iy * T(0.5) - T(0.25);
+                    int fx = (int)std::floor(lx) + kx[kk], cx = (int)std::ceil(lx) + kx[kk];
+                    int fy = (int)std::floor(ly) + ky[kk], cy = (int)std::ceil(ly) + ky[kk];
+                    T rx = (lx) + T(kx[kk]), ry = (ly) + T(ky[kk]);
+                    int ii = memloc(fx, fy, par), ir = memloc(cx, fy, par), it = memloc(fx, cy, par), itr = memloc(cx, cy, par);
+                    auto bl = [&](const T* a) {
+                        return BilinearInterpolation(a[ii], a[it], a[ir], a[itr], (T)fx, (T)cx, (T)fy, (T)cy, rx, ry);
+                    };
+                    h = bl(E.h.data()); zs = bl(E.zs.data()); u = bl(E.u.data()); v = bl(E.v.data());
+                    zb = bl(M.zb.data());
+                    cf = M.cf[memloc(kx[kk] + ix / 2, ky[kk] + iy / 2, par)];
+                } else {
+                    int qx = ix >= BW / 2 ? 1 : 0, qy = iy >= BW / 2 ? 1 : 0;
+                    int c = r[1 + qx + 2 * qy];
+                    int i0 = (ix - qx * BW / 2) * 2, j0 = (iy - qy * BW / 2) * 2;
+                    int ii = memloc(i0, j0, c), ir = memloc(i0 + 1, j0, c), it = memloc(i0, j0 + 1, c), itr = memloc(i0 + 1, j0 + 1, c);
+                    auto mean = [&](const T* a) { return T(0.25) * (a[ii] + a[ir] + a[it] + a[itr]); };
+                    h = mean(E.h.data()); zs = mean(E.zs.data()); u = mean(E.u.data()); v = mean(E.v.data());
+                    zb = mean(M.zb.data());
+                    cf = mean(M.cf.data());
+                }
+                if (bed) {
+                    T x = T(p->xo + b->xo[ib] + ix * blkdx), y = T(p->yo + b->yo[ib] + iy * blkdx);
+                    zb = bed_interp(x, y, T(blkdx), *bed);
+                }
+                if (h > eps) h = umax(eps, zs - zb);
+                N->ev.h[o] = h; N->ev.u[o] = u; N->ev.v[o] = v; N->zb[o] = zb; N->cf[o] = cf;
+            }
+    }
+    InitzbHalo(*N);
+    size_t S = (size_t)p->nblkmem * BS;
+    for (size_t i = 0; i < S; i++) N->ev.zs[i] = N->zb[i] + N->ev.h[i];
+    N->evo = N->ev;
+    calcactiveCellCPU(*N);
+    return N;
 }
 }  // namespace
 
@@ -1694,6 +1934,33 @@ extern "C" int orc_adapt_refine(const orc_param* p, const orc_blocks* b, const u
                                 int* nblk_out, int* nblkmem_out, int* active, int* level, double* xo, double* yo,
                                 int* LB, int* LT, int* RB, int* RT, int* TL, int* TR, int* BL, int* BR) {
     int* nb[8] = {LB, LT, RB, RT, TL, TR, BL, BR};
-    if (p->doubleprecision) return adapt_refine_t<double>(p, b, flags, capacity, nblk_out, nblkmem_out, active, level, xo, yo, nb);
-    return adapt_refine_t<float>(p, b, flags, capacity, nblk_out, nblkmem_out, active, level, xo, yo, nb);
+    if (p->doubleprecision)
+        return adapt_mesh_t<double>(p, b, flags, nullptr, capacity, nblk_out, nblkmem_out, active, level, xo, yo, nb, nullptr);
+    return adapt_mesh_t<float>(p, b, flags, nullptr, capacity, nblk_out, nblkmem_out, active, level, xo, yo, nb, nullptr);
+}
+
+extern "C" int orc_adapt_mesh(const orc_param* p, const orc_blocks* b, const unsigned char* flags,
+                              const unsigned char* cflags, int capacity, int* nblk_out, int* nblkmem_out, int* active,
+                              int* level, double* xo, double* yo, int* LB, int* LT, int* RB, int* RT, int* TL, int* TR,
+                              int* BL, int* BR, int* remap) {
+    int* nb[8] = {LB, LT, RB, RT, TL, TR, BL, BR};
+    if (p->doubleprecision)
+        return adapt_mesh_t<double>(p, b, flags, cflags, capacity, nblk_out, nblkmem_out, active, level, xo, yo, nb, remap);
+    return adapt_mesh_t<float>(p, b, flags, cflags, capacity, nblk_out, nblkmem_out, active, level, xo, yo, nb, remap);
+}
+
+extern "C" int orc_remesh(orc_model m, const orc_param* p, const orc_blocks* b, const int* remap, int bnx, int bny,
+                          double bxo, double byo, double bdx, const float* bed) {
+    BedSource src{bnx, bny, bxo, byo, bdx, bed};
+    const BedSource* ps = bed ? &src : nullptr;
+    if (m->m.dbl) {
+        Mdl<double>* N = remesh_t<double>(*m->m.d, p, b, remap, ps);
+        delete m->m.d;
+        m->m.d = N;
+    } else {
+        Mdl<float>* N = remesh_t<float>(*m->m.f, p, b, remap, ps);
+        delete m->m.f;
+        m->m.f = N;
+    }
+    return 0;
 }

@@ -81,6 +81,18 @@ double orc_calcres_d(double dx, int level);
 int orc_adapt_refine(const orc_param* p, const orc_blocks* b, const unsigned char* flags, int capacity,
                      int* nblk_out, int* nblkmem_out, int* active, int* level, double* xo, double* yo,
                      int* LB, int* LT, int* RB, int* RT, int* TL, int* TR, int* BL, int* BR);
+/* One adaptation pass with refinement and, when cflags is not NULL,
+ * coarsening (the reference's rules, src/Adaptation.cu:157-1876); remap gets
+ * 5 ints per output slot: kind (-1 inactive, 0 kept, 1 child, 2 coarsened)
+ * and the source blocks (child: parent, quadrant; coarsened: BL BR TL TR). */
+int orc_adapt_mesh(const orc_param* p, const orc_blocks* b, const unsigned char* flags, const unsigned char* cflags,
+                   int capacity, int* nblk_out, int* nblkmem_out, int* active, int* level, double* xo, double* yo,
+                   int* LB, int* LT, int* RB, int* RT, int* TL, int* TR, int* BL, int* BR, int* remap);
+/* Moves the model's state onto the mesh (p, b) of orc_adapt_mesh (bed from
+ * the float raster bed[bny][bnx] at (bxo, byo) with spacing bdx when bed is
+ * not NULL). */
+int orc_remesh(orc_model m, const orc_param* p, const orc_blocks* b, const int* remap, int bnx, int bny,
+               double bxo, double byo, double bdx, const float* bed);
 /* Spherical metric terms (src/Spherical.cu:15-64). */
 double orc_calccm_d(double Radius, double delta, double yo, int iy);
 double orc_calcfm_d(double Radius, double delta, double yo, double iy);

@@ -92,6 +92,13 @@ def lib():
         L.orc_adapt_refine.restype = ctypes.c_int
         L.orc_adapt_refine.argtypes = [ctypes.POINTER(OrcParam), ctypes.POINTER(OrcBlocks), ctypes.c_void_p,
                                        ctypes.c_int, ip, ip, ip, ip, dp, dp] + [ip] * 8
+        L.orc_adapt_mesh.restype = ctypes.c_int
+        L.orc_adapt_mesh.argtypes = [ctypes.POINTER(OrcParam), ctypes.POINTER(OrcBlocks), ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_int, ip, ip, ip, ip, dp, dp] + [ip] * 8 + [ip]
+        L.orc_remesh.restype = ctypes.c_int
+        L.orc_remesh.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcParam), ctypes.POINTER(OrcBlocks), ip,
+                                 ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_void_p]
         L.orc_calccm_d.restype = ctypes.c_double
         L.orc_calccm_d.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.orc_calcfm_d.restype = ctypes.c_double
@@ -215,6 +222,73 @@ class Oracle:
 
     def volume(self):
         return lib().orc_volume(self.h)
+
+    def remesh(self, p, blocks, remap, bed=None):
+        """Moves the state onto the mesh (p, blocks) of adapt_mesh (orc_remesh); bed = (values (ny, nx),
+        xo, yo, dx) samples the bed of new cells from a raster."""
+        keep = {}
+        b = _blocks(blocks, p.nblkmem, keep)
+        r = np.ascontiguousarray(remap, np.int32).reshape(-1)
+        if bed is None:
+            rc = lib().orc_remesh(self.h, ctypes.byref(p), ctypes.byref(b), _iptr(r), 0, 0, 0.0, 0.0, 0.0, None)
+        else:
+            vals = np.ascontiguousarray(bed[0], np.float32)
+            rc = lib().orc_remesh(self.h, ctypes.byref(p), ctypes.byref(b), _iptr(r), vals.shape[1], vals.shape[0],
+                                  float(bed[1]), float(bed[2]), float(bed[3]), vals.ctypes.data)
+        assert rc == 0
+        self.p = p
+        self._keep = keep
+        self.size = p.nblkmem * BS
+
+
+def _blocks(blocks, nbm, keep):
+    b = OrcBlocks()
+    for k in ("active", "level", "LeftBot", "LeftTop", "RightBot", "RightTop", "TopLeft", "TopRight",
+              "BotLeft", "BotRight"):
+        a = np.zeros(nbm, np.int32)
+        src = np.asarray(blocks[k], np.int32)
+        a[:len(src)] = src
+        if k == "active":
+            a[len(src):] = -1
+        elif k != "level":
+            a[len(src):] = np.arange(len(src), nbm)
+        keep[k] = a
+        setattr(b, k, _iptr(a))
+    for k in ("xo", "yo"):
+        a = np.zeros(nbm)
+        a[:len(blocks[k])] = blocks[k]
+        keep[k] = a
+        setattr(b, k, _dptr(a))
+    return b
+
+
+def adapt_mesh(p, blocks, flags, cflags, capacity):
+    """One adaptation pass of the oracle with coarsening when cflags is not None (orc_adapt_mesh):
+    (blocks, nblk, nblkmem, remap (nblkmem, 5))."""
+    nbm = p.nblkmem
+    keep = {}
+    b = _blocks(blocks, nbm, keep)
+    f = np.zeros(nbm, np.uint8)
+    f[:len(flags)] = np.asarray(flags, np.uint8)
+    c = None
+    if cflags is not None:
+        c = np.zeros(nbm, np.uint8)
+        c[:len(cflags)] = np.asarray(cflags, np.uint8)
+    keys = ("active", "level", "LeftBot", "LeftTop", "RightBot", "RightTop", "TopLeft", "TopRight", "BotLeft",
+            "BotRight")
+    out = {k: np.zeros(capacity, np.int32) for k in keys}
+    out["xo"] = np.zeros(capacity)
+    out["yo"] = np.zeros(capacity)
+    remap = np.zeros((capacity, 5), np.int32)
+    nb, nm = ctypes.c_int(), ctypes.c_int()
+    rc = lib().orc_adapt_mesh(ctypes.byref(p), ctypes.byref(b), f.ctypes.data, None if c is None else c.ctypes.data,
+                              capacity, ctypes.byref(nb), ctypes.byref(nm), _iptr(out["active"]),
+                              _iptr(out["level"]), _dptr(out["xo"]), _dptr(out["yo"]),
+                              *[_iptr(out[k]) for k in keys[2:]], _iptr(remap))
+    assert rc == 0, "capacity exceeded"
+    res = {k: v[:nm.value].copy() for k, v in out.items()}
+    res["active"] = res["active"][:nb.value]
+    return res, nb.value, nm.value, remap[:nm.value].copy()
 
 
 def adapt_refine(p, blocks, flags, capacity):

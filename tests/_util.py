@@ -34,3 +34,11 @@ def domain_for(sc, precision=None):
 
 def interior(sc):
     return bgf.interior_mask(sc["param"]["nblkmem"], sc["blocks"]["active"][:sc["param"]["nblk"]])
+
+
+def orc_param_with(op, **kw):
+    """A copy of an oracle parameter block with some fields changed (nblk, nblkmem after adaptation)."""
+    q = po.make_param(**{f: getattr(op, f) for f, _ in po.OrcParam._fields_})
+    for k, v in kw.items():
+        setattr(q, k, v)
+    return q
