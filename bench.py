@@ -115,8 +115,20 @@ def ensemble_bench(nmem=64, seed=7, horizon=7200.0, chunk=50, budget_s=90.0, pla
     log("ensemble: %d members, %d cells, built in %.1f s" % (nmem, sum(m["param"]["nblk"] for m in mem) * 256,
                                                              time.time() - t))
     common = dict(mem[0]["param"], precision=prec)
-    B = bgf.Batch(common, max_blocks=sum(m["param"]["nblkmem"] for m in mem), max_members=nmem)
+    cap = sum(m["param"]["nblkmem"] for m in mem)
+    # launch schedule picked by the autotuner on a scratch batch of the same members
+    from bg_flood_amd import autotune
+    t = time.time()
+    margs = [dict(param=m["param"], blocks=m["blocks"], zb=m["zb"], h=m["h"], u=m["u"], v=m["v"], cf=m["cf"],
+                  endtime=horizon) for m in mem]
+    sched, trials = autotune.autotune_batch(common, cap, nmem, margs, steps=12, repeats=2)
+    base = trials[0]["s_per_step"] if trials else None
+    best = min(t_["s_per_step"] for t_ in trials) if trials else None
+    log("ensemble schedule: %s (%d trials in %.1f s; default %.3f ms/step, best %.3f ms/step)"
+        % (sched, len(trials), time.time() - t, 1e3 * (base or 0), 1e3 * (best or 0)))
+    B = bgf.Batch(common, max_blocks=cap, max_members=nmem)
     ids = [B.add(m["param"], m["blocks"], m["zb"], m["h"], m["u"], m["v"], m["cf"], endtime=horizon) for m in mem]
+    B.set_schedule(**sched)
     gpu_sync()
     lat = {}
     t0 = time.perf_counter()
@@ -168,8 +180,65 @@ def ensemble_bench(nmem=64, seed=7, horizon=7200.0, chunk=50, budget_s=90.0, pla
                 plain_step_gcups=round(plain_step, 3), plain_run_gcups=round(plain_run, 3),
                 speedup_vs_plain_step=round(gcups / plain_step, 2), speedup_vs_plain_run=round(gcups / plain_run, 2),
                 roofline_frac=round(gcups * 1e9 * 72 / (HBM_PEAK_GBS * 1e9), 4),
+                schedule=sched, schedule_ms_per_step=dict(default=round(1e3 * base, 4) if base else None,
+                                                          tuned=round(1e3 * best, 4) if best else None),
                 note="latency = wall time from the first batch step until the member reached its horizon, "
                      "checked every %d batch steps" % chunk)
+
+
+def amr_bench(nsteps=600, every=10, prec="fp32", seed=5):
+    """AMR tsunami (3 levels, dx 40/20/10 m over 20 x 10 km, shelf, conical island, 2 m solitary wave)
+    re-adapted on the GPU state every `every` steps (refine where |grad zs| > 1e-4 or h < 5 m, coarsen
+    elsewhere, bed re-sampled from the 10 m raster), against the same steps on the initial mesh."""
+    import bg_flood_amd as bgf
+    from bg_flood_amd import scenarios
+    sc = scenarios.tsunami(precision="fp32", seed=seed)
+    p = dict(sc["param"], precision=prec)
+    out = {}
+    for mode in ("static", "adaptive"):
+        d = bgf.Domain(p, sc["blocks"])
+        d.set_state(sc["zb"], sc["h"], sc["u"], sc["v"], sc["cf"])
+        d.set_bed_source(*sc["bed_raster"])
+        L = bgf.Loop(nextoutputtime=1e30, dt=0, dtmax=0, totaltime=0)
+        d.init_dt(L)
+        d.run(L, 20)  # warm-up: graph capture
+        gpu_sync()
+        cells = 0
+        t0 = time.perf_counter()
+        if mode == "static":
+            d.run(L, nsteps)
+            cells = nsteps * d.nblk * 256
+            recs = []
+        else:
+            recs = []
+            done = 0
+            while done < nsteps:
+                n = min(every, nsteps - done)
+                d.run(L, n)
+                cells += n * d.nblk * 256
+                done += n
+                if done < nsteps:
+                    a = time.perf_counter()
+                    n0, n1, nref, nco = d.adapt(sc["criteria"], sc["nbx"], sc["nby"], coarsen=True)
+                    recs.append((n0, n1, nref, nco, time.perf_counter() - a))
+        gpu_sync()
+        wall = time.perf_counter() - t0
+        vol = d.volume()
+        blocks = [r[1] for r in recs] or [d.nblk]
+        out[mode] = dict(steps=nsteps, wall_s=round(wall, 3), gcups=round(cells / wall / 1e9, 3),
+                         sim_time_s=round(L.totaltime, 2), blocks_min=int(min(blocks)), blocks_max=int(max(blocks)),
+                         volume_m3=vol)
+        if recs:
+            out[mode].update(adaptations=len(recs), refined=int(sum(r[2] for r in recs)),
+                             coarsened=int(sum(r[3] for r in recs)),
+                             adapt_s=round(sum(r[4] for r in recs), 3),
+                             adapt_ms_mean=round(1e3 * sum(r[4] for r in recs) / len(recs), 2))
+        log("amr %s: %s" % (mode, out[mode]))
+        d.close()
+    out.update(precision=prec, every=every, initial_blocks=sc["param"]["nblk"],
+               note="adaptive: flags on the GPU state, adaptation pass on the host, state remesh on the GPU; "
+                    "gcups counts the active cells of each step")
+    return out
 
 
 def serving_bench(lam=4.0, duration=20.0, seed=11, compress=1.0 / 60.0, lmin=256, lmax=4096, frac_amr=0.2,
@@ -263,6 +332,8 @@ def main():
     ap.add_argument("--ensemble-only", action="store_true")
     ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-amr", action="store_true")
+    ap.add_argument("--amr-only", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (scale reference for HBM byte counters)")
     args = ap.parse_args()
@@ -279,6 +350,9 @@ def main():
         return
     if args.serving_only:
         print(json.dumps(serving_bench()), flush=True)
+        return
+    if args.amr_only:
+        print(json.dumps(amr_bench()), flush=True)
         return
 
     from bg_flood_amd import scenarios
@@ -328,6 +402,12 @@ def main():
             ens = ensemble_bench()
         except Exception as e:  # report, never hide
             ens = dict(error=repr(e))
+    amr = None
+    if not args.no_amr:
+        try:
+            amr = amr_bench()
+        except Exception as e:  # report, never hide
+            amr = dict(error=repr(e))
     srv = None
     if not args.no_serving:
         try:
@@ -358,6 +438,7 @@ def main():
         "variants": variants,
         "ensemble": ens,
         "serving": srv,
+        "amr": amr,
     }
     print(json.dumps(out), flush=True)
 

@@ -20,11 +20,14 @@
 // which holds here since zs is always zb + h).  Roughness: a child cell keeps
 // its parent cell's value, a coarsened cell takes the mean.
 //
-// fillCorners as restated here reads, for the top-right and bottom-right
-// corners beside a coarser diagonal block, that diagonal block; the reference
-// reads the top-left diagonal there (src/Halo.cu:5368-5374, 5399-5405).  The
-// reference fills corners only during its set-up adaptation, whose state it
-// then overwrites with the initial conditions.
+// Corners beside a coarser diagonal block differ from fillCorners: the
+// reference takes half of that block's corner cell, which touches the corner
+// only when the two blocks' corners meet (elsewhere it lies up to eight coarse
+// cells away, and can be a dry cell feeding a deep-water child), and for the
+// top-right and bottom-right corners it reads the top-left diagonal block
+// (src/Halo.cu:5338-5344, 5368-5374, 5399-5405).  The reference fills corners
+// only during its set-up adaptation, whose state it then overwrites with the
+// initial conditions.
 #include <hip/hip_runtime.h>
 
 #include "bgf_common.h"
@@ -61,8 +64,8 @@ template <class T> struct Field {
 // Corner value of block ib (cx, cy in {-1, 16}) following fillCorners
 // (src/Halo.cu:5277-5409): no diagonal block -> mean of the two adjacent halo
 // cells; same level -> its nearest cell; finer -> mean of its 2x2 corner
-// cells; coarser -> half its corner cell plus a quarter of each adjacent halo
-// cell.
+// cells.  Beside a coarser diagonal block the corner is extrapolated from the
+// block's own tile, halo + halo - corner cell (exact for planar fields).
 template <class T, class F> __device__ T corner_value(const KParams& P, int ib, int cx, int cy, const F& z) {
     const bool right = cx >= 0, top = cy >= 0;
     // the diagonal block, through the bottom or top neighbour (src/Halo.cu:5290, 5317, 5349, 5380)
@@ -84,7 +87,7 @@ template <class T, class F> __device__ T corner_value(const KParams& P, int ib, 
                   itr = memloc(ex + sx, ey + sy, n);
         return T(0.25) * (z(ii) + z(ir) + z(it) + z(itr));
     }
-    return T(0.5) * z(memloc(ex, ey, n)) + T(0.25) * (z(hx) + z(hy));
+    return z(hx) + z(hy) - z(memloc(cx - sx, cy - sy, ib));
 }
 
 // Value of a field at tile cell (x, y) of block ib, x, y in [-1, 16].

@@ -393,8 +393,36 @@ def tsunami(nbx=32, nby=16, dx=40.0, maxlevel=2, amp=2.0, seed=5, precision="fp3
     h = np.where(m, h64, 0.0).astype(T)
     u = np.where(m, u64, 0.0).astype(T)
     z = np.zeros(S, T)
+    # the bed as a raster at the finest cell size, for re-sampling after in-run adaptation
+    rdx = dx / (1 << maxlevel)
+    gx = np.arange(0.0, Lx + 0.5 * rdx, rdx)
+    gy = np.arange(0.0, Ly + 0.5 * rdx, rdx)
+    Xg, Yg = np.meshgrid(gx, gy)
+    raster = (bed(Xg, Yg).astype(np.float32), 0.0, 0.0, rdx)
     return dict(param=p, blocks=blocks, nbx=nbx, nby=nby, zb=zb, h=h, u=u, v=z.copy(), cf=np.full(S, T(0.025), T),
-                leaves=leaves)
+                leaves=leaves, bed_raster=raster, maxlevel=maxlevel,
+                criteria=[("zs", 2, refine_slope, 0.0), ("h", 1, -1.0, refine_depth)])
+
+
+def run_adaptive(domain, loop, nsteps, nbx, nby, criteria, every=10, coarsen=True, endtime=1e300, log=None):
+    """Runs nsteps on a multi-level Domain, re-adapting the mesh every `every` steps from the criteria
+    on the GPU state (Domain.adapt).  Returns per-adaptation records (step, blocks before, after,
+    refined, coarsened, seconds spent adapting)."""
+    import time
+    recs = []
+    done = 0
+    while done < nsteps:
+        n = min(every, nsteps - done)
+        domain.run(loop, n, endtime=endtime)
+        done += n
+        if done >= nsteps or loop.totaltime >= endtime:
+            break
+        t0 = time.perf_counter()
+        n0, n1, nref, nco = domain.adapt(criteria, nbx, nby, coarsen=coarsen)
+        recs.append((done, n0, n1, nref, nco, time.perf_counter() - t0))
+        if log:
+            log("adapt at step %d: %d -> %d blocks (%d refined, %d coarsened)" % (done, n0, n1, nref, nco))
+    return recs
 
 
 def block_any(blocks, nblk, nblkmem, dx, fn, precision="fp64"):

@@ -1771,9 +1771,11 @@ int adapt_mesh_t(const orc_param* p, const orc_blocks* b, const unsigned char* f
 }
 
 // ------------------------------------------------------- state on a new mesh
-// fillCorners (src/Halo.cu:5277-5409) for one block; the top-right and
-// bottom-right corners beside a coarser diagonal block read that block (the
-// reference reads the top-left diagonal there, :5368-5374 and :5399-5405).
+// fillCorners (src/Halo.cu:5277-5409) for one block, except beside a coarser
+// diagonal block: there the corner is extrapolated from the block's own tile
+// (halo + halo - corner cell) instead of half the coarse block's corner cell
+// plus a quarter of each halo cell (:5305-5311 and mirrors), whose corner cell
+// touches the corner only when the block corners meet.
 template <class T> void fillCorners(const Blk<T>& B, int ib, T* z) {
     T zz;
     int n;
@@ -1783,7 +1785,7 @@ template <class T> void fillCorners(const Blk<T>& B, int ib, T* z) {
     else if (B.level[n] == B.level[ib]) zz = z[memloc(BW - 1, BW - 1, n)];
     else if (B.level[n] > B.level[ib])
         zz = T(0.25) * (z[memloc(BW - 1, BW - 1, n)] + z[memloc(BW - 1, BW - 2, n)] + z[memloc(BW - 2, BW - 1, n)] + z[memloc(BW - 2, BW - 2, n)]);
-    else zz = T(0.5) * z[memloc(BW - 1, BW - 1, n)] + T(0.25) * (z[memloc(-1, 0, ib)] + z[memloc(0, -1, ib)]);
+    else zz = z[memloc(-1, 0, ib)] + z[memloc(0, -1, ib)] - z[memloc(0, 0, ib)];
     z[memloc(-1, -1, ib)] = zz;
     // top left
     n = B.LeftBot[B.TopLeft[ib]];
@@ -1791,7 +1793,7 @@ template <class T> void fillCorners(const Blk<T>& B, int ib, T* z) {
     else if (B.level[n] == B.level[ib]) zz = z[memloc(BW - 1, 0, n)];
     else if (B.level[n] > B.level[ib])
         zz = T(0.25) * (z[memloc(BW - 1, 0, n)] + z[memloc(BW - 1, 1, n)] + z[memloc(BW - 2, 0, n)] + z[memloc(BW - 2, 1, n)]);
-    else zz = T(0.5) * z[memloc(BW - 1, 0, n)] + T(0.25) * (z[memloc(-1, BW - 1, ib)] + z[memloc(0, BW, ib)]);
+    else zz = z[memloc(-1, BW - 1, ib)] + z[memloc(0, BW, ib)] - z[memloc(0, BW - 1, ib)];
     z[memloc(-1, BW, ib)] = zz;
     // top right
     n = B.RightBot[B.TopRight[ib]];
@@ -1799,7 +1801,7 @@ template <class T> void fillCorners(const Blk<T>& B, int ib, T* z) {
     else if (B.level[n] == B.level[ib]) zz = z[memloc(0, 0, n)];
     else if (B.level[n] > B.level[ib])
         zz = T(0.25) * (z[memloc(0, 0, n)] + z[memloc(0, 1, n)] + z[memloc(1, 0, n)] + z[memloc(1, 1, n)]);
-    else zz = T(0.5) * z[memloc(0, 0, n)] + T(0.25) * (z[memloc(BW, BW - 1, ib)] + z[memloc(BW - 1, BW, ib)]);
+    else zz = z[memloc(BW, BW - 1, ib)] + z[memloc(BW - 1, BW, ib)] - z[memloc(BW - 1, BW - 1, ib)];
     z[memloc(BW, BW, ib)] = zz;
     // bottom right
     n = B.RightBot[B.BotRight[ib]];
@@ -1807,7 +1809,7 @@ template <class T> void fillCorners(const Blk<T>& B, int ib, T* z) {
     else if (B.level[n] == B.level[ib]) zz = z[memloc(0, BW - 1, n)];
     else if (B.level[n] > B.level[ib])
         zz = T(0.25) * (z[memloc(0, BW - 1, n)] + z[memloc(0, BW - 2, n)] + z[memloc(1, BW - 1, n)] + z[memloc(1, BW - 2, n)]);
-    else zz = T(0.5) * z[memloc(0, BW - 1, n)] + T(0.25) * (z[memloc(BW - 1, -1, ib)] + z[memloc(BW, 0, ib)]);
+    else zz = z[memloc(BW, 0, ib)] + z[memloc(BW - 1, -1, ib)] - z[memloc(BW - 1, 0, ib)];
     z[memloc(BW, -1, ib)] = zz;
 }
 

@@ -253,3 +253,29 @@ def test_coarsening_conserves_volume_without_a_raster():
     o1.set_state(*state_arrays(b1, m1, dx, T, field, lambda x, y: 0 * x, lambda x, y: 0 * x,
                                lambda x, y: -5.0 + 0 * x))
     assert abs(o.volume() - o1.volume()) <= 1e-12 * o1.volume()
+
+
+def test_lake_at_rest_away_from_the_shore_on_an_island_mesh():
+    """Still water around an island on a 3-level mesh, refined where the bed crosses the still level
+    and coarsened elsewhere, bed re-sampled from a raster: every cell deeper than 1 m keeps zs = 0
+    exactly, including child cells at corners beside coarser diagonal blocks (the corner is
+    extrapolated from the block's own tile there)."""
+    from tests.test_gpu_remesh import scenario, raster, NBX, NBY
+    from tests._util import oracle_for, orc_param_with
+    sc = scenario("fp64")
+    sc["h"] = np.maximum(0.0 - sc["zb"], 0.0)
+    o = oracle_for(sc, precision="fp64")
+    p, blocks = sc["param"], sc["blocks"]
+    zb = sc["zb"].reshape(-1, MW, MW)[:, 1:MW - 1, 1:MW - 1]
+    act = np.asarray(blocks["active"][:p["nblk"]])
+    ref = np.zeros(p["nblkmem"], np.uint8)
+    ref[act] = ((zb[act] >= -0.5) & (zb[act] <= 0.5)).any(axis=(1, 2))
+    co = np.zeros(p["nblkmem"], np.uint8)
+    co[act] = 1 - ref[act]
+    b2, nb, nm, remap = bgf.adapt_mesh(p, blocks, ref, co, NBX, NBY)
+    assert np.any(remap[b2["active"][:nb], 0] == 1)
+    o.remesh(orc_param_with(o.p, nblk=nb, nblkmem=nm), b2, remap, bed=raster())
+    m = interior_of(b2, nb, nm)
+    zs, z = o.get("zs")[m], o.get("zb")[m]
+    deep = z < -1.0
+    assert deep.sum() > 1000 and np.max(np.abs(zs[deep])) == 0.0
