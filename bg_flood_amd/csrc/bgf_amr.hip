@@ -712,8 +712,8 @@ __global__ __launch_bounds__(256) void k_amr_advance(KParams P, AdvArgs<T> A, Fl
         // cleanup then implicit uniform rain, src/Updateforcing.cu:530-581
         if (P.rain_on) {
             const T act = (double)A.zb[i] < P.mask ? T(1.0) : T(0.0);
-            T rainhh = T(L->rain);
-            rainhh = smax(rainhh / T(1000.0) / T(3600.0) * T(L->dt), T(0.0)) * act;
+            const T rate = sizeof(T) == 8 ? T(L->rate64) : T(L->rate32);  // T(rain) / T(1000) / T(3600)
+            T rainhh = smax(rate * T(L->dt), T(0.0)) * act;
             const T qvol = ho / (ho + rainhh);
             const bool wet = (double)ho > P.eps;
             ho = ho + rainhh;

@@ -377,7 +377,7 @@ bgf_status bgf_batch_add(bgf_batch b, const bgf_param* p, const bgf_blocks* blk,
     L.done = 0;
     L.nextoutputtime = endtime;
     L.endtime = endtime;
-    L.rain = M.rain;
+    bgf::set_rain(L, M.rain);
     L.dtmax = dtinit > 0 ? dtinit / 1.5 : p->delta / std::sqrt(p->g * 10.0);
     HIPCHK(hipMemcpy(b->d_loop + m, &L, sizeof(L), hipMemcpyHostToDevice));
     const bgf::KParams P = member_view(b, m, active_offset(b, m));

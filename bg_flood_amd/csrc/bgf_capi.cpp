@@ -56,6 +56,14 @@ struct bgf_handle_s {
     // launch schedule of the persistent one-level kernels
     BlockXY xy;
     int tile_x = TILE_X, tile_y = TILE_Y, wg_pred = 0, wg_corr = 0;
+    // block slots the device arrays hold (>= nblkmem; adapted meshes grow into it)
+    int cap = 0;
+    // bgf_remesh staging: the new mesh (active, level, 8 neighbours, inv), the
+    // remap table and block origins; refine flags
+    int* d_mesh2[11] = {nullptr};
+    int* d_remap = nullptr;
+    double *d_bxo = nullptr, *d_byo = nullptr;
+    unsigned char* d_flags = nullptr;
     // bed source raster sampled by bgf_remesh for new cells
     float* d_bed = nullptr;
     int bed_nx = 0, bed_ny = 0;
@@ -75,6 +83,8 @@ void free_handle(bgf_handle h) {
     F(h->d_ch);
     F(h->d_metric);
     F(h->d_bed);
+    for (int k = 0; k < 11; k++) F(h->d_mesh2[k]);
+    F(h->d_remap); F(h->d_bxo); F(h->d_byo); F(h->d_flags);
     for (int k = 0; k < 17; k++) F(h->d_amr[k]);
     for (int k = 0; k < 4; k++) if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
     if (h->graph) (void)hipGraphExecDestroy(h->graph);
@@ -90,7 +100,7 @@ bgf_status upload_loop(bgf_handle h, const bgf_loop* loop, double endtime) {
     L.dtmax = loop->dtmax;
     L.totaltime = loop->totaltime;
     L.endtime = endtime;
-    L.rain = (double)(float)h->p.rain;  // Rain.nowvalue is a float (src/Forcing.h)
+    bgf::set_rain(L, (double)(float)h->p.rain);  // Rain.nowvalue is a float (src/Forcing.h)
     L.done = loop->totaltime >= endtime ? 1 : 0;
     for (int k = 0; k < bgf::NSLOT; k++) {
         L.slot32[k] = 0x7f800000u;
@@ -234,27 +244,29 @@ int bgf_device_count(void) {
     return n;
 }
 
-bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) {
-    if (!out || !b) return fail(BGF_ERR_INVALID_ARG, "null argument");
-    *out = nullptr;
-    bgf_status s = check_param(p);
-    if (s != BGF_OK) return s;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(BGF_ERR_NO_DEVICE, "no HIP device");
-    const int nblk = p->nblk, nbm = p->nblkmem;
-    // Mesh validation.  One-level meshes (every side a boundary or one
-    // same-level neighbour) run the fused kernels; others the multi-level ones,
-    // which need neighbour levels within one of each other (2:1 balance).
+}  // extern "C"
+
+namespace {
+
+// Mesh validation.  One-level meshes (every side a boundary or one same-level
+// neighbour) run the fused kernels; others the multi-level ones, which need
+// neighbour levels within one of each other (2:1 balance).
+struct MeshInfo {
     bool multilevel = false;
     int minlev = 1 << 30, maxlev = -(1 << 30);
-    std::vector<int> inv(nbm, -1);
+    std::vector<int> inv;
+};
+
+bgf_status check_mesh(const bgf_param* p, const bgf_blocks* b, MeshInfo& m) {
+    const int nblk = p->nblk, nbm = p->nblkmem;
+    m.inv.assign(nbm, -1);
     for (int ibl = 0; ibl < nblk; ibl++) {
         int ib = b->active[ibl];
         if (ib < 0 || ib >= nbm) return fail(BGF_ERR_INVALID_ARG, "active block index out of range");
-        if (inv[ib] >= 0) return fail(BGF_ERR_INVALID_ARG, "block listed twice in active");
-        inv[ib] = ibl;
-        minlev = std::min(minlev, b->level[ib]);
-        maxlev = std::max(maxlev, b->level[ib]);
+        if (m.inv[ib] >= 0) return fail(BGF_ERR_INVALID_ARG, "block listed twice in active");
+        m.inv[ib] = ibl;
+        m.minlev = std::min(m.minlev, b->level[ib]);
+        m.maxlev = std::max(m.maxlev, b->level[ib]);
     }
     for (int ibl = 0; ibl < nblk; ibl++) {
         int ib = b->active[ibl];
@@ -263,48 +275,79 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
         for (auto& pr : pairs) {
             int n0 = pr[0][ib], n1 = pr[1][ib];
             if (n0 < 0 || n0 >= nbm || n1 < 0 || n1 >= nbm) return fail(BGF_ERR_INVALID_ARG, "neighbour out of range");
-            if (inv[n0] < 0 || inv[n1] < 0) return fail(BGF_ERR_INVALID_ARG, "neighbour is not an active block");
+            if (m.inv[n0] < 0 || m.inv[n1] < 0) return fail(BGF_ERR_INVALID_ARG, "neighbour is not an active block");
             for (int n : {n0, n1})
                 if (std::abs(b->level[n] - b->level[ib]) > 1)
                     return fail(BGF_ERR_INVALID_ARG, "neighbour levels differ by more than one");
-            if (n0 != n1 || (n0 != ib && b->level[n0] != b->level[ib])) multilevel = true;
+            if (n0 != n1 || (n0 != ib && b->level[n0] != b->level[ib])) m.multilevel = true;
         }
     }
-    if (minlev != maxlev) multilevel = true;
+    if (m.minlev != m.maxlev) m.multilevel = true;
     // spherical grids run on the general (multi-level) kernels, which take per-row metric terms
-    if (p->spherical) multilevel = true;
+    if (p->spherical) m.multilevel = true;
+    return BGF_OK;
+}
+
+// Per-block metric terms of a spherical mesh, uploaded to the handle.
+bool upload_metric(bgf_handle h, const bgf_param* p, const bgf_blocks* b, int nbm) {
+    if (h->prec == 0) {
+        std::vector<double> M = metric_table<double>(p, b, nbm);
+        return hipMemcpy(h->d_metric, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice) == hipSuccess;
+    }
+    std::vector<float> M = metric_table<float>(p, b, nbm);
+    return hipMemcpy(h->d_metric, M.data(), sizeof(float) * M.size(), hipMemcpyHostToDevice) == hipSuccess;
+}
+
+// bgf_create with device arrays for cap >= nblkmem block slots.
+bgf_status create_handle(const bgf_param* p, const bgf_blocks* b, int cap, bgf_handle* out) {
+    if (!out || !b) return fail(BGF_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    bgf_status s = check_param(p);
+    if (s != BGF_OK) return s;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(BGF_ERR_NO_DEVICE, "no HIP device");
+    const int nblk = p->nblk, nbm = p->nblkmem;
+    if (cap < nbm) cap = nbm;
+    MeshInfo mi;
+    s = check_mesh(p, b, mi);
+    if (s != BGF_OK) return s;
+    const bool multilevel = mi.multilevel;
+    const int minlev = mi.minlev, maxlev = mi.maxlev;
+    std::vector<int>& inv = mi.inv;
     auto* h = new bgf_handle_s();
     h->p = *p;
     h->prec = p->precision;
     h->es = p->precision == 0 ? sizeof(double) : sizeof(float);
     h->nblk = nblk;
     h->nblkmem = nbm;
+    h->cap = cap;
     h->S = (size_t)nbm * BS;
+    const size_t SA = (size_t)cap * BS;  // values per allocated array
     h->active.assign(b->active, b->active + nblk);
     auto A = [&](void** ptr, size_t bytes) -> bool {
         if (hipMalloc(ptr, bytes) != hipSuccess) return false;
         return hipMemset(*ptr, 0, bytes) == hipSuccess;
     };
     bool ok = true;
-    ok = ok && A((void**)&h->d_active, sizeof(int) * nblk);
-    ok = ok && A((void**)&h->d_order, sizeof(int) * nblk);
-    ok = ok && A((void**)&h->d_level, sizeof(int) * nbm);
-    for (int k = 0; k < 8; k++) ok = ok && A((void**)&h->d_nb[k], sizeof(int) * nbm);
+    ok = ok && A((void**)&h->d_active, sizeof(int) * cap);
+    ok = ok && A((void**)&h->d_order, sizeof(int) * cap);
+    ok = ok && A((void**)&h->d_level, sizeof(int) * cap);
+    for (int k = 0; k < 8; k++) ok = ok && A((void**)&h->d_nb[k], sizeof(int) * cap);
     for (int k = 0; k < 2; k++) {
-        ok = ok && A(&h->d_h[k], h->es * h->S);
-        ok = ok && A(&h->d_u[k], h->es * h->S);
-        ok = ok && A(&h->d_v[k], h->es * h->S);
+        ok = ok && A(&h->d_h[k], h->es * SA);
+        ok = ok && A(&h->d_u[k], h->es * SA);
+        ok = ok && A(&h->d_v[k], h->es * SA);
     }
-    ok = ok && A(&h->d_zb, h->es * h->S) && A(&h->d_cf, h->es * h->S);
-    ok = ok && A(&h->d_dh, h->es * h->S) && A(&h->d_dhu, h->es * h->S) && A(&h->d_dhv, h->es * h->S);
-    ok = ok && A((void**)&h->d_volpart, sizeof(double) * nblk) && A((void**)&h->d_volout, sizeof(double));
+    ok = ok && A(&h->d_zb, h->es * SA) && A(&h->d_cf, h->es * SA);
+    ok = ok && A(&h->d_dh, h->es * SA) && A(&h->d_dhu, h->es * SA) && A(&h->d_dhv, h->es * SA);
+    ok = ok && A((void**)&h->d_volpart, sizeof(double) * cap) && A((void**)&h->d_volout, sizeof(double));
     ok = ok && A((void**)&h->d_loop, sizeof(bgf::DevLoop));
     h->amr = multilevel;
-    if (p->precision == BGF_FP32_COMP && !multilevel) ok = ok && A(&h->d_ch, h->es * h->S);
-    if (p->spherical) ok = ok && A(&h->d_metric, h->es * (size_t)nbm * bgf::METRIC_STRIDE);
+    if (p->precision == BGF_FP32_COMP && !multilevel) ok = ok && A(&h->d_ch, h->es * SA);
+    if (p->spherical) ok = ok && A(&h->d_metric, h->es * (size_t)cap * bgf::METRIC_STRIDE);
     if (multilevel) {
-        ok = ok && A((void**)&h->d_inv, sizeof(int) * nbm);
-        for (int k = 0; k < 17; k++) ok = ok && A(&h->d_amr[k], h->es * h->S);
+        ok = ok && A((void**)&h->d_inv, sizeof(int) * cap);
+        for (int k = 0; k < 17; k++) ok = ok && A(&h->d_amr[k], h->es * SA);
     }
     if (!ok) {
         free_handle(h);
@@ -318,15 +361,7 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
     cp = cp && hipMemcpy(h->d_order, order.data(), sizeof(int) * nblk, hipMemcpyHostToDevice) == hipSuccess;
     cp = cp && hipMemcpy(h->d_level, b->level, sizeof(int) * nbm, hipMemcpyHostToDevice) == hipSuccess;
     for (int k = 0; k < 8; k++) cp = cp && hipMemcpy(h->d_nb[k], nbs[k], sizeof(int) * nbm, hipMemcpyHostToDevice) == hipSuccess;
-    if (p->spherical) {
-        if (h->prec == 0) {
-            std::vector<double> M = metric_table<double>(p, b, nbm);
-            cp = cp && hipMemcpy(h->d_metric, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice) == hipSuccess;
-        } else {
-            std::vector<float> M = metric_table<float>(p, b, nbm);
-            cp = cp && hipMemcpy(h->d_metric, M.data(), sizeof(float) * M.size(), hipMemcpyHostToDevice) == hipSuccess;
-        }
-    }
+    if (p->spherical) cp = cp && upload_metric(h, p, b, nbm);
     if (multilevel) {
         cp = cp && hipMemcpy(h->d_inv, inv.data(), sizeof(int) * nbm, hipMemcpyHostToDevice) == hipSuccess;
     }
@@ -378,6 +413,14 @@ bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) 
     *out = h;
     g_err.clear();
     return BGF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+bgf_status bgf_create(const bgf_param* p, const bgf_blocks* b, bgf_handle* out) {
+    return create_handle(p, b, p ? p->nblkmem : 0, out);
 }
 
 bgf_status bgf_destroy(bgf_handle h) {
@@ -528,17 +571,13 @@ bgf_status bgf_refine_flags(bgf_handle h, const char* field, int mode, double a,
     else if (f == "zb") x = h->d_zb;
     else if (f == "zs") { x = h->d_h[0]; y = h->d_zb; }
     else return fail(BGF_ERR_INVALID_ARG, "unknown field " + f);
-    unsigned char* d = nullptr;
-    HIPCHK(hipMalloc((void**)&d, h->nblkmem));
-    if (hipMemsetAsync(d, 0, h->nblkmem, h->stream) != hipSuccess) {
-        (void)hipFree(d);
-        return fail(BGF_ERR_HIP, "refine flags: memset failed");
-    }
+    if (!h->d_flags) HIPCHK(hipMalloc((void**)&h->d_flags, std::max(h->cap, h->nblkmem)));
+    unsigned char* d = h->d_flags;
+    HIPCHK(hipMemsetAsync(d, 0, h->nblkmem, h->stream));
     bgf::launch_refine_flags(h->prec, h->kp, x, y, mode, a, b, d, h->stream);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(flags, d, h->nblkmem, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    (void)hipFree(d);
     if (e != hipSuccess) return fail(BGF_ERR_HIP, std::string("refine flags: ") + hipGetErrorString(e));
     return BGF_OK;
 }
@@ -575,12 +614,15 @@ bgf_status bgf_remesh(bgf_handle h, const bgf_param* p, const bgf_blocks* b, con
     q.nblkmem = p->nblkmem;
     const int nbo = h->nblkmem, nbm = q.nblkmem;
     if (q.nblk <= 0 || nbm < q.nblk) return fail(BGF_ERR_INVALID_ARG, "bad sizes");
+    MeshInfo mi;
+    bgf_status s = check_mesh(&q, b, mi);
+    if (s != BGF_OK) return s;
+    if (!mi.multilevel) return fail(BGF_ERR_UNSUPPORTED, "an adapted mesh must stay on the multi-level kernels");
     std::vector<char> was(nbo, 0);
     for (int ib : h->active) was[ib] = 1;
     auto old_ok = [&](int i) { return i >= 0 && i < nbo && was[i]; };
     for (int k = 0; k < q.nblk; k++) {
         const int ib = b->active[k];
-        if (ib < 0 || ib >= nbm) return fail(BGF_ERR_INVALID_ARG, "active block index out of range");
         const int* r = remap + 5 * ib;
         bool ok = false;
         if (r[0] == 0) ok = r[1] == ib && old_ok(ib);
@@ -588,53 +630,127 @@ bgf_status bgf_remesh(bgf_handle h, const bgf_param* p, const bgf_blocks* b, con
         else if (r[0] == 2) ok = old_ok(r[1]) && old_ok(r[2]) && old_ok(r[3]) && old_ok(r[4]);
         if (!ok) return fail(BGF_ERR_INVALID_ARG, "remap entry of block " + std::to_string(ib) + " is not valid");
     }
-    bgf_handle nh = nullptr;
-    bgf_status s = bgf_create(&q, b, &nh);
-    if (s != BGF_OK) return s;
-    int* d_remap = nullptr;
-    double *d_xo = nullptr, *d_yo = nullptr;
-    auto cleanup = [&](bgf_status e) {
-        (void)hipFree(d_remap);
-        (void)hipFree(d_xo);
-        (void)hipFree(d_yo);
-        if (nh) bgf_destroy(nh);
-        return e;
-    };
-    bool ok = hipMalloc((void**)&d_remap, sizeof(int) * 5 * (size_t)nbm) == hipSuccess &&
-              hipMalloc((void**)&d_xo, sizeof(double) * nbm) == hipSuccess &&
-              hipMalloc((void**)&d_yo, sizeof(double) * nbm) == hipSuccess;
-    ok = ok && hipMemcpy(d_remap, remap, sizeof(int) * 5 * (size_t)nbm, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(d_xo, b->xo, sizeof(double) * nbm, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(d_yo, b->yo, sizeof(double) * nbm, hipMemcpyHostToDevice) == hipSuccess;
-    if (!ok) return cleanup(fail(BGF_ERR_HIP, "remesh: upload failed"));
     const int prec = h->prec == 0 ? 0 : 1;
-    // the old state's halo, as the step fills it, then the new cells
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (nbm > h->cap) {
+        // grow: a handle with room for half as many blocks again, the state moved into it
+        bgf_handle nh = nullptr;
+        s = create_handle(&q, b, nbm + nbm / 2, &nh);
+        if (s != BGF_OK) return s;
+        bool ok = hipMalloc((void**)&nh->d_remap, sizeof(int) * 5 * (size_t)nh->cap) == hipSuccess &&
+                  hipMalloc((void**)&nh->d_bxo, sizeof(double) * nh->cap) == hipSuccess &&
+                  hipMalloc((void**)&nh->d_byo, sizeof(double) * nh->cap) == hipSuccess;
+        ok = ok && hipMemcpy(nh->d_remap, remap, sizeof(int) * 5 * (size_t)nbm, hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(nh->d_bxo, b->xo, sizeof(double) * nbm, hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(nh->d_byo, b->yo, sizeof(double) * nbm, hipMemcpyHostToDevice) == hipSuccess;
+        if (!ok) {
+            bgf_destroy(nh);
+            return fail(BGF_ERR_HIP, "remesh: upload failed");
+        }
+        bgf::launch_amr_halo(prec, h->kp, h->d_h[0], h->d_u[0], h->d_v[0], h->d_zb, h->stream);
+        bgf::RemeshLaunch R;
+        R.nblk = q.nblk;
+        R.h = h->d_h[0]; R.u = h->d_u[0]; R.v = h->d_v[0]; R.zb = h->d_zb; R.cf = h->d_cf;
+        R.active = nh->d_active; R.level = nh->d_level; R.xo = nh->d_bxo; R.yo = nh->d_byo; R.remap = nh->d_remap;
+        R.nh = nh->d_h[0]; R.nu = nh->d_u[0]; R.nv = nh->d_v[0]; R.nzb = nh->d_zb; R.ncf = nh->d_cf;
+        R.bed = h->d_bed; R.bnx = h->bed_nx; R.bny = h->bed_ny; R.bxo = h->bed_xo; R.byo = h->bed_yo;
+        R.bdx = h->bed_dx; R.pxo = q.xo; R.pyo = q.yo; R.dx = q.dx; R.eps = q.eps;
+        bgf::launch_remesh(prec, h->kp, R, h->stream);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e == hipSuccess) {
+            bgf::launch_amr_fill(prec, nh->kp, nh->d_zb, nh->stream);  // the bed's halo (src/InitialConditions.cu:135-147)
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipStreamSynchronize(nh->stream);
+        }
+        if (e != hipSuccess) {
+            bgf_destroy(nh);
+            return fail(BGF_ERR_HIP, std::string("remesh: ") + hipGetErrorString(e));
+        }
+        nh->state_set = true;
+        nh->timing = h->timing;
+        nh->use_graph = h->use_graph;
+        std::swap(nh->d_bed, h->d_bed);
+        nh->bed_nx = h->bed_nx; nh->bed_ny = h->bed_ny;
+        nh->bed_xo = h->bed_xo; nh->bed_yo = h->bed_yo; nh->bed_dx = h->bed_dx;
+        std::swap(*h, *nh);  // h takes the new mesh; nh (the old one) is freed
+        bgf_destroy(nh);
+        g_err.clear();
+        return BGF_OK;
+    }
+    // in place: the new mesh goes to the staging arrays, the new state to the
+    // scratch buffers (buffer 1 of h, u, v; two of the stage arrays for the bed
+    // and the roughness), then the pointers are exchanged
+    const int cap = h->cap;
+    bool ok = true;
+    for (int k = 0; k < 11; k++)
+        if (!h->d_mesh2[k]) ok = ok && hipMalloc((void**)&h->d_mesh2[k], sizeof(int) * cap) == hipSuccess;
+    if (!h->d_remap) ok = ok && hipMalloc((void**)&h->d_remap, sizeof(int) * 5 * (size_t)cap) == hipSuccess;
+    if (!h->d_bxo) ok = ok && hipMalloc((void**)&h->d_bxo, sizeof(double) * cap) == hipSuccess;
+    if (!h->d_byo) ok = ok && hipMalloc((void**)&h->d_byo, sizeof(double) * cap) == hipSuccess;
+    if (!ok) return fail(BGF_ERR_OOM, "remesh: staging allocation failed");
+    const int* src[10] = {b->level, b->LeftBot, b->LeftTop, b->RightBot, b->RightTop,
+                          b->TopLeft, b->TopRight, b->BotLeft, b->BotRight, mi.inv.data()};
+    ok = hipMemcpy(h->d_mesh2[0], b->active, sizeof(int) * q.nblk, hipMemcpyHostToDevice) == hipSuccess;
+    for (int k = 0; k < 10; k++)
+        ok = ok && hipMemcpy(h->d_mesh2[1 + k], src[k], sizeof(int) * nbm, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(h->d_remap, remap, sizeof(int) * 5 * (size_t)nbm, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(h->d_bxo, b->xo, sizeof(double) * nbm, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(h->d_byo, b->yo, sizeof(double) * nbm, hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) return fail(BGF_ERR_HIP, "remesh: upload failed");
     bgf::launch_amr_halo(prec, h->kp, h->d_h[0], h->d_u[0], h->d_v[0], h->d_zb, h->stream);
     bgf::RemeshLaunch R;
     R.nblk = q.nblk;
     R.h = h->d_h[0]; R.u = h->d_u[0]; R.v = h->d_v[0]; R.zb = h->d_zb; R.cf = h->d_cf;
-    R.active = nh->d_active; R.level = nh->d_level; R.xo = d_xo; R.yo = d_yo; R.remap = d_remap;
-    R.nh = nh->d_h[0]; R.nu = nh->d_u[0]; R.nv = nh->d_v[0]; R.nzb = nh->d_zb; R.ncf = nh->d_cf;
+    R.active = h->d_mesh2[0]; R.level = h->d_mesh2[1]; R.xo = h->d_bxo; R.yo = h->d_byo; R.remap = h->d_remap;
+    R.nh = h->d_h[1]; R.nu = h->d_u[1]; R.nv = h->d_v[1]; R.nzb = h->d_amr[0]; R.ncf = h->d_amr[1];
     R.bed = h->d_bed; R.bnx = h->bed_nx; R.bny = h->bed_ny; R.bxo = h->bed_xo; R.byo = h->bed_yo; R.bdx = h->bed_dx;
     R.pxo = q.xo; R.pyo = q.yo; R.dx = q.dx; R.eps = q.eps;
     bgf::launch_remesh(prec, h->kp, R, h->stream);
+    HIPCHK(hipGetLastError());
+    std::swap(h->d_h[0], h->d_h[1]);
+    std::swap(h->d_u[0], h->d_u[1]);
+    std::swap(h->d_v[0], h->d_v[1]);
+    std::swap(h->d_zb, h->d_amr[0]);
+    std::swap(h->d_cf, h->d_amr[1]);
+    std::swap(h->d_active, h->d_mesh2[0]);
+    std::swap(h->d_level, h->d_mesh2[1]);
+    for (int k = 0; k < 8; k++) std::swap(h->d_nb[k], h->d_mesh2[2 + k]);
+    std::swap(h->d_inv, h->d_mesh2[10]);
+    h->p.nblk = q.nblk;
+    h->p.nblkmem = nbm;
+    h->nblk = q.nblk;
+    h->nblkmem = nbm;
+    h->S = (size_t)nbm * BS;
+    h->active.assign(b->active, b->active + q.nblk);
+    h->xy = block_xy(b, q.nblk, q.dx);
+    bgf::KParams& kp = h->kp;
+    kp.nblk = q.nblk;
+    kp.minlevel = mi.minlev;
+    kp.maxlevel = mi.maxlev;
+    kp.delta = mi.minlev < 0 ? q.delta * (1 << -mi.minlev) : q.delta / (1 << mi.minlev);
+    kp.active = h->d_active;
+    kp.level = h->d_level;
+    kp.LeftBot = h->d_nb[0]; kp.LeftTop = h->d_nb[1]; kp.RightBot = h->d_nb[2]; kp.RightTop = h->d_nb[3];
+    kp.TopLeft = h->d_nb[4]; kp.TopRight = h->d_nb[5]; kp.BotLeft = h->d_nb[6]; kp.BotRight = h->d_nb[7];
+    kp.inv = h->d_inv;
+    std::vector<int> order = processing_order(h->xy, h->tile_x, h->tile_y);
+    ok = hipMemcpyAsync(h->d_order, order.data(), sizeof(int) * q.nblk, hipMemcpyHostToDevice, h->stream) == hipSuccess;
+    if (q.spherical) ok = ok && upload_metric(h, &q, b, nbm);
+    if (!ok) return fail(BGF_ERR_HIP, "remesh: upload failed");
+    kp.grid_pred = bgf::persistent_grid(h->prec, 0, q.nblk, h->wg_pred);
+    kp.grid_corr = bgf::persistent_grid(h->prec, 1, q.nblk, h->wg_corr);
+    // the bed's halo on the new mesh (src/InitialConditions.cu:135-147)
+    bgf::launch_amr_fill(prec, kp, h->d_zb, h->stream);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    if (e != hipSuccess) return cleanup(fail(BGF_ERR_HIP, std::string("remesh: ") + hipGetErrorString(e)));
-    // the bed's halo on the new mesh (src/InitialConditions.cu:135-147)
-    bgf::launch_amr_fill(prec, nh->kp, nh->d_zb, nh->stream);
-    e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(nh->stream);
-    if (e != hipSuccess) return cleanup(fail(BGF_ERR_HIP, std::string("remesh: ") + hipGetErrorString(e)));
-    nh->state_set = true;
-    nh->timing = h->timing;
-    nh->use_graph = h->use_graph;
-    std::swap(nh->d_bed, h->d_bed);
-    nh->bed_nx = h->bed_nx; nh->bed_ny = h->bed_ny;
-    nh->bed_xo = h->bed_xo; nh->bed_yo = h->bed_yo; nh->bed_dx = h->bed_dx;
-    std::swap(*h, *nh);  // h takes the new mesh; nh (the old one) is freed
+    if (e != hipSuccess) return fail(BGF_ERR_HIP, std::string("remesh: ") + hipGetErrorString(e));
+    if (h->graph) {
+        (void)hipGraphExecDestroy(h->graph);
+        h->graph = nullptr;
+    }
     g_err.clear();
-    return cleanup(BGF_OK);
+    return BGF_OK;
 }
 
 bgf_status bgf_last_run_steps(bgf_handle h, int* nsteps) {

@@ -8,6 +8,10 @@ constexpr int NSLOT = 64;  // block-minimum dt slots (atomic min, spread to avoi
 
 // Device-resident Loop<T> time keeping (src/Arrays.h:358-366), run flags and
 // the dt reduction slots (bit patterns of positive floats; +inf when empty).
+struct DevLoop;
+// Sets a loop state's rain rate and its per-second depth rates.
+inline void set_rain(DevLoop& L, double rain_mmh);
+
 struct DevLoop {
     double nextoutputtime;
     double dt;
@@ -15,6 +19,8 @@ struct DevLoop {
     double totaltime;
     double endtime;  // the run stops once totaltime reaches it
     double rain;     // uniform rain rate, mm/h (Rain.nowvalue)
+    double rate64;   // T(rain) / T(1000) / T(3600) in FP64 and FP32: the depth rate in m/s
+    float rate32;    // (src/Updateforcing.cu:559), formed once per loop instead of per cell
     int done;        // totaltime reached the end time: later steps are no-ops
     int bad;         // a non-finite or non-positive dt was produced
     int steps_done;  // steps executed since the loop was uploaded
@@ -24,6 +30,15 @@ struct DevLoop {
     unsigned int slot32[NSLOT];
     unsigned long long slot64[NSLOT];
 };
+
+inline void set_rain(DevLoop& L, double rain_mmh) {
+    L.rain = rain_mmh;
+    L.rate64 = rain_mmh / 1000.0 / 3600.0;
+    volatile float r = (float)rain_mmh;  // each division rounded to float, as on the device
+    r = r / 1000.0f;
+    r = r / 3600.0f;
+    L.rate32 = r;
+}
 
 // Scalar parameters and block neighbour arrays of one domain.
 struct KParams {

@@ -671,7 +671,8 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
         // implicit uniform rain after cleanup, src/Updateforcing.cu:530-581
         if (P.rain_on) {
             T act = (double)zb0 < C.mask_d ? T(1.0) : T(0.0);
-            T rainhh = dmax(T(Lm->rain) / T(1000.0) / T(3600.0) * dt, T(0.0)) * act;
+            const T rate = sizeof(T) == 8 ? T(Lm->rate64) : T(Lm->rate32);  // T(rain) / T(1000) / T(3600)
+            T rainhh = dmax(rate * dt, T(0.0)) * act;
             T qvol = FAST ? h1 * Ar<T, FAST>::rcp(h1 + rainhh) : h1 / (h1 + rainhh);
             bool wetold = (double)h1 > C.eps_d;
             h1 = COMP ? kahan_add(h1, rainhh, comp) : h1 + rainhh;
