@@ -133,6 +133,9 @@ def lib():
         L.bgf_remesh.argtypes = [vp, ctypes.POINTER(BgfParam), ctypes.POINTER(BgfBlocks), ip]
         L.bgf_set_bed_source.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, vp]
+        L.bgf_set_rain_series.argtypes = [vp, ctypes.c_int, vp, vp]
+        L.bgf_set_rain_map.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double, vp]
         L.bgf_set_schedule.argtypes = [vp] + [ctypes.c_int] * 4
         L.bgf_get_schedule.argtypes = [vp] + [ip] * 6
         L.bgf_batch_set_schedule.argtypes = [vp] + [ctypes.c_int] * 4
@@ -147,7 +150,7 @@ def lib():
                      "bgf_batch_sample_gauges", "bgf_batch_stats", "bgf_batch_get_gauges",
                      "bgf_batch_track_maxdepth", "bgf_set_schedule", "bgf_get_schedule",
                      "bgf_batch_set_schedule", "bgf_batch_get_schedule", "bgf_adapt_mesh", "bgf_remesh",
-                     "bgf_set_bed_source"):
+                     "bgf_set_bed_source", "bgf_set_rain_series", "bgf_set_rain_map"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -378,6 +381,28 @@ class Domain:
         f = np.zeros(self.nblkmem, np.uint8)
         _check(lib().bgf_refine_flags(self.h, field.encode(), int(mode), float(a), float(b), f.ctypes.data))
         return f
+
+    def set_rain_series(self, times, values):
+        """Uniform rain rate [mm/h] as a time series (bgf_set_rain_series); empty removes it."""
+        t = np.ascontiguousarray(times, np.float64)
+        v = np.ascontiguousarray(values, np.float64)
+        if t.shape != v.shape:
+            raise BgfError("times and values differ in length")
+        _check(lib().bgf_set_rain_series(self.h, t.size, t.ctypes.data if t.size else None,
+                                         v.ctypes.data if v.size else None))
+
+    def set_rain_map(self, frames, xo, yo, dx, t0=0.0, dt=0.0):
+        """Rain on a grid (bgf_set_rain_map): frames (nframes, ny, nx) [mm/h] at t0 + k*dt, or one
+        (ny, nx) map; None removes it."""
+        if frames is None:
+            _check(lib().bgf_set_rain_map(self.h, 0, 0, 0.0, 0.0, 0.0, 0, 0.0, 0.0, None))
+            return
+        f = np.ascontiguousarray(frames, np.float32)
+        if f.ndim == 2:
+            f = f[None]
+        nf, ny, nx = f.shape
+        _check(lib().bgf_set_rain_map(self.h, nx, ny, float(xo), float(yo), float(dx), nf, float(t0), float(dt),
+                                      f.ctypes.data))
 
     def set_bed_source(self, values, xo, yo, dx):
         """Bathymetry raster (ny, nx) that remesh samples for new cells; None removes it."""

@@ -712,8 +712,7 @@ __global__ __launch_bounds__(256) void k_amr_advance(KParams P, AdvArgs<T> A, Fl
         // cleanup then implicit uniform rain, src/Updateforcing.cu:530-581
         if (P.rain_on) {
             const T act = (double)A.zb[i] < P.mask ? T(1.0) : T(0.0);
-            const T rate = sizeof(T) == 8 ? T(L->rate64) : T(L->rate32);  // T(rain) / T(1000) / T(3600)
-            T rainhh = smax(rate * T(L->dt), T(0.0)) * act;
+            T rainhh = rain_depth<T>(P, L, ib, x, y, T(L->dt)) * act;
             const T qvol = ho / (ho + rainhh);
             const bool wet = (double)ho > P.eps;
             ho = ho + rainhh;
@@ -755,6 +754,7 @@ void stage(const KParams& P, const AmrBuffers<T>& B, DevLoop* L, int corr, hipSt
 template <class T>
 void amr_step(const KParams& P, const AmrBuffers<T>& B, DevLoop* L, int advance_time, hipStream_t st, hipEvent_t* ev) {
     const dim3 blk(P.nblk), tpb(256);
+    launch_rain_now(P, L, st);
     if (ev) (void)hipEventRecord(ev[0], st);
     stage(P, B, L, 0, st);
     if (ev) (void)hipEventRecord(ev[1], st);

@@ -64,6 +64,11 @@ struct bgf_handle_s {
     int* d_remap = nullptr;
     double *d_bxo = nullptr, *d_byo = nullptr;
     unsigned char* d_flags = nullptr;
+    // block origins (double, cap) for cell positions; rain forcing
+    double *d_blkxo = nullptr, *d_blkyo = nullptr;
+    double *d_rain_t = nullptr, *d_rain_v = nullptr;
+    int rain_n = 0;
+    float *d_rframes = nullptr, *d_rnow = nullptr;
     // bed source raster sampled by bgf_remesh for new cells
     float* d_bed = nullptr;
     int bed_nx = 0, bed_ny = 0;
@@ -83,6 +88,7 @@ void free_handle(bgf_handle h) {
     F(h->d_ch);
     F(h->d_metric);
     F(h->d_bed);
+    F(h->d_blkxo); F(h->d_blkyo); F(h->d_rain_t); F(h->d_rain_v); F(h->d_rframes); F(h->d_rnow);
     for (int k = 0; k < 11; k++) F(h->d_mesh2[k]);
     F(h->d_remap); F(h->d_bxo); F(h->d_byo); F(h->d_flags);
     for (int k = 0; k < 17; k++) F(h->d_amr[k]);
@@ -101,6 +107,9 @@ bgf_status upload_loop(bgf_handle h, const bgf_loop* loop, double endtime) {
     L.totaltime = loop->totaltime;
     L.endtime = endtime;
     bgf::set_rain(L, (double)(float)h->p.rain);  // Rain.nowvalue is a float (src/Forcing.h)
+    L.rain_t = h->d_rain_t;
+    L.rain_v = h->d_rain_v;
+    L.rain_n = h->rain_n;
     L.done = loop->totaltime >= endtime ? 1 : 0;
     for (int k = 0; k < bgf::NSLOT; k++) {
         L.slot32[k] = 0x7f800000u;
@@ -342,6 +351,7 @@ bgf_status create_handle(const bgf_param* p, const bgf_blocks* b, int cap, bgf_h
     ok = ok && A(&h->d_dh, h->es * SA) && A(&h->d_dhu, h->es * SA) && A(&h->d_dhv, h->es * SA);
     ok = ok && A((void**)&h->d_volpart, sizeof(double) * cap) && A((void**)&h->d_volout, sizeof(double));
     ok = ok && A((void**)&h->d_loop, sizeof(bgf::DevLoop));
+    ok = ok && A((void**)&h->d_blkxo, sizeof(double) * cap) && A((void**)&h->d_blkyo, sizeof(double) * cap);
     h->amr = multilevel;
     if (p->precision == BGF_FP32_COMP && !multilevel) ok = ok && A(&h->d_ch, h->es * SA);
     if (p->spherical) ok = ok && A(&h->d_metric, h->es * (size_t)cap * bgf::METRIC_STRIDE);
@@ -360,6 +370,8 @@ bgf_status create_handle(const bgf_param* p, const bgf_blocks* b, int cap, bgf_h
     bool cp = hipMemcpy(h->d_active, b->active, sizeof(int) * nblk, hipMemcpyHostToDevice) == hipSuccess;
     cp = cp && hipMemcpy(h->d_order, order.data(), sizeof(int) * nblk, hipMemcpyHostToDevice) == hipSuccess;
     cp = cp && hipMemcpy(h->d_level, b->level, sizeof(int) * nbm, hipMemcpyHostToDevice) == hipSuccess;
+    cp = cp && hipMemcpy(h->d_blkxo, b->xo, sizeof(double) * nbm, hipMemcpyHostToDevice) == hipSuccess;
+    cp = cp && hipMemcpy(h->d_blkyo, b->yo, sizeof(double) * nbm, hipMemcpyHostToDevice) == hipSuccess;
     for (int k = 0; k < 8; k++) cp = cp && hipMemcpy(h->d_nb[k], nbs[k], sizeof(int) * nbm, hipMemcpyHostToDevice) == hipSuccess;
     if (p->spherical) cp = cp && upload_metric(h, p, b, nbm);
     if (multilevel) {
@@ -410,6 +422,11 @@ bgf_status create_handle(const bgf_param* p, const bgf_blocks* b, int cap, bgf_h
     kp.inv = h->d_inv;
     kp.metric = h->d_metric;
     kp.spherical = p->spherical;
+    kp.xo = p->xo;
+    kp.yo = p->yo;
+    kp.dx = p->dx;
+    kp.bxo = h->d_blkxo;
+    kp.byo = h->d_blkyo;
     *out = h;
     g_err.clear();
     return BGF_OK;
@@ -582,6 +599,64 @@ bgf_status bgf_refine_flags(bgf_handle h, const char* field, int mode, double a,
     return BGF_OK;
 }
 
+bgf_status bgf_set_rain_series(bgf_handle h, int n, const double* times, const double* values) {
+    if (!h) return fail(BGF_ERR_INVALID_ARG, "null handle");
+    if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
+    if (n != 0 && (n < 2 || !times || !values)) return fail(BGF_ERR_INVALID_ARG, "a rain series needs n >= 2 pairs");
+    for (int k = 1; k < n; k++)
+        if (!(times[k] > times[k - 1])) return fail(BGF_ERR_INVALID_ARG, "rain series times must increase");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    (void)hipFree(h->d_rain_t);
+    (void)hipFree(h->d_rain_v);
+    h->d_rain_t = h->d_rain_v = nullptr;
+    h->rain_n = 0;
+    if (n > 0) {
+        HIPCHK(hipMalloc((void**)&h->d_rain_t, sizeof(double) * n));
+        HIPCHK(hipMalloc((void**)&h->d_rain_v, sizeof(double) * n));
+        HIPCHK(hipMemcpy(h->d_rain_t, times, sizeof(double) * n, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->d_rain_v, values, sizeof(double) * n, hipMemcpyHostToDevice));
+        h->rain_n = n;
+    }
+    h->kp.rain_on = h->p.rain_on || h->rain_n > 0 || h->kp.rframes ? 1 : 0;
+    if (h->graph) {
+        (void)hipGraphExecDestroy(h->graph);
+        h->graph = nullptr;
+    }
+    return BGF_OK;
+}
+
+bgf_status bgf_set_rain_map(bgf_handle h, int nx, int ny, double xo, double yo, double dx, int nframes, double t0,
+                            double dt, const float* frames) {
+    if (!h) return fail(BGF_ERR_INVALID_ARG, "null handle");
+    if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
+    if (nframes != 0 && (nx < 2 || ny < 2 || !(dx > 0) || !frames || nframes < 0 || (nframes > 1 && !(dt > 0))))
+        return fail(BGF_ERR_INVALID_ARG, "a rain map needs nx, ny >= 2, dx > 0 and dt > 0 between frames");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    (void)hipFree(h->d_rframes);
+    (void)hipFree(h->d_rnow);
+    h->d_rframes = h->d_rnow = nullptr;
+    bgf::KParams& kp = h->kp;
+    kp.rframes = nullptr;
+    kp.rnow = nullptr;
+    kp.rnframes = 0;
+    if (nframes > 0) {
+        const size_t n = (size_t)nx * ny;
+        HIPCHK(hipMalloc((void**)&h->d_rframes, sizeof(float) * n * nframes));
+        HIPCHK(hipMemcpy(h->d_rframes, frames, sizeof(float) * n * nframes, hipMemcpyHostToDevice));
+        if (nframes > 1) HIPCHK(hipMalloc((void**)&h->d_rnow, sizeof(float) * n));
+        kp.rframes = h->d_rframes;
+        kp.rnow = nframes > 1 ? h->d_rnow : h->d_rframes;
+        kp.rnx = nx; kp.rny = ny; kp.rnframes = nframes;
+        kp.rxo = xo; kp.ryo = yo; kp.rdx = dx; kp.rt0 = t0; kp.rdtf = dt;
+    }
+    kp.rain_on = h->p.rain_on || h->rain_n > 0 || kp.rframes ? 1 : 0;
+    if (h->graph) {
+        (void)hipGraphExecDestroy(h->graph);
+        h->graph = nullptr;
+    }
+    return BGF_OK;
+}
+
 bgf_status bgf_set_bed_source(bgf_handle h, int nx, int ny, double xo, double yo, double dx, const float* values) {
     if (!h) return fail(BGF_ERR_INVALID_ARG, "null handle");
     if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
@@ -673,6 +748,18 @@ bgf_status bgf_remesh(bgf_handle h, const bgf_param* p, const bgf_blocks* b, con
         std::swap(nh->d_bed, h->d_bed);
         nh->bed_nx = h->bed_nx; nh->bed_ny = h->bed_ny;
         nh->bed_xo = h->bed_xo; nh->bed_yo = h->bed_yo; nh->bed_dx = h->bed_dx;
+        std::swap(nh->d_rain_t, h->d_rain_t);
+        std::swap(nh->d_rain_v, h->d_rain_v);
+        std::swap(nh->d_rframes, h->d_rframes);
+        std::swap(nh->d_rnow, h->d_rnow);
+        nh->rain_n = h->rain_n;
+        const bgf::KParams& ok0 = h->kp;
+        nh->kp.rain_on = ok0.rain_on;
+        nh->kp.rframes = nh->d_rframes;
+        nh->kp.rnow = ok0.rnow == ok0.rframes ? nh->d_rframes : nh->d_rnow;
+        nh->kp.rnx = ok0.rnx; nh->kp.rny = ok0.rny; nh->kp.rnframes = ok0.rnframes;
+        nh->kp.rxo = ok0.rxo; nh->kp.ryo = ok0.ryo; nh->kp.rdx = ok0.rdx; nh->kp.rt0 = ok0.rt0; nh->kp.rdtf = ok0.rdtf;
+        nh->p.rain_on = h->p.rain_on;
         std::swap(*h, *nh);  // h takes the new mesh; nh (the old one) is freed
         bgf_destroy(nh);
         g_err.clear();
@@ -717,6 +804,8 @@ bgf_status bgf_remesh(bgf_handle h, const bgf_param* p, const bgf_blocks* b, con
     std::swap(h->d_level, h->d_mesh2[1]);
     for (int k = 0; k < 8; k++) std::swap(h->d_nb[k], h->d_mesh2[2 + k]);
     std::swap(h->d_inv, h->d_mesh2[10]);
+    std::swap(h->d_blkxo, h->d_bxo);
+    std::swap(h->d_blkyo, h->d_byo);
     h->p.nblk = q.nblk;
     h->p.nblkmem = nbm;
     h->nblk = q.nblk;
@@ -734,6 +823,8 @@ bgf_status bgf_remesh(bgf_handle h, const bgf_param* p, const bgf_blocks* b, con
     kp.LeftBot = h->d_nb[0]; kp.LeftTop = h->d_nb[1]; kp.RightBot = h->d_nb[2]; kp.RightTop = h->d_nb[3];
     kp.TopLeft = h->d_nb[4]; kp.TopRight = h->d_nb[5]; kp.BotLeft = h->d_nb[6]; kp.BotRight = h->d_nb[7];
     kp.inv = h->d_inv;
+    kp.bxo = h->d_blkxo;
+    kp.byo = h->d_blkyo;
     std::vector<int> order = processing_order(h->xy, h->tile_x, h->tile_y);
     ok = hipMemcpyAsync(h->d_order, order.data(), sizeof(int) * q.nblk, hipMemcpyHostToDevice, h->stream) == hipSuccess;
     if (q.spherical) ok = ok && upload_metric(h, &q, b, nbm);

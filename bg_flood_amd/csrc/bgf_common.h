@@ -42,4 +42,38 @@ template <class T> __device__ __forceinline__ T calcres(T dx, int level) {
     return level < 0 ? dx * T(1 << -level) : dx / T(1 << level);
 }
 
+// Rain rate [mm/h] of the current rain map at cell (ix, iy) of block ib, as
+// AddrainforcingImplicitCPU samples it (src/Updateforcing.cu:549-558): the
+// cell position in T, then interp2BUQ (src/GridManip.cu:520-560) in double.
+template <class T> __device__ T rain_map_rate(const KParams& P, int ib, int ix, int iy) {
+    const T delta = calcres(T(P.dx), P.level[ib]);
+    const T x = T(P.xo) + T(P.bxo[ib]) + T(ix) * delta;
+    const T y = T(P.yo) + T(P.byo[ib]) + T(iy) * delta;
+    const double xmax = P.rxo + (P.rnx - 1) * P.rdx, ymax = P.ryo + (P.rny - 1) * P.rdx;
+    const double xi = fmax(fmin((double)x, xmax), P.rxo);
+    const double yi = fmax(fmin((double)y, ymax), P.ryo);
+    int cfi = (int)floor((xi - P.rxo) / P.rdx);
+    cfi = min(max(cfi, 0), P.rnx - 2);
+    int cfj = (int)floor((yi - P.ryo) / P.rdx);
+    cfj = min(max(cfj, 0), P.rny - 2);
+    const int cfip = cfi + 1, cfjp = cfj + 1;
+    const double x1 = P.rxo + P.rdx * cfi, x2 = P.rxo + P.rdx * cfip;
+    const double y1 = P.ryo + P.rdx * cfj, y2 = P.ryo + P.rdx * cfjp;
+    const double q11 = P.rnow[cfi + cfj * P.rnx], q12 = P.rnow[cfi + cfjp * P.rnx];
+    const double q21 = P.rnow[cfip + cfj * P.rnx], q22 = P.rnow[cfip + cfjp * P.rnx];
+    // BilinearInterpolation (src/Util_CPU.cu:129-144)
+    const double x2x1 = x2 - x1, y2y1 = y2 - y1, x2x = x2 - xi, y2y = y2 - yi, yy1 = yi - y1, xx1 = xi - x1;
+    return T(1.0 / (x2x1 * y2y1) * (q11 * x2x * y2y + q21 * xx1 * y2y + q12 * x2x * yy1 + q22 * xx1 * yy1));
+}
+
+// The per-step rain depth of a cell, m: max(rate / 1000 / 3600 * dt, 0)
+// (src/Updateforcing.cu:560) from the rain map, or from the loop's uniform
+// rate (precomputed as rate64 / rate32).
+template <class T> __device__ __forceinline__ T rain_depth(const KParams& P, const DevLoop* L, int ib, int ix, int iy, T dt) {
+    T a;
+    if (P.rnow) a = rain_map_rate<T>(P, ib, ix, iy) / T(1000.0) / T(3600.0) * dt;
+    else a = (sizeof(T) == 8 ? T(L->rate64) : T(L->rate32)) * dt;
+    return a < T(0.0) ? T(0.0) : a;  // std::max(a, 0)
+}
+
 }  // namespace bgf

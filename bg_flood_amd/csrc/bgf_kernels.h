@@ -27,6 +27,11 @@ struct DevLoop {
     int go;          // set by the dt reduction: the corrector half of this step runs
     int nstep;       // steps since the domain started (gauge sampling counts these)
     int gevery;      // gauge sampling period in steps (batch members)
+    // uniform rain as a time series (rain_n > 0): the rate of each step is
+    // interpolated at its start by the dt reduction, which sets rate64/rate32
+    const double* rain_t;
+    const double* rain_v;
+    int rain_n;
     unsigned int slot32[NSLOT];
     unsigned long long slot64[NSLOT];
 };
@@ -70,6 +75,16 @@ struct KParams {
     int spherical;
     // running maximum depth of the interior cells, updated by every corrector (optional)
     float* hmax;
+    // rain on a grid (optional): frames of rates [mm/h] (nx*ny each) at t0 + k*dtf,
+    // the map of the current step (written at its start), and what the cells
+    // need to sample it: Param.xo/yo/dx and the block origins
+    const float* rframes;
+    float* rnow;
+    int rnx, rny, rnframes;
+    double rxo, ryo, rdx, rt0, rdtf;
+    double xo, yo, dx;
+    const double* bxo;
+    const double* byo;
 };
 
 // Layout of a block's metric terms (spherical grids, src/Spherical.cu:15-64):
@@ -140,6 +155,8 @@ void launch_amr_step(int precision, const KParams& P, const void* buffers, DevLo
                      hipStream_t st, hipEvent_t* ev);
 // Halo of a static field (the bed) on a multi-level mesh, src/InitialConditions.cu:135-147.
 void launch_amr_fill(int precision, const KParams& P, void* z, hipStream_t st);
+// The rain map of the coming step from the rain frames (nothing without frames).
+void launch_rain_now(const KParams& P, const DevLoop* L, hipStream_t st);
 // The step's halo fill of h, u, v on a multi-level mesh (src/Halo.cu:586-616) by itself.
 void launch_amr_halo(int precision, const KParams& P, void* h, void* u, void* v, const void* zb, hipStream_t st);
 

@@ -671,8 +671,7 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
         // implicit uniform rain after cleanup, src/Updateforcing.cu:530-581
         if (P.rain_on) {
             T act = (double)zb0 < C.mask_d ? T(1.0) : T(0.0);
-            const T rate = sizeof(T) == 8 ? T(Lm->rate64) : T(Lm->rate32);  // T(rain) / T(1000) / T(3600)
-            T rainhh = dmax(rate * dt, T(0.0)) * act;
+            T rainhh = rain_depth<T>(P, Lm, ib, x, y, dt) * act;
             T qvol = FAST ? h1 * Ar<T, FAST>::rcp(h1 + rainhh) : h1 / (h1 + rainhh);
             bool wetold = (double)h1 > C.eps_d;
             h1 = COMP ? kahan_add(h1, rainhh, comp) : h1 + rainhh;
@@ -720,6 +719,18 @@ __global__ __launch_bounds__(64) void k_reduce_dt(DevLoop* L, int mode) {
         m = o < m ? o : m;
     }
     if (t == 0) {
+        if (L->rain_n > 0) {
+            // the uniform rain of this step, at its start (Forcingthisstep,
+            // src/Updateforcing.cu:45-60, interptime src/Util_CPU.cu:107-110)
+            const double tt = L->totaltime;
+            int k = 1;
+            while (k < L->rain_n - 1 && L->rain_t[k] - tt < 0.0) k++;
+            const double prev = L->rain_v[k - 1];
+            const double now = prev + (tt - L->rain_t[k - 1]) / (L->rain_t[k] - L->rain_t[k - 1]) * (L->rain_v[k] - prev);
+            L->rain = now;
+            L->rate64 = now / 1000.0 / 3600.0;
+            L->rate32 = (float)now / 1000.0f / 3600.0f;
+        }
         T dt = T(1.0) / epsi;
         dt = m < dt ? m : dt;
         if ((double)dt > 1.5 * L->dtmax) dt = T(1.5 * L->dtmax);
@@ -738,6 +749,24 @@ __global__ __launch_bounds__(64) void k_reduce_dt(DevLoop* L, int mode) {
             if (L->totaltime >= L->endtime) L->done = 1;
         }
     }
+}
+
+// The rain map of a step, at its start: InterpstepCPU (src/GridManip.cu:602-622)
+// between the frames that bracket totaltime (Forcingthisstep,
+// src/Updateforcing.cu:61-127), in double, stored as float.
+__global__ __launch_bounds__(256) void k_rain_now(KParams P, const DevLoop* L) {
+    const int n = P.rnx * P.rny;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n || L->done) return;
+    const double tt = L->totaltime;
+    const int k = min(max((int)floor((tt - P.rt0) / P.rdtf), 0), P.rnframes - 2);
+    const float uo = P.rframes[(size_t)k * n + i], un = P.rframes[(size_t)(k + 1) * n + i];
+    P.rnow[i] = (float)((double)uo + (tt - P.rdtf * k) * (double)(un - uo) / P.rdtf);
+}
+
+void launch_rain_now(const KParams& P, const DevLoop* L, hipStream_t st) {
+    if (!P.rframes || P.rnframes < 2) return;
+    hipLaunchKernelGGL(k_rain_now, dim3((P.rnx * P.rny + 255) / 256), dim3(256), 0, st, P, L);
 }
 
 // Per-cell initial step delta/sqrt(g*max(h,eps)) and its block minimum,
@@ -852,6 +881,7 @@ __global__ __launch_bounds__(256) void k_settle(KParams P, const T* __restrict__
 template <class T, bool FAST, bool BATCH, bool COMP>
 void launch_step_t(const KParams& P, const StepBuffers<T>& B, DevLoop* L, int advance_time, hipStream_t st,
                    hipEvent_t* ev) {
+    if (!BATCH) launch_rain_now(P, L, st);
     if (ev) (void)hipEventRecord(ev[0], st);
     const Cst<T> C = make_cst<T, FAST>(P);
     hipLaunchKernelGGL((k_predictor<T, FAST, BATCH, COMP>), dim3(P.grid_pred), dim3(256), 0, st, P, C, B, L);

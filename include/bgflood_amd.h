@@ -248,6 +248,25 @@ bgf_status bgf_adapt_mesh(const bgf_param* p, const bgf_blocks* b, const unsigne
  * next bgf_step or bgf_run continues on the new mesh. */
 bgf_status bgf_remesh(bgf_handle h, const bgf_param* p, const bgf_blocks* b, const int* remap);
 
+/* Rain forcing beyond the constant Param rate (the Rain forcing of
+ * src/Forcing.h:28-73, evaluated at the start of every step as updateforcing
+ * does, src/Updateforcing.cu:9-127, then applied by the implicit rain of the
+ * corrector, :530-581):
+ *  - bgf_set_rain_series: a uniform rate as n >= 2 (time [s], rate [mm/h])
+ *    pairs with increasing times, interpolated linearly in time (interptime);
+ *    before the first and after the last pair the end segments extend;
+ *  - bgf_set_rain_map: nframes frames of nx * ny rates [mm/h], row-major from
+ *    (xo, yo) with spacing dx, at times t0 + k * dt; the map of a step is the
+ *    linear interpolation between the frames that bracket its start
+ *    (InterpstepCPU, src/GridManip.cu:602-622), sampled bilinearly at each
+ *    cell centre (interp2BUQ).  One frame is a static map.
+ * A map takes precedence over a series, a series over the constant rate.
+ * n = 0 / nframes = 0 removes the forcing.  Handles only (batch members keep
+ * their constant rates). */
+bgf_status bgf_set_rain_series(bgf_handle h, int n, const double* times, const double* values);
+bgf_status bgf_set_rain_map(bgf_handle h, int nx, int ny, double xo, double yo, double dx, int nframes, double t0,
+                            double dt, const float* frames);
+
 /* Bathymetry raster that bgf_remesh samples for new cells, as the reference
  * re-interpolates its bathymetry forcing after adaptation
  * (StaticForcingP<float>, src/Forcing.h): nx * ny values, row-major from

@@ -70,6 +70,11 @@ template <class T> int ftoi(T value) {
 
 // src/Util_CPU.cu:29-46 (utils::sq, utils::max, utils::min)
 template <class T> inline T sq(T a) { return a * a; }
+
+// src/Util_CPU.cu:107-110
+inline double interptime(double next, double prev, double timenext, double time) {
+    return prev + (time) / (timenext) * (next - prev);
+}
 template <class T> inline T umax(T a, T b) { return (a < b) ? b : a; }
 template <class T> inline T umin(T a, T b) { return !(b < a) ? a : b; }
 
@@ -165,7 +170,19 @@ template <class T> struct Ev { std::vector<T> zs, h, u, v; };
 
 struct Side { std::vector<int> blk; int isright, istop; };
 
+// Rain forcing beyond a constant rate: a uniform time series (forcingmap with
+// uniform = 1) or a map with frames at t0 + k*dt (src/Forcing.h:28-73).
+struct RainForcing {
+    std::vector<double> t, v;      // uniform series: time [s], rate [mm/h]
+    int nx = 0, ny = 0, nframes = 0;
+    double xo = 0, yo = 0, dx = 0, t0 = 0, dtf = 0;
+    std::vector<float> frames;     // nframes * nx * ny
+    std::vector<float> now;        // the map of this step (DynForcingP::val)
+    double nowvalue = 0;           // the uniform rate of this step
+};
+
 template <class T> struct Mdl {
+    RainForcing rf;
     Prm p;
     Blk<T> b;
     Ev<T> ev, evo;
@@ -1242,6 +1259,46 @@ template <class T> void cleanupCPU(Mdl<T>& M, const Ev<T>& Eo, Ev<T>& E) {
 }
 
 // src/Updateforcing.cu:530-581 (uniform rain)
+// Forcingthisstep (src/Updateforcing.cu:45-127) for the rain: the uniform
+// rate by interptime between the bracketing series entries, or the map by
+// InterpstepCPU (src/GridManip.cu:602-622) between the frames bracketing
+// totaltime (its hdstep*hddt omits the first frame's time t0, kept here).
+void rain_this_step(RainForcing& R, double totaltime) {
+    if (!R.t.empty()) {
+        int k = 1;
+        while (k < (int)R.t.size() - 1 && R.t[k] - totaltime < 0.0) k++;  // the reference does not stop at the end
+        R.nowvalue = interptime(R.v[k], R.v[k - 1], R.t[k] - R.t[k - 1], totaltime - R.t[k - 1]);
+    }
+    if (R.nframes > 0) {
+        const int n = R.nx * R.ny;
+        R.now.resize(n);
+        if (R.nframes == 1) {
+            std::copy(R.frames.begin(), R.frames.end(), R.now.begin());
+            return;
+        }
+        int k = std::min(std::max((int)std::floor((totaltime - R.t0) / R.dtf), 0), R.nframes - 2);
+        const float* Uo = &R.frames[(size_t)k * n];
+        const float* Un = &R.frames[(size_t)(k + 1) * n];
+        for (int i = 0; i < n; i++) R.now[i] = float(Uo[i] + (totaltime - R.dtf * k) * (Un[i] - Uo[i]) / R.dtf);
+    }
+}
+
+// interp2BUQ(x, y, forcing) (src/GridManip.cu:520-560) on the rain map of this step.
+template <class T> T rain_map_at(const RainForcing& R, T x, T y) {
+    double xmax = R.xo + (R.nx - 1) * R.dx, ymax = R.yo + (R.ny - 1) * R.dx;
+    double xi = umax(umin(double(x), xmax), R.xo);
+    double yi = umax(umin(double(y), ymax), R.yo);
+    int cfi = std::min(std::max((int)std::floor((xi - R.xo) / R.dx), 0), R.nx - 2);
+    int cfip = cfi + 1;
+    double x1 = R.xo + R.dx * cfi, x2 = R.xo + R.dx * cfip;
+    int cfj = std::min(std::max((int)std::floor((yi - R.yo) / R.dx), 0), R.ny - 2);
+    int cfjp = cfj + 1;
+    double y1 = R.yo + R.dx * cfj, y2 = R.yo + R.dx * cfjp;
+    double q11 = R.now[cfi + cfj * R.nx], q12 = R.now[cfi + cfjp * R.nx];
+    double q21 = R.now[cfip + cfj * R.nx], q22 = R.now[cfip + cfjp * R.nx];
+    return T(BilinearInterpolation(q11, q12, q21, q22, x1, x2, y1, y2, xi, yi));
+}
+
 template <class T> void AddrainforcingImplicitCPU(Mdl<T>& M, const orc_loop& L, Ev<T>& E) {
     const Prm& P = M.p;
     for (int ibl = 0; ibl < P.nblk; ibl++) {
@@ -1250,7 +1307,17 @@ template <class T> void AddrainforcingImplicitCPU(Mdl<T>& M, const orc_loop& L, 
             for (int ix = 0; ix < BW; ix++) {
                 int i = memloc(ix, iy, ib);
                 T hi = E.h[i];
-                T Rainhh = T(P.rain);
+                T Rainhh;
+                if (M.rf.nframes > 0) {
+                    T delta = calcres(T(P.dx), M.b.level[ib]);
+                    T x = T(P.xo) + M.b.xo[ib] + ix * delta;
+                    T y = T(P.yo) + M.b.yo[ib] + iy * delta;
+                    Rainhh = rain_map_at(M.rf, x, y);
+                } else if (!M.rf.t.empty()) {
+                    Rainhh = T(M.rf.nowvalue);
+                } else {
+                    Rainhh = T(P.rain);
+                }
                 Rainhh = std::max(Rainhh / T(1000.0) / T(3600.0) * T(L.dt), T(0.0)) * M.activeCell[i];
                 T qvol = hi / (hi + Rainhh);
                 E.h[i] = hi + Rainhh;
@@ -1289,7 +1356,7 @@ template <class T> void FlowCPU(Mdl<T>& M, orc_loop& L) {
     AdvkernelCPU(M, dt, M.ev, M.evo);
     bottomfrictionCPU(M, dt, M.evo);
     cleanupCPU(M, M.evo, M.ev);
-    if (M.p.rain_on) AddrainforcingImplicitCPU(M, L, M.ev);
+    if (M.p.rain_on || !M.rf.t.empty() || M.rf.nframes > 0) AddrainforcingImplicitCPU(M, L, M.ev);
 }
 
 // src/Mainloop.cu:569-589
@@ -1545,8 +1612,24 @@ int orc_init_loop(orc_model m, orc_loop* loop) {
 }
 
 int orc_step(orc_model m, orc_loop* loop) {
-    if (m->m.dbl) FlowCPU(*m->m.d, *loop);
-    else FlowCPU(*m->m.f, *loop);
+    // updateforcing before the step (src/Mainloop.cu:41-45)
+    if (m->m.dbl) { rain_this_step(m->m.d->rf, loop->totaltime); FlowCPU(*m->m.d, *loop); }
+    else { rain_this_step(m->m.f->rf, loop->totaltime); FlowCPU(*m->m.f, *loop); }
+    return 0;
+}
+
+int orc_set_rain_series(orc_model m, int n, const double* t, const double* v) {
+    RainForcing& R = m->m.dbl ? m->m.d->rf : m->m.f->rf;
+    R.t.assign(t, t + n);
+    R.v.assign(v, v + n);
+    return 0;
+}
+
+int orc_set_rain_map(orc_model m, int nx, int ny, double xo, double yo, double dx, int nframes, double t0, double dtf,
+                     const float* frames) {
+    RainForcing& R = m->m.dbl ? m->m.d->rf : m->m.f->rf;
+    R.nx = nx; R.ny = ny; R.xo = xo; R.yo = yo; R.dx = dx; R.nframes = nframes; R.t0 = t0; R.dtf = dtf;
+    R.frames.assign(frames, frames + (size_t)nframes * nx * ny);
     return 0;
 }
 
@@ -1874,6 +1957,7 @@ template <class T> Mdl<T>* remesh_t(Mdl<T>& M, const orc_param* p, const orc_blo
         fillCorners(M.b, ib, M.zb.data());
     }
     Mdl<T>* N = create<T>(p, b);
+    N->rf = M.rf;
     const T eps = T(M.p.eps);
     const Ev<T>& E = M.ev;
     for (int ibl = 0; ibl < p->nblk; ibl++) {

@@ -66,6 +66,13 @@ int orc_step(orc_model m, orc_loop* loop);
 int orc_run(orc_model m, orc_loop* loop, int nsteps, double* seconds);
 double orc_volume(orc_model m);
 
+/* Rain forcing beyond the constant Param rate: a uniform series of n (time [s], rate [mm/h]) pairs,
+ * or a map of nframes frames (nx*ny rates each, row-major from (xo, yo), spacing dx) at times
+ * t0 + k*dtf; evaluated at the start of every step as the reference's updateforcing. */
+int orc_set_rain_series(orc_model m, int n, const double* t, const double* v);
+int orc_set_rain_map(orc_model m, int nx, int ny, double xo, double yo, double dx, int nframes, double t0, double dtf,
+                     const float* frames);
+
 /* Primitive restatements for unit tests. */
 double orc_minmod2_d(double theta, double s0, double s1, double s2);
 float  orc_minmod2_f(float theta, float s0, float s1, float s2);

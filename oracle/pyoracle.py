@@ -99,6 +99,12 @@ def lib():
         L.orc_remesh.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcParam), ctypes.POINTER(OrcBlocks), ip,
                                  ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_void_p]
+        L.orc_set_rain_series.restype = ctypes.c_int
+        L.orc_set_rain_series.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_set_rain_map.restype = ctypes.c_int
+        L.orc_set_rain_map.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_void_p]
         L.orc_calccm_d.restype = ctypes.c_double
         L.orc_calccm_d.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.orc_calcfm_d.restype = ctypes.c_double
@@ -222,6 +228,19 @@ class Oracle:
 
     def volume(self):
         return lib().orc_volume(self.h)
+
+    def set_rain_series(self, times, values):
+        t = np.ascontiguousarray(times, np.float64)
+        v = np.ascontiguousarray(values, np.float64)
+        lib().orc_set_rain_series(self.h, t.size, t.ctypes.data, v.ctypes.data)
+
+    def set_rain_map(self, frames, xo, yo, dx, t0=0.0, dt=0.0):
+        f = np.ascontiguousarray(frames, np.float32)
+        if f.ndim == 2:
+            f = f[None]
+        nf, ny, nx = f.shape
+        lib().orc_set_rain_map(self.h, nx, ny, float(xo), float(yo), float(dx), nf, float(t0), float(dt),
+                               f.ctypes.data)
 
     def remesh(self, p, blocks, remap, bed=None):
         """Moves the state onto the mesh (p, blocks) of adapt_mesh (orc_remesh); bed = (values (ny, nx),
