@@ -133,6 +133,7 @@ def lib():
         L.bgf_remesh.argtypes = [vp, ctypes.POINTER(BgfParam), ctypes.POINTER(BgfBlocks), ip]
         L.bgf_set_bed_source.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, vp]
+        L.bgf_set_boundary.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp]
         L.bgf_set_rain_series.argtypes = [vp, ctypes.c_int, vp, vp]
         L.bgf_set_rain_map.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double, vp]
@@ -150,7 +151,7 @@ def lib():
                      "bgf_batch_sample_gauges", "bgf_batch_stats", "bgf_batch_get_gauges",
                      "bgf_batch_track_maxdepth", "bgf_set_schedule", "bgf_get_schedule",
                      "bgf_batch_set_schedule", "bgf_batch_get_schedule", "bgf_adapt_mesh", "bgf_remesh",
-                     "bgf_set_bed_source", "bgf_set_rain_series", "bgf_set_rain_map"):
+                     "bgf_set_bed_source", "bgf_set_rain_series", "bgf_set_rain_map", "bgf_set_boundary"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -381,6 +382,19 @@ class Domain:
         f = np.zeros(self.nblkmem, np.uint8)
         _check(lib().bgf_refine_flags(self.h, field.encode(), int(mode), float(a), float(b), f.ctypes.data))
         return f
+
+    SIDES = {"left": 0, "right": 1, "bottom": 2, "bot": 2, "top": 3}
+
+    def set_boundary(self, side, btype, times=(), levels=()):
+        """Boundary type of one side (bgf_set_boundary): 0 wall, 1 Neumann, 2 Dirichlet, 3 absorbing;
+        types 2 and 3 take a water-level series (times [s], levels [m])."""
+        sd = self.SIDES[side] if isinstance(side, str) else int(side)
+        t = np.ascontiguousarray(times, np.float64)
+        z = np.ascontiguousarray(levels, np.float64)
+        if t.shape != z.shape:
+            raise BgfError("times and levels differ in length")
+        _check(lib().bgf_set_boundary(self.h, sd, int(btype), t.size, t.ctypes.data if t.size else None,
+                                      z.ctypes.data if z.size else None))
 
     def set_rain_series(self, times, values):
         """Uniform rain rate [mm/h] as a time series (bgf_set_rain_series); empty removes it."""

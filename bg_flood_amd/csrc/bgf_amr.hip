@@ -606,7 +606,8 @@ __device__ __forceinline__ int face_at(int s, int j, int b) {
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_amr_fluxfix(KParams P, FluxOut<T> F, DevLoop* L, int corr) {
+__global__ __launch_bounds__(256) void k_amr_fluxfix(KParams P, FluxOut<T> F, StageIn<T> E0, const T* zb, DevLoop* L,
+                                                     int corr) {
     if (corr ? !L->go : L->done) return;
     const int t = threadIdx.x;
     const int ib = quad_block(P, t);
@@ -624,8 +625,23 @@ __global__ __launch_bounds__(256) void k_amr_fluxfix(KParams P, FluxOut<T> F, De
     }
     // wall boundary (noslipbndQ, src/Boundary.cu:1887-1891): no mass flux through
     // the boundary face and no normal-momentum flux difference in the edge cell
-    if (P.aoibnd == 0 && (nb_first(P, s, ib) == ib || nb_second(P, s, ib) == ib)) {
+    const int ty = P.btype[s];
+    if ((ty == 0 || ty >= 2) && (nb_first(P, s, ib) == ib || nb_second(P, s, ib) == ib)) {
         const int i = face_at(s, j, ib);
+        if (ty >= 2) {
+            // types 2 and 3 (bndFluxGPUSideCPU): the face's mass flux from the
+            // step-start state (src/FlowCPU.cu:94, 187 pass evolv to both stages),
+            // at totaltime + dt/2 (previous dt) or totaltime + dt
+            const int in = s == SL ? memloc(1, j, ib) : s == SR ? memloc(BW - 1, j, ib) : s == SB ? memloc(j, 1, ib)
+                                                                                          : memloc(j, BW - 1, ib);
+            const double ts = corr ? L->tcorr : L->totaltime + L->dt * 0.5;
+            T q = A[0][i];
+            if (bnd_flux<T>(P, s, ty, bnd_level(P, s, ts), zb[i] + E0.h[i], E0.h[i], zb[in] + E0.h[in], E0.h[in],
+                            (s < 2 ? E0.u : E0.v)[in], q)) {
+                A[0][i] = q;
+                return;
+            }
+        }
         A[0][i] = T(0.0);
         if (s == SL) F.Su[memloc(1, j, ib)] = F.Fqux[i];
         else if (s == SR) F.Fqux[memloc(BW - 1, j, ib)] = F.Su[i];
@@ -748,7 +764,8 @@ void stage(const KParams& P, const AmrBuffers<T>& B, DevLoop* L, int corr, hipSt
     hipLaunchKernelGGL(k_amr_grad<T>, blk, tpb, 0, st, P, h, u, v, (const T*)B.zb, G, L, corr);
     const StageIn<T> E{h, u, v};
     hipLaunchKernelGGL(k_amr_flux<T>, blk, tpb, 0, st, P, E, (const T*)B.zb, G, F, L, corr);
-    hipLaunchKernelGGL(k_amr_fluxfix<T>, quad, tpb, 0, st, P, F, L, corr);
+    const StageIn<T> E0{B.h, B.u, B.v};
+    hipLaunchKernelGGL(k_amr_fluxfix<T>, quad, tpb, 0, st, P, F, E0, (const T*)B.zb, L, corr);
 }
 
 template <class T>

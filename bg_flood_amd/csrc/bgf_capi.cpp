@@ -71,6 +71,8 @@ struct bgf_handle_s {
     float *d_rframes = nullptr, *d_rnow = nullptr;
     // bed source raster sampled by bgf_remesh for new cells
     float* d_bed = nullptr;
+    // boundary water-level series of sides of type 2/3 (time, level)
+    double* d_bnd[4] = {nullptr, nullptr, nullptr, nullptr};
     int bed_nx = 0, bed_ny = 0;
     double bed_xo = 0, bed_yo = 0, bed_dx = 0;
 };
@@ -88,6 +90,7 @@ void free_handle(bgf_handle h) {
     F(h->d_ch);
     F(h->d_metric);
     F(h->d_bed);
+    for (int k = 0; k < 4; k++) F(h->d_bnd[k]);
     F(h->d_blkxo); F(h->d_blkyo); F(h->d_rain_t); F(h->d_rain_v); F(h->d_rframes); F(h->d_rnow);
     for (int k = 0; k < 11; k++) F(h->d_mesh2[k]);
     F(h->d_remap); F(h->d_bxo); F(h->d_byo); F(h->d_flags);
@@ -427,6 +430,7 @@ bgf_status create_handle(const bgf_param* p, const bgf_blocks* b, int cap, bgf_h
     kp.dx = p->dx;
     kp.bxo = h->d_blkxo;
     kp.byo = h->d_blkyo;
+    for (int k = 0; k < 4; k++) kp.btype[k] = p->aoibnd;
     *out = h;
     g_err.clear();
     return BGF_OK;
@@ -657,6 +661,37 @@ bgf_status bgf_set_rain_map(bgf_handle h, int nx, int ny, double xo, double yo, 
     return BGF_OK;
 }
 
+bgf_status bgf_set_boundary(bgf_handle h, int side, int type, int n, const double* times, const double* levels) {
+    if (!h) return fail(BGF_ERR_INVALID_ARG, "null handle");
+    if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
+    if (side < 0 || side > 3) return fail(BGF_ERR_INVALID_ARG, "side must be 0 (left), 1 (right), 2 (bottom) or 3 (top)");
+    if (type < 0 || type > 3) return fail(BGF_ERR_UNSUPPORTED, "boundary types 0 (wall) to 3 (absorbing) are provided");
+    if (type >= 2 && (n < 2 || !times || !levels))
+        return fail(BGF_ERR_INVALID_ARG, "types 2 and 3 need a water-level series of n >= 2 pairs");
+    for (int k = 1; type >= 2 && k < n; k++)
+        if (!(times[k] > times[k - 1])) return fail(BGF_ERR_INVALID_ARG, "boundary series times must increase");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    (void)hipFree(h->d_bnd[side]);
+    h->d_bnd[side] = nullptr;
+    bgf::KParams& kp = h->kp;
+    kp.bt[side] = kp.bz[side] = nullptr;
+    kp.bn[side] = 0;
+    if (type >= 2) {
+        HIPCHK(hipMalloc((void**)&h->d_bnd[side], sizeof(double) * 2 * n));
+        HIPCHK(hipMemcpy(h->d_bnd[side], times, sizeof(double) * n, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->d_bnd[side] + n, levels, sizeof(double) * n, hipMemcpyHostToDevice));
+        kp.bt[side] = h->d_bnd[side];
+        kp.bz[side] = h->d_bnd[side] + n;
+        kp.bn[side] = n;
+    }
+    kp.btype[side] = type;
+    if (h->graph) {
+        (void)hipGraphExecDestroy(h->graph);
+        h->graph = nullptr;
+    }
+    return BGF_OK;
+}
+
 bgf_status bgf_set_bed_source(bgf_handle h, int nx, int ny, double xo, double yo, double dx, const float* values) {
     if (!h) return fail(BGF_ERR_INVALID_ARG, "null handle");
     if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
@@ -760,6 +795,13 @@ bgf_status bgf_remesh(bgf_handle h, const bgf_param* p, const bgf_blocks* b, con
         nh->kp.rnx = ok0.rnx; nh->kp.rny = ok0.rny; nh->kp.rnframes = ok0.rnframes;
         nh->kp.rxo = ok0.rxo; nh->kp.ryo = ok0.ryo; nh->kp.rdx = ok0.rdx; nh->kp.rt0 = ok0.rt0; nh->kp.rdtf = ok0.rdtf;
         nh->p.rain_on = h->p.rain_on;
+        for (int k = 0; k < 4; k++) {
+            std::swap(nh->d_bnd[k], h->d_bnd[k]);
+            nh->kp.btype[k] = ok0.btype[k];
+            nh->kp.bn[k] = ok0.bn[k];
+            nh->kp.bt[k] = nh->d_bnd[k];
+            nh->kp.bz[k] = nh->d_bnd[k] ? nh->d_bnd[k] + ok0.bn[k] : nullptr;
+        }
         std::swap(*h, *nh);  // h takes the new mesh; nh (the old one) is freed
         bgf_destroy(nh);
         g_err.clear();

@@ -76,4 +76,49 @@ template <class T> __device__ __forceinline__ T rain_depth(const KParams& P, con
     return a < T(0.0) ? T(0.0) : a;  // std::max(a, 0)
 }
 
+// ------------------------------------------------------ boundary types 2, 3
+// Water level of side s at time t from its uniform series (FlowbndFlux,
+// src/Boundary.cu:116-145: the bracketing entries, interptime), passed on as
+// a float (the zsbnd argument of bndFluxGPUSideCPU).
+__device__ inline float bnd_level(const KParams& P, int s, double t) {
+    const double* tt = P.bt[s];
+    const double* zz = P.bz[s];
+    const int n = P.bn[s];
+    int k = 1;
+    while (k < n - 1 && tt[k] - t < 0.0) k++;  // the reference does not stop at the end
+    return (float)(zz[k - 1] + (t - tt[k - 1]) / (tt[k] - tt[k - 1]) * (zz[k] - zz[k - 1]));
+}
+
+// Mass flux through a boundary face of type 2 (Dirichlet1Q) or 3 (ABS1DQ)
+// from the step-start state at the face cell (zsi, hi) and the inside cell
+// (zsin, hin, and its normal velocity unin), bndFluxGPUSideCPU
+// (src/Boundary.cu:587-741, 1943-2032) without atmospheric pressure, taper
+// or zsoffset, and with the absorbing boundary's qmean filter at rest (it
+// moves only with a fixed Param.dt, and stays 0 otherwise).  Returns false
+// where the reference falls back to the no-slip patch (a dry face that the
+// boundary level does not flood).
+template <class T>
+__device__ bool bnd_flux(const KParams& P, int s, int type, float zsbnd, T zsi, T hi, T zsin, T hin, T unin, T& q) {
+    const T sign = (s == 1 || s == 3) ? T(1.0) : T(-1.0);
+    const T zsatm = T(0.0), taper = T(1.0);
+    const T zsX = T(((double)(T(zsbnd) + zsatm) - 0.5 * (double)(zsi + zsin)) * (double)taper +
+                    0.5 * (double)(zsi + zsin));
+    if (!((double)hi > P.eps || zsX > zsi)) return false;
+    const T g = T(P.g);
+    if (type == 3) {
+        const T qmean = hi < T(0.01) ? T(0.0) : T(0.0) * q + T(1.0) * (T(1.0) - T(0.0)) * T(0.0);
+        const T zn = zsX < zsi - hi ? zsi - hi : zsX;
+        const T hn = hi < T(0.0001) ? T(0.0001) : hi;
+        const T un = sign * sqrt(g / hn) * (zsi - zn);
+        q = un * hn + qmean;
+    } else {
+        const T zbin = zsin - hin;
+        const T a = hin < T(0.0) ? T(0.0) : hin;
+        const T b = zsX - zbin < T(0.0) ? T(0.0) : zsX - zbin;
+        const T un = sign * T(2.0) * (sqrt(g * a) - sqrt(g * b)) + unin;
+        q = un * hin;
+    }
+    return true;
+}
+
 }  // namespace bgf

@@ -32,6 +32,7 @@ struct DevLoop {
     const double* rain_t;
     const double* rain_v;
     int rain_n;
+    double tcorr;    // time of the corrector's boundary values: the step's start + dt (set by the dt reduction)
     unsigned int slot32[NSLOT];
     unsigned long long slot64[NSLOT];
 };
@@ -85,6 +86,13 @@ struct KParams {
     double xo, yo, dx;
     const double* bxo;
     const double* byo;
+    // boundary type of each side (0 left, 1 right, 2 bottom, 3 top): 0 wall,
+    // 1 Neumann, 2 Dirichlet water level, 3 absorbing; sides 2/3 take their
+    // water level from a uniform series (bn[s] pairs of time bt, level bz)
+    int btype[4];
+    const double* bt[4];
+    const double* bz[4];
+    int bn[4];
 };
 
 // Layout of a block's metric terms (spherical grids, src/Spherical.cu:15-64):

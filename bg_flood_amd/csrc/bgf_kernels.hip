@@ -239,6 +239,37 @@ __device__ __forceinline__ bool kurg_face(const Cst<T>& C, T hi, T hn, T zsi, T 
     return wet;
 }
 
+// Boundary of the block being computed: the type of each side at the domain
+// edge (-1 inside the domain), the stage's water level of sides of type 2/3,
+// and the step-start state those sides read.
+template <class T> struct Bnd {
+    int ty[4];
+    float zs[4];
+    const T *h, *u, *v, *zb;
+    int ib;
+    bool any;
+};
+
+template <class T>
+__device__ __forceinline__ Bnd<T> block_bnd(const KParams& P, int ib, int nL, int nR, int nB, int nT, double tstage,
+                                            const T* h, const T* u, const T* v, const T* zb) {
+    Bnd<T> b;
+    const int nb[4] = {nL, nR, nB, nT};
+    b.any = false;
+    b.h = h; b.u = u; b.v = v; b.zb = zb;
+    b.ib = ib;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        b.ty[s] = nb[s] == ib ? P.btype[s] : -1;
+        b.zs[s] = 0.0f;
+        if (b.ty[s] >= 2) {
+            b.any = true;
+            b.zs[s] = bnd_level(P, s, tstage);
+        }
+    }
+    return b;
+}
+
 template <class T> struct Tile {
     Q4<T> q[20 * TP];   // (h, zs, u, v) at x, y in [-2, 17], corners unused
     Q4<T> gx[16 * GXP]; // slopes along x of (h, zs, u, v) at x in [-1,16], y in [0,15]
@@ -364,8 +395,9 @@ __device__ __forceinline__ bool yface(Tile<T>& s, const Cst<T>& C, int x, int yf
 // dtmax: its X face then its Y face, a dry face storing 1/epsi outright
 // (src/Kurganov.cu:102-105, 159, 1488-1491, 1537).
 template <class T, bool FAST, bool WANT_DT, bool DEV>
-__device__ __forceinline__ T tile_fluxes(Tile<T>& s, const Cst<T>& C, int t, bool wl, bool wr, bool wb, bool wt,
+__device__ __forceinline__ T tile_fluxes(Tile<T>& s, const Cst<T>& C, const KParams& P, int t, const Bnd<T>& bd,
                                          T& dh, T& dhu, T& dhv) {
+    const bool wl = bd.ty[0] == 0, wr = bd.ty[1] == 0, wb = bd.ty[2] == 0, wt = bd.ty[3] == 0;
     tile_gradients<T, FAST, DEV>(s, C, t);
     __syncthreads();
     const int x = t & 15, y = t >> 4;
@@ -386,6 +418,29 @@ __device__ __forceinline__ T tile_fluxes(Tile<T>& s, const Cst<T>& C, int t, boo
     if (wr && x == BW - 1) { Fhr = T(0.0); Fql = Sr; }
     if (wb && y == 0) { Fhb = T(0.0); St = Fqb; }
     if (wt && y == BW - 1) { Fht = T(0.0); Fqb = St; }
+    if (bd.any) {
+        // boundary types 2 and 3 (bndFluxGPUSideCPU): the face's mass flux from
+        // the step-start state, or the no-slip patch on a dry face
+        auto open = [&](int sd, int ci, int cin, T& Fh) {
+            const T hi = bd.h[ci], hin = bd.h[cin];
+            T q = Fh;
+            const bool ok = bnd_flux<T>(P, sd, bd.ty[sd], bd.zs[sd], bd.zb[ci] + hi, hi, bd.zb[cin] + hin, hin,
+                                        (sd < 2 ? bd.u : bd.v)[cin], q);
+            if (ok) Fh = q;
+            return ok;
+        };
+        const int ib = bd.ib;
+        if (bd.ty[0] >= 2 && x == 0 && !open(0, memloc(0, y, ib), memloc(1, y, ib), Fhl)) { Fhl = T(0.0); Sr = Fql; }
+        if (bd.ty[1] >= 2 && x == BW - 1 && !open(1, memloc(BW - 1, y, ib), memloc(BW - 1, y, ib), Fhr)) {
+            Fhr = T(0.0);
+            Fql = Sr;
+        }
+        if (bd.ty[2] >= 2 && y == 0 && !open(2, memloc(x, 0, ib), memloc(x, 1, ib), Fhb)) { Fhb = T(0.0); St = Fqb; }
+        if (bd.ty[3] >= 2 && y == BW - 1 && !open(3, memloc(x, BW - 1, ib), memloc(x, BW - 1, ib), Fht)) {
+            Fht = T(0.0);
+            Fqb = St;
+        }
+    }
     // src/Advection.cu:236-250 (Cartesian: the metric terms vanish)
     dh = T(-1.0) * (Fhr - Fhl + Fht - Fhb) * C.cmdinv;
     dhu = (Fql + Fb.d - Sr - Ft.d) * C.cmdinv + C.fc * q.a * q.d;
@@ -491,7 +546,6 @@ __global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuff
     const int t = threadIdx.x;
     const int x = t & 15, y = t >> 4;
     const Sched S = schedule(blockIdx.x, gridDim.x, P.nblk);
-    const bool wall = P.aoibnd == 0;
     const int cs = strip_cell(t);
     T bmin = C.inv_epsi;
     int csid = -1;
@@ -535,8 +589,10 @@ __global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuff
             if (ldu(&L[n.sid].done, 0)) continue;
         }
         T dh, dhu, dhv;
-        T dtc = tile_fluxes<T, FAST, true, DEV>(s, C, t, wall && n.nL == n.ib, wall && n.nR == n.ib, wall && n.nB == n.ib,
-                                           wall && n.nT == n.ib, dh, dhu, dhv);
+        // boundary values at the predictor's time, totaltime + dt/2 with the previous dt (src/FlowCPU.cu:94)
+        const Bnd<T> bd = block_bnd<T>(P, n.ib, n.nL, n.nR, n.nB, n.nT, BATCH ? 0.0 : L->totaltime + L->dt * 0.5, B.h,
+                                       B.u, B.v, B.zb);
+        T dtc = tile_fluxes<T, FAST, true, DEV>(s, C, P, t, bd, dh, dhu, dhv);
         const int i = memloc(x, y, n.ib);
         B.dh[i] = dh;
         B.dhu[i] = dhu;
@@ -569,7 +625,6 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
     const int t = threadIdx.x;
     const int x = t & 15, y = t >> 4;
     const Sched S = schedule(blockIdx.x, gridDim.x, P.nblk);
-    const bool wall = P.aoibnd == 0;
     const int cs = strip_cell(t);
     const int fmc = P.ForceMassConserve;
     T ph = T(0), pu = T(0), pv = T(0), pdh = T(0), pdhu = T(0), pdhv = T(0), pz = T(0), pcf = T(0);
@@ -626,8 +681,9 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
         }
         const int ib = n.ib, nL = n.nL, nR = n.nR, nB = n.nB, nT = n.nT;
         T dh, dhu, dhv;
-        (void)tile_fluxes<T, FAST, false, COMP>(s, C, t, wall && nL == ib, wall && nR == ib, wall && nB == ib,
-                                          wall && nT == ib, dh, dhu, dhv);
+        // boundary values at totaltime + dt, from the step-start state (src/FlowCPU.cu:187)
+        const Bnd<T> bd = block_bnd<T>(P, ib, nL, nR, nB, nT, BATCH ? 0.0 : L->tcorr, B.h, B.u, B.v, B.zb);
+        (void)tile_fluxes<T, FAST, false, COMP>(s, C, P, t, bd, dh, dhu, dhv);
         T h1, u1, v1;
         const int i = memloc(x, y, ib);
         T comp = COMP ? B.ch[i] : T(0);
@@ -738,6 +794,7 @@ __global__ __launch_bounds__(64) void k_reduce_dt(DevLoop* L, int mode) {
         if (ceil(rem / (double)dt) > 0.0) dt = T(rem / ceil(rem / (double)dt));
         L->dt = (double)dt;
         L->dtmax = (double)dt;
+        L->tcorr = L->totaltime + L->dt;
         if (!isfinite((double)dt) || dt <= T(0)) L->bad = 1;
         if (mode != 2) {
             L->go = 1;

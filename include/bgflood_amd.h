@@ -248,6 +248,18 @@ bgf_status bgf_adapt_mesh(const bgf_param* p, const bgf_blocks* b, const unsigne
  * next bgf_step or bgf_run continues on the new mesh. */
 bgf_status bgf_remesh(bgf_handle h, const bgf_param* p, const bgf_blocks* b, const int* remap);
 
+/* Boundary segment of one side of the domain (0 left, 1 right, 2 bottom,
+ * 3 top), as the reference's left/right/bot/top boundary keys define one
+ * (bndsegment, src/Forcing.h:106-144): type 0 wall (noslipbndQ), 1 Neumann,
+ * 2 Dirichlet water level (Dirichlet1Q), 3 absorbing (ABS1DQ)
+ * (bndFluxGPUSideCPU, src/Boundary.cu:587-741, 1887-2032); types 2 and 3 take
+ * a uniform water-level series of n >= 2 (time [s], level [m]) pairs,
+ * interpolated at each stage's time (src/Boundary.cu:116-145).  Sides without
+ * a segment keep Param.aoibnd.  Not provided: spatially varying levels,
+ * velocities (type 4), tapering, zsoffset and the absorbing boundary's mean
+ * flux filter (inactive unless Param.dt is set). */
+bgf_status bgf_set_boundary(bgf_handle h, int side, int type, int n, const double* times, const double* levels);
+
 /* Rain forcing beyond the constant Param rate (the Rain forcing of
  * src/Forcing.h:28-73, evaluated at the start of every step as updateforcing
  * does, src/Updateforcing.cu:9-127, then applied by the implicit rain of the

@@ -193,6 +193,10 @@ template <class T> struct Mdl {
     std::vector<int> activeCell;
     int bndtype;                 // type of the remainder segment (aoibnd)
     Side left, right, top, bot;  // boundary block lists of the remainder segment
+    // per-side segments (0 left, 1 right, 2 bottom, 3 top): type (-1: the
+    // remainder's) and the uniform water-level series of types 2 and 3
+    int stype[4] = {-1, -1, -1, -1};
+    std::vector<double> sbt[4], sbz[4];
 };
 
 // ---------------------------------------------------------------- halo fill
@@ -1086,35 +1090,82 @@ int Inside(int isright, int istop, int ix, int iy, int ib) {
     return memloc(ix, iy - 1, ib);
 }
 
-// src/Boundary.cu:587-760 restricted to type 0 (wall, noslipbndQ :1887-1891).
-// Type 1 (Neumann) segments are skipped by FlowbndFlux (src/Boundary.cu:154).
-template <class T> void bndFluxSideWall(const Side& side, T* Fh, T* Fq, T* Ss) {
+// ABS1DQ (src/Boundary.cu:1943-1977)
+template <class T> void ABS1DQ(T g, T sign, T factime, T facrel, T zs, T zsbnd, T zsinside, T h, T& qmean, T& q, T& G,
+                               T& S) {
+    qmean = h < T(0.01) ? T(0.0) : factime * q + facrel * (T(1.0) - factime) * qmean;
+    T un;
+    T zn = umax(zsbnd, (zs - h));
+    T hn = umax(h, T(0.0001));
+    un = sign * std::sqrt(g / hn) * (zs - zn);
+    q = un * hn + qmean;
+}
+
+// Dirichlet1Q (src/Boundary.cu:2020-2032)
+template <class T> void Dirichlet1Q(T g, T sign, T zsbnd, T zsinside, T hinside, T uninside, T& q) {
+    T zbinside = zsinside - hinside;
+    T un = sign * T(2.0) * (std::sqrt(g * umax(hinside, T(0.0))) - std::sqrt(g * umax(zsbnd - zbinside, T(0.0)))) + uninside;
+    q = un * hinside;
+}
+
+// bndFluxGPUSideCPU (src/Boundary.cu:587-741) for a uniform water level, no
+// atmospheric pressure and no taper; qmean starts at 0 and, with the default
+// Param.dt = 0 (factime 0, facrel 1), stays 0.
+template <class T> void bndFluxSide(const Prm& P, const Side& side, int type, float zsbnd, const Ev<T>& E, const T* un,
+                                    T* Fh, T* Fq, T* Ss) {
     for (size_t ibl = 0; ibl < side.blk.size(); ibl++) {
         int ib = side.blk[ibl];
         for (int tx = 0; tx < BW; tx++) {
             int ix, iy;
             if (side.isright == 0) { ix = tx; iy = side.istop < 0 ? 0 : BW; }
             else { iy = tx; ix = side.isright < 0 ? 0 : BW; }
+            T sign = T(side.isright) + T(side.istop);
+            T zsatm = T(0.0);
+            T taper = T(1.0);
+            float zb_ = zsbnd;  // zsbnd + zsoffset with zsoffset 0
             int i = memloc(ix, iy, ib);
             int inside = Inside(side.isright, side.istop, ix, iy, ib);
+            T zsi = E.zs[i], zsinside = E.zs[inside], hinside = E.h[inside], uninside = un[inside];
+            T zsX = (zb_ + zsatm - 0.5 * (zsi + zsinside)) * taper + 0.5 * (zsi + zsinside);
+            T qmean = T(0.0);
             T F, G, S;
             if (side.isright < 0 || side.istop < 0) { F = Fh[i]; G = Fq[i]; S = Ss[inside]; }
             else { F = Fh[i]; G = Ss[i]; S = Fq[inside]; }
-            F = T(0.0); S = G;  // noslipbndQ
+            T factime = std::min(T(0.0 / 60.0), T(1.0));
+            T facrel = T(1.0) - std::min(T(0.0 / 3600.0), T(1.0));
+            if (type == 0) { F = T(0.0); S = G; }
+            else if (type == 3) {
+                if (E.h[i] > P.eps || zsX > zsi) ABS1DQ(T(P.g), sign, factime, facrel, zsi, zsX, zsinside, E.h[i], qmean, F, G, S);
+                else { F = T(0.0); S = G; qmean = T(0.0); }
+            } else if (type == 2) {
+                if (E.h[i] > P.eps || zsX > zsi) Dirichlet1Q(T(P.g), sign, zsX, zsinside, hinside, uninside, F);
+                else { F = T(0.0); S = G; qmean = T(0.0); }
+            }
             if (side.isright < 0 || side.istop < 0) { Fh[i] = F; Fq[i] = G; Ss[inside] = S; }
             else { Fh[i] = F; Ss[i] = G; Fq[inside] = S; }
         }
     }
 }
 
-// src/Boundary.cu:108-195
-template <class T> void FlowbndFlux(Mdl<T>& M) {
-    if (M.bndtype == 1) return;
-    if (M.bndtype != 0) { std::fprintf(stderr, "oracle: boundary type %d not restated\n", M.bndtype); std::abort(); }
-    bndFluxSideWall(M.left, M.Fhu.data(), M.Fqux.data(), M.Su.data());
-    bndFluxSideWall(M.right, M.Fhu.data(), M.Fqux.data(), M.Su.data());
-    bndFluxSideWall(M.top, M.Fhv.data(), M.Fqvy.data(), M.Sv.data());
-    bndFluxSideWall(M.bot, M.Fhv.data(), M.Fqvy.data(), M.Sv.data());
+// FlowbndFlux (src/Boundary.cu:108-195) for each side at time t (the stage's),
+// on the step-start state: the side's segment type, or the remainder's.
+template <class T> void FlowbndFlux(Mdl<T>& M, double t) {
+    const Side* sides[4] = {&M.left, &M.right, &M.bot, &M.top};
+    for (int s = 0; s < 4; s++) {
+        const int type = M.stype[s] >= 0 ? M.stype[s] : M.bndtype;
+        if (type == 1) continue;
+        double zsbnd = 0.0;
+        if (type >= 2) {
+            const std::vector<double>& tt = M.sbt[s];
+            const std::vector<double>& zz = M.sbz[s];
+            int k = 1;
+            while (k < (int)tt.size() - 1 && tt[k] - t < 0.0) k++;  // the reference does not stop at the end
+            zsbnd = interptime(zz[k], zz[k - 1], tt[k] - tt[k - 1], t - tt[k - 1]);
+        }
+        const bool x = s < 2;
+        bndFluxSide(M.p, *sides[s], type, float(zsbnd), M.ev, x ? M.ev.u.data() : M.ev.v.data(),
+                    x ? M.Fhu.data() : M.Fhv.data(), x ? M.Fqux.data() : M.Fqvy.data(), x ? M.Su.data() : M.Sv.data());
+    }
 }
 
 // --------------------------------------------------------------- time step
@@ -1339,7 +1390,7 @@ template <class T> void FlowCPU(Mdl<T>& M, orc_loop& L) {
     updateKurgXCPU(M, M.ev);
     updateKurgYCPU(M, M.ev);
     fillHaloFlux(M);
-    FlowbndFlux(M);
+    FlowbndFlux(M, L.totaltime + L.dt * 0.5);
     L.dt = double(timestep_clamp(timestepreductionCPU(M), L));
     L.dtmax = L.dt;
     T dt = T(L.dt);
@@ -1351,7 +1402,7 @@ template <class T> void FlowCPU(Mdl<T>& M, orc_loop& L) {
     updateKurgXCPU(M, M.evo);
     updateKurgYCPU(M, M.evo);
     fillHaloFlux(M);
-    FlowbndFlux(M);
+    FlowbndFlux(M, L.totaltime + L.dt);
     updateEVCPU(M, M.evo);
     AdvkernelCPU(M, dt, M.ev, M.evo);
     bottomfrictionCPU(M, dt, M.evo);
@@ -1615,6 +1666,17 @@ int orc_step(orc_model m, orc_loop* loop) {
     // updateforcing before the step (src/Mainloop.cu:41-45)
     if (m->m.dbl) { rain_this_step(m->m.d->rf, loop->totaltime); FlowCPU(*m->m.d, *loop); }
     else { rain_this_step(m->m.f->rf, loop->totaltime); FlowCPU(*m->m.f, *loop); }
+    return 0;
+}
+
+int orc_set_boundary(orc_model m, int side, int type, int n, const double* t, const double* z) {
+    auto set = [&](auto& M) {
+        M.stype[side] = type;
+        M.sbt[side].assign(t, t + (type >= 2 ? n : 0));
+        M.sbz[side].assign(z, z + (type >= 2 ? n : 0));
+    };
+    if (m->m.dbl) set(*m->m.d);
+    else set(*m->m.f);
     return 0;
 }
 
@@ -1958,6 +2020,7 @@ template <class T> Mdl<T>* remesh_t(Mdl<T>& M, const orc_param* p, const orc_blo
     }
     Mdl<T>* N = create<T>(p, b);
     N->rf = M.rf;
+    for (int k = 0; k < 4; k++) { N->stype[k] = M.stype[k]; N->sbt[k] = M.sbt[k]; N->sbz[k] = M.sbz[k]; }
     const T eps = T(M.p.eps);
     const Ev<T>& E = M.ev;
     for (int ibl = 0; ibl < p->nblk; ibl++) {

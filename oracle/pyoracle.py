@@ -99,6 +99,9 @@ def lib():
         L.orc_remesh.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcParam), ctypes.POINTER(OrcBlocks), ip,
                                  ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_void_p]
+        L.orc_set_boundary.restype = ctypes.c_int
+        L.orc_set_boundary.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.c_void_p]
         L.orc_set_rain_series.restype = ctypes.c_int
         L.orc_set_rain_series.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_set_rain_map.restype = ctypes.c_int
@@ -228,6 +231,11 @@ class Oracle:
 
     def volume(self):
         return lib().orc_volume(self.h)
+
+    def set_boundary(self, side, btype, times=(), levels=()):
+        t = np.ascontiguousarray(times, np.float64)
+        z = np.ascontiguousarray(levels, np.float64)
+        lib().orc_set_boundary(self.h, int(side), int(btype), t.size, t.ctypes.data, z.ctypes.data)
 
     def set_rain_series(self, times, values):
         t = np.ascontiguousarray(times, np.float64)

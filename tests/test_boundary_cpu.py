@@ -1,0 +1,98 @@
+"""Boundary types 2 (Dirichlet water level) and 3 (absorbing) on the oracle, pinned by closed forms:
+still water at the boundary level stays still; the volume of one step changes by the boundary mass
+fluxes the reference's formulas give (Dirichlet1Q, ABS1DQ, src/Boundary.cu:1943-2032); an outgoing
+wave reflects less from an absorbing side than from a wall."""
+import numpy as np
+import pytest
+
+from bg_flood_amd import scenarios
+from oracle import pyoracle as po
+from tests._util import oracle_for, interior
+
+G = 9.81
+
+
+def flat(prec, n=32, dx=4.0, depth=2.0, level=0.0, aoibnd=0):
+    sc = scenarios.gaussian_hump(n=n, dx=dx, depth=depth, amp=0.0, precision=prec, aoibnd=aoibnd)
+    T = np.float64 if prec == "fp64" else np.float32
+    m = interior(sc)
+    sc["h"] = np.where(m, T(depth + level), 0).astype(T)
+    return sc
+
+
+def step(o, L):
+    o.step(L)
+    L.totaltime += L.dt
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("btype", [2, 3])
+def test_still_water_at_the_boundary_level_stays_still(prec, btype):
+    sc = flat(prec, level=0.0)
+    o = oracle_for(sc, precision=prec)
+    for side in range(4):
+        o.set_boundary(side, btype, [0.0, 1e6], [0.0, 0.0])
+    h0 = o.get("h").copy()
+    L = po.OrcLoop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+    o.init_loop(L)
+    for _ in range(20):
+        step(o, L)
+    m = interior(sc)
+    assert np.array_equal(o.get("h")[m], h0[m])
+    assert not np.any(o.get("u")[m]) and not np.any(o.get("v")[m])
+
+
+@pytest.mark.parametrize("btype", [2, 3])
+def test_one_step_volume_change_is_the_boundary_flux(btype):
+    """Water 0.5 m above a boundary level of 0 on the right side, walls elsewhere: interior fluxes
+    conserve volume, and both stages read the step-start state at the boundary, so one step
+    changes the volume by -dt * dx * sum of the face fluxes of the closed form."""
+    n, dx, depth, lev = 32, 4.0, 2.0, 0.5
+    sc = flat("fp64", n=n, dx=dx, depth=depth, level=lev)
+    o = oracle_for(sc)
+    o.set_boundary(1, btype, [0.0, 1e6], [0.0, 0.0])
+    v0 = o.volume()
+    L = po.OrcLoop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+    o.init_loop(L)
+    step(o, L)
+    h = depth + lev
+    if btype == 3:
+        # ABS1DQ: zn = max(zsbnd, zs - h) = 0, q = sqrt(g/h) * (zs - zn) * h
+        q = np.sqrt(G / h) * lev * h
+    else:
+        # Dirichlet1Q: q = 2 * (sqrt(g h) - sqrt(g (zsbnd - zb))) * h
+        q = 2.0 * (np.sqrt(G * h) - np.sqrt(G * depth)) * h
+    assert q > 0
+    want = v0 - L.dt * dx * n * q
+    assert abs(o.volume() - want) <= 1e-12 * v0
+
+
+def test_dirichlet_fills_from_a_higher_level():
+    sc = flat("fp64", level=0.0)
+    o = oracle_for(sc)
+    o.set_boundary(0, 2, [0.0, 1e6], [0.3, 0.3])
+    v0 = o.volume()
+    L = po.OrcLoop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+    o.init_loop(L)
+    for _ in range(40):
+        step(o, L)
+    assert o.volume() > v0
+    u = scenarios.grid_from_buq(o.get("u"), 2, 2)
+    assert u[:, 0].min() > 0  # inflow through the left side
+
+
+def test_absorbing_side_reflects_less_than_a_wall():
+    def energy_after(btype):
+        sc = scenarios.gaussian_hump(n=64, dx=4.0, depth=2.0, amp=0.2, radius=12.0, precision="fp64")
+        o = oracle_for(sc)
+        if btype is not None:
+            for side in range(4):
+                o.set_boundary(side, btype, [0.0, 1e6], [0.0, 0.0])
+        L = po.OrcLoop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+        o.init_loop(L)
+        while L.totaltime < 60.0:
+            step(o, L)
+        m = interior(sc)
+        zs, h, u, v = (o.get(k)[m] for k in ("zs", "h", "u", "v"))
+        return float(np.sum(0.5 * G * zs ** 2 + 0.5 * h * (u ** 2 + v ** 2)))
+    assert energy_after(3) < 0.5 * energy_after(None)

@@ -1,0 +1,87 @@
+"""Boundary types 2 (Dirichlet) and 3 (absorbing) on the GPU against the oracle, bit for bit, with
+time-varying water levels, on one-level meshes (stepped and graph-replayed) and multi-level meshes,
+FP64 and FP32."""
+import numpy as np
+import pytest
+
+import bg_flood_amd as bgf
+from bg_flood_amd import scenarios
+from oracle import pyoracle as po
+from tests._util import domain_for, oracle_for
+
+pytestmark = pytest.mark.gpu
+
+LEVELS = {0: (2, [0.0, 4.0, 30.0], [0.0, 0.4, -0.2]),    # left: Dirichlet, rising then falling
+          1: (3, [0.0, 100.0], [0.0, 0.0]),               # right: absorbing
+          2: (3, [0.0, 10.0, 40.0], [0.1, -0.1, 0.2]),    # bottom: absorbing with a moving level
+          3: (0, [], [])}                                  # top: wall
+
+
+def configure(x):
+    for side, (btype, t, z) in LEVELS.items():
+        x.set_boundary(side, btype, t, z)
+
+
+def compare(d, o):
+    m = bgf.interior_mask(d.nblkmem, d.active)
+    st = d.get_state()
+    for k in ("h", "u", "v"):
+        a, b = st[k][m], o.get(k)[m]
+        assert np.array_equal(a, b), "%s: max diff %g" % (k, np.abs(a - b).max())
+
+
+def run_pair(sc, prec, nsteps, graph):
+    print("case boundary %s nblk=%d graph=%s" % (prec, sc["param"]["nblk"], graph), flush=True)
+    d = domain_for(sc, precision=prec)
+    o = oracle_for(sc, precision=prec)
+    configure(d)
+    configure(o)
+    lg = bgf.Loop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+    lo = po.OrcLoop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+    d.init_dt(lg)
+    o.init_loop(lo)
+    if graph:
+        d.run(lg, nsteps)
+        for _ in range(nsteps):
+            o.step(lo)
+            lo.totaltime += lo.dt
+        assert lg.totaltime == lo.totaltime
+    else:
+        for k in range(nsteps):
+            d.step(lg)
+            o.step(lo)
+            assert lg.dt == lo.dt, "step %d" % k
+            lg.totaltime += lg.dt
+            lo.totaltime += lo.dt
+    compare(d, o)
+    return lo.totaltime
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_one_level_boundaries_exact(prec):
+    sc = scenarios.gaussian_hump(n=64, dx=4.0, depth=2.0, amp=0.3, radius=20.0, precision=prec, aoibnd=1)
+    t = run_pair(sc, prec, 60, graph=False)
+    assert t > 10.0
+    run_pair(sc, prec, 60, graph=True)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_multi_level_boundaries_exact(prec):
+    sc = scenarios.amr_basin(precision=prec, island=True, amp=0.5, aoibnd=1)
+    run_pair(sc, prec, 40, graph=False)
+    run_pair(sc, prec, 40, graph=True)
+
+
+def test_fast_kernels_take_the_boundaries():
+    sc = scenarios.gaussian_hump(n=64, dx=4.0, depth=2.0, amp=0.3, radius=20.0, precision="fp32", aoibnd=1)
+    d = domain_for(sc, precision="fp32_fast")
+    e = domain_for(sc, precision="fp32")
+    configure(d)
+    configure(e)
+    for x in (d, e):
+        L = bgf.Loop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+        x.init_dt(L)
+        x.run(L, 80)
+    m = bgf.interior_mask(d.nblkmem, d.active)
+    a, b = d.get_state()["h"][m], e.get_state()["h"][m]
+    assert np.max(np.abs(a - b)) < 1e-3
