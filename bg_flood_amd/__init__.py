@@ -134,6 +134,7 @@ def lib():
         L.bgf_set_bed_source.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, vp]
         L.bgf_set_boundary.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp]
+        L.bgf_set_rivers.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
         L.bgf_set_rain_series.argtypes = [vp, ctypes.c_int, vp, vp]
         L.bgf_set_rain_map.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double, vp]
@@ -151,7 +152,8 @@ def lib():
                      "bgf_batch_sample_gauges", "bgf_batch_stats", "bgf_batch_get_gauges",
                      "bgf_batch_track_maxdepth", "bgf_set_schedule", "bgf_get_schedule",
                      "bgf_batch_set_schedule", "bgf_batch_get_schedule", "bgf_adapt_mesh", "bgf_remesh",
-                     "bgf_set_bed_source", "bgf_set_rain_series", "bgf_set_rain_map", "bgf_set_boundary"):
+                     "bgf_set_bed_source", "bgf_set_rain_series", "bgf_set_rain_map", "bgf_set_boundary",
+                     "bgf_set_rivers"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -395,6 +397,16 @@ class Domain:
             raise BgfError("times and levels differ in length")
         _check(lib().bgf_set_boundary(self.h, sd, int(btype), t.size, t.ctypes.data if t.size else None,
                                       z.ctypes.data if z.size else None))
+
+    def set_rivers(self, rivers):
+        """River discharges (bgf_set_rivers): a list of (xstart, xend, ystart, yend, times, flows)."""
+        boxes = np.array([r[:4] for r in rivers], np.float64).reshape(-1)
+        npts = np.array([len(r[4]) for r in rivers], np.int32)
+        t = np.concatenate([np.asarray(r[4], np.float64) for r in rivers]) if rivers else np.zeros(0)
+        q = np.concatenate([np.asarray(r[5], np.float64) for r in rivers]) if rivers else np.zeros(0)
+        _check(lib().bgf_set_rivers(self.h, len(rivers), boxes.ctypes.data if len(rivers) else None,
+                                    npts.ctypes.data if len(rivers) else None, t.ctypes.data if t.size else None,
+                                    q.ctypes.data if q.size else None))
 
     def set_rain_series(self, times, values):
         """Uniform rain rate [mm/h] as a time series (bgf_set_rain_series); empty removes it."""

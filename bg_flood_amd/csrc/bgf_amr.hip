@@ -677,7 +677,8 @@ __global__ __launch_bounds__(256) void k_amr_advance(KParams P, AdvArgs<T> A, Fl
     const T hi = A.h[i], uui = A.u[i], vvi = A.v[i];
     const T cmdinv = T(1.0) / (cm * delta);
     const T ga = T(0.5) * g;
-    const T dhi = T(-1.0) * (F.Fhu[iright] - F.Fhu[i] + F.Fhv[itop] - F.Fhv[i]) * cmdinv;
+    // tendencies, then the river discharges (AddRiverForcing after updateEV, src/FlowCPU.cu:120-124)
+    const T dhi = river_inflow<T>(P, ib, x, y, T(-1.0) * (F.Fhu[iright] - F.Fhu[i] + F.Fhv[itop] - F.Fhv[i]) * cmdinv);
     const T dmdl = (fmup - fmu) / (cm * delta);
     const T dmdt = (fmvp - fmv) / (cm * delta);
     const T fG = vvi * dmdl - uui * dmdt;

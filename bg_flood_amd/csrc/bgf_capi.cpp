@@ -73,6 +73,15 @@ struct bgf_handle_s {
     float* d_bed = nullptr;
     // boundary water-level series of sides of type 2/3 (time, level)
     double* d_bnd[4] = {nullptr, nullptr, nullptr, nullptr};
+    // host copy of the mesh (active list, levels, origins) and the rivers as given
+    std::vector<int> h_level;
+    std::vector<double> h_xo, h_yo;
+    std::vector<double> riv_box, riv_t, riv_q;
+    std::vector<int> riv_n;
+    // rivers on the device: boxes, series, offsets, areas, block flags, rates
+    double *d_rbox = nullptr, *d_rt = nullptr, *d_rq = nullptr, *d_rarea = nullptr, *d_rrate = nullptr;
+    int* d_roff = nullptr;
+    unsigned char* d_rblk = nullptr;
     int bed_nx = 0, bed_ny = 0;
     double bed_xo = 0, bed_yo = 0, bed_dx = 0;
 };
@@ -91,6 +100,7 @@ void free_handle(bgf_handle h) {
     F(h->d_metric);
     F(h->d_bed);
     for (int k = 0; k < 4; k++) F(h->d_bnd[k]);
+    F(h->d_rbox); F(h->d_rt); F(h->d_rq); F(h->d_rarea); F(h->d_rrate); F(h->d_roff); F(h->d_rblk);
     F(h->d_blkxo); F(h->d_blkyo); F(h->d_rain_t); F(h->d_rain_v); F(h->d_rframes); F(h->d_rnow);
     for (int k = 0; k < 11; k++) F(h->d_mesh2[k]);
     F(h->d_remap); F(h->d_bxo); F(h->d_byo); F(h->d_flags);
@@ -310,6 +320,82 @@ bool upload_metric(bgf_handle h, const bgf_param* p, const bgf_blocks* b, int nb
     return hipMemcpy(h->d_metric, M.data(), sizeof(float) * M.size(), hipMemcpyHostToDevice) == hipSuccess;
 }
 
+// The rivers on the handle's current mesh (InitialConditions.cu:346-445):
+// each river's discharge cells are those whose box, formed in double,
+// overlaps the river's; its discharge area sums their areas; rivers without
+// a cell are dropped, as the reference drops them; the blocks holding a
+// discharge cell of any river are flagged.  Uploads everything the kernels read.
+bgf_status setup_rivers(bgf_handle h) {
+    auto F = [](auto*& p) { if (p) (void)hipFree(p); p = nullptr; };
+    F(h->d_rbox); F(h->d_rt); F(h->d_rq); F(h->d_rarea); F(h->d_rrate); F(h->d_roff); F(h->d_rblk);
+    bgf::KParams& kp = h->kp;
+    kp.nriver = 0;
+    kp.rbox = kp.rt = kp.rq = kp.rarea = nullptr;
+    kp.rrate = nullptr;
+    kp.roff = nullptr;
+    kp.rblk = nullptr;
+    const int nr = (int)h->riv_n.size();
+    if (nr == 0) return BGF_OK;
+    const bgf_param& p = h->p;
+    std::vector<double> box, tt, qq, area;
+    std::vector<int> off(1, 0);
+    std::vector<unsigned char> blk(h->cap, 0);
+    int o = 0;
+    for (int r = 0; r < nr; r++) {
+        const double* bx = &h->riv_box[4 * r];
+        double a = 0.0;
+        std::vector<int> hit;
+        for (int ib : h->active) {
+            const double levdx = level_dx(p.dx, h->h_level[ib]), levdelta = level_dx(p.delta, h->h_level[ib]);
+            bool any = false;
+            for (int j = 0; j < BW; j++)
+                for (int i = 0; i < BW; i++) {
+                    const double xi = p.xo + h->h_xo[ib] + i * levdx, yi = p.yo + h->h_yo[ib] + j * levdx;
+                    const double xl = xi - 0.5 * levdx, yb = yi - 0.5 * levdx, xr = xi + 0.5 * levdx,
+                                 yt = yi + 0.5 * levdx;
+                    if (bx[0] <= xr && bx[1] >= xl && bx[2] <= yt && bx[3] >= yb) {
+                        a = a + levdelta * levdelta;
+                        any = true;
+                    }
+                }
+            if (any) hit.push_back(ib);
+        }
+        const int n = h->riv_n[r];
+        if (a == 0.0) {
+            o += n;
+            continue;  // outside the active domain
+        }
+        for (int ib : hit) blk[ib] = 1;
+        box.insert(box.end(), bx, bx + 4);
+        tt.insert(tt.end(), h->riv_t.begin() + o, h->riv_t.begin() + o + n);
+        qq.insert(qq.end(), h->riv_q.begin() + o, h->riv_q.begin() + o + n);
+        off.push_back(off.back() + n);
+        area.push_back(a);
+        o += n;
+    }
+    const int nk = (int)area.size();
+    if (nk == 0) return BGF_OK;
+    bool ok = hipMalloc((void**)&h->d_rbox, sizeof(double) * box.size()) == hipSuccess &&
+              hipMalloc((void**)&h->d_rt, sizeof(double) * tt.size()) == hipSuccess &&
+              hipMalloc((void**)&h->d_rq, sizeof(double) * qq.size()) == hipSuccess &&
+              hipMalloc((void**)&h->d_rarea, sizeof(double) * nk) == hipSuccess &&
+              hipMalloc((void**)&h->d_rrate, sizeof(double) * 2 * nk) == hipSuccess &&
+              hipMalloc((void**)&h->d_roff, sizeof(int) * (nk + 1)) == hipSuccess &&
+              hipMalloc((void**)&h->d_rblk, h->cap) == hipSuccess;
+    ok = ok && hipMemcpy(h->d_rbox, box.data(), sizeof(double) * box.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(h->d_rt, tt.data(), sizeof(double) * tt.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(h->d_rq, qq.data(), sizeof(double) * qq.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(h->d_rarea, area.data(), sizeof(double) * nk, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(h->d_roff, off.data(), sizeof(int) * (nk + 1), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(h->d_rblk, blk.data(), h->cap, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemset(h->d_rrate, 0, sizeof(double) * 2 * nk) == hipSuccess;
+    if (!ok) return fail(BGF_ERR_HIP, "river upload failed");
+    kp.nriver = nk;
+    kp.rbox = h->d_rbox; kp.rt = h->d_rt; kp.rq = h->d_rq; kp.rarea = h->d_rarea; kp.rrate = h->d_rrate;
+    kp.roff = h->d_roff; kp.rblk = h->d_rblk;
+    return BGF_OK;
+}
+
 // bgf_create with device arrays for cap >= nblkmem block slots.
 bgf_status create_handle(const bgf_param* p, const bgf_blocks* b, int cap, bgf_handle* out) {
     if (!out || !b) return fail(BGF_ERR_INVALID_ARG, "null argument");
@@ -336,6 +422,9 @@ bgf_status create_handle(const bgf_param* p, const bgf_blocks* b, int cap, bgf_h
     h->S = (size_t)nbm * BS;
     const size_t SA = (size_t)cap * BS;  // values per allocated array
     h->active.assign(b->active, b->active + nblk);
+    h->h_level.assign(b->level, b->level + nbm);
+    h->h_xo.assign(b->xo, b->xo + nbm);
+    h->h_yo.assign(b->yo, b->yo + nbm);
     auto A = [&](void** ptr, size_t bytes) -> bool {
         if (hipMalloc(ptr, bytes) != hipSuccess) return false;
         return hipMemset(*ptr, 0, bytes) == hipSuccess;
@@ -661,6 +750,33 @@ bgf_status bgf_set_rain_map(bgf_handle h, int nx, int ny, double xo, double yo, 
     return BGF_OK;
 }
 
+bgf_status bgf_set_rivers(bgf_handle h, int nriver, const double* boxes, const int* npts, const double* times,
+                          const double* flows) {
+    if (!h) return fail(BGF_ERR_INVALID_ARG, "null handle");
+    if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
+    if (nriver < 0 || nriver > 64 || (nriver > 0 && (!boxes || !npts || !times || !flows)))
+        return fail(BGF_ERR_INVALID_ARG, "0..64 rivers, each with a box and a flow series");
+    if (nriver > 0 && h->p.spherical) return fail(BGF_ERR_UNSUPPORTED, "rivers on spherical grids are not provided");
+    int tot = 0;
+    for (int r = 0; r < nriver; r++) {
+        if (npts[r] < 2) return fail(BGF_ERR_INVALID_ARG, "a river flow series needs >= 2 entries");
+        for (int k = 1; k < npts[r]; k++)
+            if (!(times[tot + k] > times[tot + k - 1])) return fail(BGF_ERR_INVALID_ARG, "river times must increase");
+        tot += npts[r];
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->riv_box.assign(boxes, boxes + 4 * nriver);
+    h->riv_n.assign(npts, npts + nriver);
+    h->riv_t.assign(times, times + tot);
+    h->riv_q.assign(flows, flows + tot);
+    bgf_status s = setup_rivers(h);
+    if (h->graph) {
+        (void)hipGraphExecDestroy(h->graph);
+        h->graph = nullptr;
+    }
+    return s;
+}
+
 bgf_status bgf_set_boundary(bgf_handle h, int side, int type, int n, const double* times, const double* levels) {
     if (!h) return fail(BGF_ERR_INVALID_ARG, "null handle");
     if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
@@ -802,8 +918,13 @@ bgf_status bgf_remesh(bgf_handle h, const bgf_param* p, const bgf_blocks* b, con
             nh->kp.bt[k] = nh->d_bnd[k];
             nh->kp.bz[k] = nh->d_bnd[k] ? nh->d_bnd[k] + ok0.bn[k] : nullptr;
         }
+        nh->riv_box = h->riv_box; nh->riv_n = h->riv_n; nh->riv_t = h->riv_t; nh->riv_q = h->riv_q;
         std::swap(*h, *nh);  // h takes the new mesh; nh (the old one) is freed
         bgf_destroy(nh);
+        if (!h->riv_n.empty()) {
+            s = setup_rivers(h);
+            if (s != BGF_OK) return s;
+        }
         g_err.clear();
         return BGF_OK;
     }
@@ -873,6 +994,14 @@ bgf_status bgf_remesh(bgf_handle h, const bgf_param* p, const bgf_blocks* b, con
     if (!ok) return fail(BGF_ERR_HIP, "remesh: upload failed");
     kp.grid_pred = bgf::persistent_grid(h->prec, 0, q.nblk, h->wg_pred);
     kp.grid_corr = bgf::persistent_grid(h->prec, 1, q.nblk, h->wg_corr);
+    h->h_level.assign(b->level, b->level + nbm);
+    h->h_xo.assign(b->xo, b->xo + nbm);
+    h->h_yo.assign(b->yo, b->yo + nbm);
+    if (!h->riv_n.empty()) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        s = setup_rivers(h);  // the discharge cells of the new mesh
+        if (s != BGF_OK) return s;
+    }
     // the bed's halo on the new mesh (src/InitialConditions.cu:135-147)
     bgf::launch_amr_fill(prec, kp, h->d_zb, h->stream);
     hipError_t e = hipGetLastError();

@@ -121,4 +121,25 @@ __device__ bool bnd_flux(const KParams& P, int s, int type, float zsbnd, T zsi, 
     return true;
 }
 
+// ------------------------------------------------------------------ rivers
+// Discharges added to the depth tendency of cell (ix, iy) of block ib, river
+// by river (InjectRiverCPU, src/Updateforcing.cu:314-363): a cell receives a
+// river's rate when its box, formed in T, overlaps the river's (OBBdetect,
+// src/Util_CPU.cu:286-296); only blocks that held a discharge cell when the
+// rivers were set (or the mesh last changed) are visited.
+template <class T> __device__ __forceinline__ T river_inflow(const KParams& P, int ib, int ix, int iy, T dh) {
+    if (!P.nriver || !P.rblk[ib]) return dh;
+    const T levdx = calcres(T(P.dx), P.level[ib]);
+    const T xllo = T(P.xo + P.bxo[ib]), yllo = T(P.yo + P.byo[ib]);
+    const T xl = xllo + T(ix) * levdx - T(0.5) * levdx, yb = yllo + T(iy) * levdx - T(0.5) * levdx;
+    const T xr = xllo + T(ix) * levdx + T(0.5) * levdx, yt = yllo + T(iy) * levdx + T(0.5) * levdx;
+    for (int r = 0; r < P.nriver; r++) {
+        const double* b = P.rbox + 4 * r;
+        const bool tx = T(b[0]) <= xr && T(b[1]) >= xl;
+        const bool ty = T(b[2]) <= yt && T(b[3]) >= yb;
+        if (tx && ty) dh += sizeof(T) == 8 ? T(P.rrate[2 * r]) : T(P.rrate[2 * r + 1]);
+    }
+    return dh;
+}
+
 }  // namespace bgf

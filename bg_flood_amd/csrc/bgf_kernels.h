@@ -93,6 +93,18 @@ struct KParams {
     const double* bt[4];
     const double* bz[4];
     int bn[4];
+    // river discharges (nriver > 0): each river's box (xstart, xend, ystart,
+    // yend), flow series (time rt, flow rq from roff[r] to roff[r+1]), its
+    // discharge area, the blocks that may receive water, and the rates of the
+    // step (rrate[2r] in FP64, rrate[2r+1] the FP32 value), set at its start
+    int nriver;
+    const double* rbox;
+    const double* rt;
+    const double* rq;
+    const int* roff;
+    const double* rarea;
+    const unsigned char* rblk;
+    double* rrate;
 };
 
 // Layout of a block's metric terms (spherical grids, src/Spherical.cu:15-64):
@@ -163,7 +175,8 @@ void launch_amr_step(int precision, const KParams& P, const void* buffers, DevLo
                      hipStream_t st, hipEvent_t* ev);
 // Halo of a static field (the bed) on a multi-level mesh, src/InitialConditions.cu:135-147.
 void launch_amr_fill(int precision, const KParams& P, void* z, hipStream_t st);
-// The rain map of the coming step from the rain frames (nothing without frames).
+// The rain map of the coming step from the rain frames (nothing without frames),
+// and the river discharge rates of the coming step (nothing without rivers).
 void launch_rain_now(const KParams& P, const DevLoop* L, hipStream_t st);
 // The step's halo fill of h, u, v on a multi-level mesh (src/Halo.cu:586-616) by itself.
 void launch_amr_halo(int precision, const KParams& P, void* h, void* u, void* v, const void* zb, hipStream_t st);

@@ -443,6 +443,7 @@ __device__ __forceinline__ T tile_fluxes(Tile<T>& s, const Cst<T>& C, const KPar
     }
     // src/Advection.cu:236-250 (Cartesian: the metric terms vanish)
     dh = T(-1.0) * (Fhr - Fhl + Fht - Fhb) * C.cmdinv;
+    dh = river_inflow<T>(P, bd.ib, x, y, dh);
     dhu = (Fql + Fb.d - Sr - Ft.d) * C.cmdinv + C.fc * q.a * q.d;
     dhv = (Fqb + Fl.d - St - Fr.d) * C.cmdinv - C.fc * q.a * q.c;
     return dtc;
@@ -821,7 +822,26 @@ __global__ __launch_bounds__(256) void k_rain_now(KParams P, const DevLoop* L) {
     P.rnow[i] = (float)((double)uo + (tt - P.rdtf * k) * (double)(un - uo) / P.rdtf);
 }
 
+// River discharge rates of a step, at its start (AddRiverForcing,
+// src/Updateforcing.cu:146-165): the flow interpolated at totaltime between the
+// first entry later than it and the one before, over the discharge area.
+__global__ void k_river_rates(KParams P, const DevLoop* L) {
+    const int r = threadIdx.x;
+    if (r >= P.nriver || L->done) return;
+    const double* t = P.rt + P.roff[r];
+    const double* q = P.rq + P.roff[r];
+    const int n = P.roff[r + 1] - P.roff[r];
+    const double tt = L->totaltime;
+    int b = 0;
+    while (b < n - 1 && t[b] - tt <= 0.0) b++;  // the reference does not stop at the end
+    const int a = b - 1 > 0 ? b - 1 : 0;
+    const double flow = q[a] + (tt - t[a]) / (t[b] - t[a]) * (q[b] - q[a]);
+    P.rrate[2 * r] = (double)(flow / P.rarea[r]);
+    P.rrate[2 * r + 1] = (double)(float)((double)(float)flow / P.rarea[r]);
+}
+
 void launch_rain_now(const KParams& P, const DevLoop* L, hipStream_t st) {
+    if (P.nriver > 0) hipLaunchKernelGGL(k_river_rates, dim3(1), dim3(64), 0, st, P, L);
     if (!P.rframes || P.rnframes < 2) return;
     hipLaunchKernelGGL(k_rain_now, dim3((P.rnx * P.rny + 255) / 256), dim3(256), 0, st, P, L);
 }

@@ -248,6 +248,19 @@ bgf_status bgf_adapt_mesh(const bgf_param* p, const bgf_blocks* b, const unsigne
  * next bgf_step or bgf_run continues on the new mesh. */
 bgf_status bgf_remesh(bgf_handle h, const bgf_param* p, const bgf_blocks* b, const int* remap);
 
+/* River discharges (the reference's rivers, src/Updateforcing.cu:146-363,
+ * src/InitialConditions.cu:346-445): nriver rectangles boxes[4r..4r+3] =
+ * (xstart, xend, ystart, yend) in the coordinates of Param.xo/yo, each with
+ * a flow series of npts[r] (time [s], flow [m3/s]) pairs (times and flows
+ * concatenated river by river).  A river discharges into the cells whose box
+ * overlaps its rectangle, spread over their total area, added to the depth
+ * tendency of both stages at the rate interpolated at the step's start.
+ * Rivers that reach no cell are dropped, as in the reference; the discharge
+ * cells are found again after bgf_remesh.  nriver = 0 removes them.
+ * Cartesian grids only. */
+bgf_status bgf_set_rivers(bgf_handle h, int nriver, const double* boxes, const int* npts, const double* times,
+                          const double* flows);
+
 /* Boundary segment of one side of the domain (0 left, 1 right, 2 bottom,
  * 3 top), as the reference's left/right/bot/top boundary keys define one
  * (bndsegment, src/Forcing.h:106-144): type 0 wall (noslipbndQ), 1 Neumann,

@@ -181,8 +181,18 @@ struct RainForcing {
     double nowvalue = 0;           // the uniform rate of this step
 };
 
+// Rivers (src/Input.h:45-54): a discharge rectangle and a flow series; the
+// discharge area and the river blocks come from the mesh (InitialConditions.cu:346-445).
+struct River {
+    double xstart, xend, ystart, yend;
+    std::vector<double> t, q;
+    double disarea = 0.0;
+};
+
 template <class T> struct Mdl {
     RainForcing rf;
+    std::vector<River> rivers;
+    std::vector<int> riverblk;
     Prm p;
     Blk<T> b;
     Ev<T> ev, evo;
@@ -1381,6 +1391,76 @@ template <class T> void AddrainforcingImplicitCPU(Mdl<T>& M, const orc_loop& L, 
     }
 }
 
+// Discharge cells and areas of the rivers on the current mesh
+// (src/InitialConditions.cu:346-445): cells whose box, in double, overlaps a
+// river's rectangle; rivers without any are dropped; the sorted union of the
+// blocks holding such cells.
+template <class T> void InitRivers(Mdl<T>& M) {
+    const Prm& P = M.p;
+    std::vector<int> all;
+    std::vector<River> keep;
+    for (River R : M.rivers) {
+        double dischargeArea = 0.0;
+        for (int ibl = 0; ibl < P.nblk; ibl++) {
+            int ib = M.b.active[ibl];
+            double levdx = calcres(P.dx, M.b.level[ib]);
+            double levdelta = calcres(P.delta, M.b.level[ib]);
+            for (int j = 0; j < BW; j++)
+                for (int i = 0; i < BW; i++) {
+                    double xi = P.xo + M.b.xo[ib] + i * levdx;
+                    double yi = P.yo + M.b.yo[ib] + j * levdx;
+                    double xl = xi - 0.5 * levdx, yb = yi - 0.5 * levdx, xr = xi + 0.5 * levdx, yt = yi + 0.5 * levdx;
+                    bool testX = R.xstart <= xr && R.xend >= xl;
+                    bool testY = R.ystart <= yt && R.yend >= yb;
+                    if (testX && testY) {
+                        all.push_back(ib);
+                        dischargeArea = dischargeArea + levdelta * levdelta;
+                    }
+                }
+        }
+        R.disarea = dischargeArea;
+        if (R.disarea != 0.0) keep.push_back(R);
+    }
+    M.rivers = keep;
+    std::sort(all.begin(), all.end());
+    all.erase(std::unique(all.begin(), all.end()), all.end());
+    M.riverblk = all;
+}
+
+// AddRiverForcing + InjectRiverCPU (src/Updateforcing.cu:146-363): each
+// river's flow at the step's start over its area, added to dh of the cells of
+// the river blocks whose box, in T, overlaps the river's rectangle.
+template <class T> void AddRiverForcing(Mdl<T>& M, const orc_loop& L) {
+    const Prm& P = M.p;
+    for (const River& R : M.rivers) {
+        int bndstep = 0;
+        double difft = R.t[bndstep] - L.totaltime;
+        while (difft <= 0.0 && bndstep < (int)R.t.size() - 1) {  // the reference does not stop at the end
+            bndstep++;
+            difft = R.t[bndstep] - L.totaltime;
+        }
+        int a = std::max(bndstep - 1, 0);
+        T qnow = T(interptime(R.q[bndstep], R.q[a], R.t[bndstep] - R.t[a], L.totaltime - R.t[a]));
+        T rate = T(qnow / R.disarea);
+        for (int ib : M.riverblk) {
+            T levdx = calcres(T(P.dx), M.b.level[ib]);
+            T xllo = T(P.xo + M.b.xo[ib]);
+            T yllo = T(P.yo + M.b.yo[ib]);
+            for (int iy = 0; iy < BW; iy++)
+                for (int ix = 0; ix < BW; ix++) {
+                    int i = memloc(ix, iy, ib);
+                    T xl = xllo + ix * levdx - T(0.5) * levdx;
+                    T yb = yllo + iy * levdx - T(0.5) * levdx;
+                    T xr = xllo + ix * levdx + T(0.5) * levdx;
+                    T yt = yllo + iy * levdx + T(0.5) * levdx;
+                    bool testX = T(R.xstart) <= xr && T(R.xend) >= xl;
+                    bool testY = T(R.ystart) <= yt && T(R.yend) >= yb;
+                    if (testX && testY) M.dh[i] += rate;
+                }
+        }
+    }
+}
+
 // src/FlowCPU.cu:15-253 (engine 2; no atmospheric pressure, wind, rivers,
 // infiltration, velocity threshold or conserveElevation)
 template <class T> void FlowCPU(Mdl<T>& M, orc_loop& L) {
@@ -1395,6 +1475,7 @@ template <class T> void FlowCPU(Mdl<T>& M, orc_loop& L) {
     L.dtmax = L.dt;
     T dt = T(L.dt);
     updateEVCPU(M, M.ev);
+    if (!M.rivers.empty()) AddRiverForcing(M, L);
     AdvkernelCPU(M, dt * T(0.5), M.ev, M.evo);
 
     fillHaloEv(M, M.evo);
@@ -1404,6 +1485,7 @@ template <class T> void FlowCPU(Mdl<T>& M, orc_loop& L) {
     fillHaloFlux(M);
     FlowbndFlux(M, L.totaltime + L.dt);
     updateEVCPU(M, M.evo);
+    if (!M.rivers.empty()) AddRiverForcing(M, L);
     AdvkernelCPU(M, dt, M.ev, M.evo);
     bottomfrictionCPU(M, dt, M.evo);
     cleanupCPU(M, M.evo, M.ev);
@@ -1666,6 +1748,22 @@ int orc_step(orc_model m, orc_loop* loop) {
     // updateforcing before the step (src/Mainloop.cu:41-45)
     if (m->m.dbl) { rain_this_step(m->m.d->rf, loop->totaltime); FlowCPU(*m->m.d, *loop); }
     else { rain_this_step(m->m.f->rf, loop->totaltime); FlowCPU(*m->m.f, *loop); }
+    return 0;
+}
+
+int orc_set_rivers(orc_model m, int nriver, const double* boxes, const int* npts, const double* t, const double* q) {
+    std::vector<River> rs;
+    int o = 0;
+    for (int r = 0; r < nriver; r++) {
+        River R;
+        R.xstart = boxes[4 * r]; R.xend = boxes[4 * r + 1]; R.ystart = boxes[4 * r + 2]; R.yend = boxes[4 * r + 3];
+        R.t.assign(t + o, t + o + npts[r]);
+        R.q.assign(q + o, q + o + npts[r]);
+        o += npts[r];
+        rs.push_back(R);
+    }
+    if (m->m.dbl) { m->m.d->rivers = rs; InitRivers(*m->m.d); }
+    else { m->m.f->rivers = rs; InitRivers(*m->m.f); }
     return 0;
 }
 
@@ -2020,6 +2118,7 @@ template <class T> Mdl<T>* remesh_t(Mdl<T>& M, const orc_param* p, const orc_blo
     }
     Mdl<T>* N = create<T>(p, b);
     N->rf = M.rf;
+    N->rivers = M.rivers;
     for (int k = 0; k < 4; k++) { N->stype[k] = M.stype[k]; N->sbt[k] = M.sbt[k]; N->sbz[k] = M.sbz[k]; }
     const T eps = T(M.p.eps);
     const Ev<T>& E = M.ev;
@@ -2075,6 +2174,7 @@ template <class T> Mdl<T>* remesh_t(Mdl<T>& M, const orc_param* p, const orc_blo
     for (size_t i = 0; i < S; i++) N->ev.zs[i] = N->zb[i] + N->ev.h[i];
     N->evo = N->ev;
     calcactiveCellCPU(*N);
+    if (!N->rivers.empty()) InitRivers(*N);
     return N;
 }
 }  // namespace

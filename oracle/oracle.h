@@ -73,6 +73,9 @@ int orc_set_rain_series(orc_model m, int n, const double* t, const double* v);
 int orc_set_rain_map(orc_model m, int nx, int ny, double xo, double yo, double dx, int nframes, double t0, double dtf,
                      const float* frames);
 
+/* Rivers: nriver rectangles (xstart, xend, ystart, yend) with flow series of npts[r] (time, m3/s)
+ * pairs, concatenated; discharge cells found on the model's mesh. */
+int orc_set_rivers(orc_model m, int nriver, const double* boxes, const int* npts, const double* t, const double* q);
 /* Boundary segment of one side (0 left, 1 right, 2 bottom, 3 top): type 0 wall, 1 Neumann, 2
  * Dirichlet, 3 absorbing; types 2 and 3 take a uniform water-level series of n (time, level) pairs. */
 int orc_set_boundary(orc_model m, int side, int type, int n, const double* t, const double* z);

@@ -99,6 +99,9 @@ def lib():
         L.orc_remesh.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcParam), ctypes.POINTER(OrcBlocks), ip,
                                  ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_void_p]
+        L.orc_set_rivers.restype = ctypes.c_int
+        L.orc_set_rivers.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_void_p]
         L.orc_set_boundary.restype = ctypes.c_int
         L.orc_set_boundary.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                        ctypes.c_void_p]
@@ -231,6 +234,13 @@ class Oracle:
 
     def volume(self):
         return lib().orc_volume(self.h)
+
+    def set_rivers(self, rivers):
+        boxes = np.array([r[:4] for r in rivers], np.float64).reshape(-1)
+        npts = np.array([len(r[4]) for r in rivers], np.int32)
+        t = np.concatenate([np.asarray(r[4], np.float64) for r in rivers])
+        q = np.concatenate([np.asarray(r[5], np.float64) for r in rivers])
+        lib().orc_set_rivers(self.h, len(rivers), boxes.ctypes.data, npts.ctypes.data, t.ctypes.data, q.ctypes.data)
 
     def set_boundary(self, side, btype, times=(), levels=()):
         t = np.ascontiguousarray(times, np.float64)

@@ -1,0 +1,94 @@
+"""River discharges on the GPU against the oracle, bit for bit: one-level meshes (stepped and
+graph-replayed) and multi-level meshes, FP64 and FP32, and the discharge cells found again after a
+remesh."""
+import numpy as np
+import pytest
+
+import bg_flood_amd as bgf
+from bg_flood_amd import scenarios
+from oracle import pyoracle as po
+from tests._util import domain_for, oracle_for, orc_param_with
+
+pytestmark = pytest.mark.gpu
+
+
+def rivers_for(L):
+    """Two rivers in a domain of side L: a small one and a wide one straddling block edges."""
+    return [(0.2 * L, 0.23 * L, 0.5 * L, 0.52 * L, [0.0, 6.0, 200.0], [0.0, 25.0, 5.0]),
+            (0.6 * L, 0.75 * L, 0.24 * L, 0.26 * L, [0.0, 1e6], [12.0, 12.0])]
+
+
+def compare(d, o):
+    m = bgf.interior_mask(d.nblkmem, d.active)
+    st = d.get_state()
+    for k in ("h", "u", "v"):
+        a, b = st[k][m], o.get(k)[m]
+        assert np.array_equal(a, b), "%s: max diff %g" % (k, np.abs(a - b).max())
+
+
+def run_pair(sc, prec, nsteps, graph, L):
+    print("case rivers %s nblk=%d graph=%s" % (prec, sc["param"]["nblk"], graph), flush=True)
+    d = domain_for(sc, precision=prec)
+    o = oracle_for(sc, precision=prec)
+    d.set_rivers(rivers_for(L))
+    o.set_rivers(rivers_for(L))
+    lg = bgf.Loop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+    lo = po.OrcLoop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+    d.init_dt(lg)
+    o.init_loop(lo)
+    if graph:
+        d.run(lg, nsteps)
+        for _ in range(nsteps):
+            o.step(lo)
+            lo.totaltime += lo.dt
+        assert lg.totaltime == lo.totaltime
+    else:
+        for k in range(nsteps):
+            d.step(lg)
+            o.step(lo)
+            assert lg.dt == lo.dt, "step %d" % k
+            lg.totaltime += lg.dt
+            lo.totaltime += lo.dt
+    compare(d, o)
+    return d, o
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_one_level_rivers_exact(prec):
+    sc = scenarios.gaussian_hump(n=64, dx=4.0, depth=1.0, amp=0.0, precision=prec, aoibnd=0)
+    run_pair(sc, prec, 40, False, 256.0)
+    run_pair(sc, prec, 40, True, 256.0)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_multi_level_rivers_exact(prec):
+    sc = scenarios.amr_basin(precision=prec, island=True, amp=0.3)
+    L = sc["nbx"] * 16 * sc["param"]["dx"]
+    run_pair(sc, prec, 30, False, L)
+    run_pair(sc, prec, 30, True, L)
+
+
+def test_rivers_after_remesh():
+    from tests.test_gpu_remesh import scenario, NBX, NBY, DX
+    sc = scenario("fp64")
+    L = NBX * 16 * DX
+    d, o = run_pair(sc, "fp64", 8, False, L)
+    lg = bgf.Loop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+    lo = po.OrcLoop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0)
+    d.init_dt(lg)
+    o.init_loop(lo)
+    for cycle in range(3):
+        ref = d.refine_flags("zs", 2, 1e-3)
+        co = np.zeros(d.nblkmem, np.uint8)
+        co[d.active] = 1 - ref[d.active]
+        blocks, nb, nm, remap = bgf.adapt_mesh(d.param, d.blocks, ref, co, NBX, NBY)
+        d.remesh(blocks, nb, nm, remap)
+        o.remesh(orc_param_with(o.p, nblk=nb, nblkmem=nm), blocks, remap)
+        compare(d, o)
+        for _ in range(8):
+            d.step(lg)
+            o.step(lo)
+            assert lg.dt == lo.dt
+            lg.totaltime += lg.dt
+            lo.totaltime += lo.dt
+        compare(d, o)
