@@ -801,6 +801,8 @@ bgf_status bgf_set_boundary(bgf_handle h, int side, int type, int n, const doubl
         kp.bn[side] = n;
     }
     kp.btype[side] = type;
+    kp.bopen = 0;
+    for (int k = 0; k < 4; k++) kp.bopen |= kp.btype[k] >= 2 ? 1 : 0;
     if (h->graph) {
         (void)hipGraphExecDestroy(h->graph);
         h->graph = nullptr;
@@ -914,6 +916,7 @@ bgf_status bgf_remesh(bgf_handle h, const bgf_param* p, const bgf_blocks* b, con
         for (int k = 0; k < 4; k++) {
             std::swap(nh->d_bnd[k], h->d_bnd[k]);
             nh->kp.btype[k] = ok0.btype[k];
+            nh->kp.bopen = ok0.bopen;
             nh->kp.bn[k] = ok0.bn[k];
             nh->kp.bt[k] = nh->d_bnd[k];
             nh->kp.bz[k] = nh->d_bnd[k] ? nh->d_bnd[k] + ok0.bn[k] : nullptr;

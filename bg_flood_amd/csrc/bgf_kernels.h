@@ -90,6 +90,7 @@ struct KParams {
     // 1 Neumann, 2 Dirichlet water level, 3 absorbing; sides 2/3 take their
     // water level from a uniform series (bn[s] pairs of time bt, level bz)
     int btype[4];
+    int bopen;  // some side has type 2 or 3
     const double* bt[4];
     const double* bz[4];
     int bn[4];

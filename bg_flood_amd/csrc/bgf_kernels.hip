@@ -239,37 +239,6 @@ __device__ __forceinline__ bool kurg_face(const Cst<T>& C, T hi, T hn, T zsi, T 
     return wet;
 }
 
-// Boundary of the block being computed: the type of each side at the domain
-// edge (-1 inside the domain), the stage's water level of sides of type 2/3,
-// and the step-start state those sides read.
-template <class T> struct Bnd {
-    int ty[4];
-    float zs[4];
-    const T *h, *u, *v, *zb;
-    int ib;
-    bool any;
-};
-
-template <class T>
-__device__ __forceinline__ Bnd<T> block_bnd(const KParams& P, int ib, int nL, int nR, int nB, int nT, double tstage,
-                                            const T* h, const T* u, const T* v, const T* zb) {
-    Bnd<T> b;
-    const int nb[4] = {nL, nR, nB, nT};
-    b.any = false;
-    b.h = h; b.u = u; b.v = v; b.zb = zb;
-    b.ib = ib;
-#pragma unroll
-    for (int s = 0; s < 4; s++) {
-        b.ty[s] = nb[s] == ib ? P.btype[s] : -1;
-        b.zs[s] = 0.0f;
-        if (b.ty[s] >= 2) {
-            b.any = true;
-            b.zs[s] = bnd_level(P, s, tstage);
-        }
-    }
-    return b;
-}
-
 template <class T> struct Tile {
     Q4<T> q[20 * TP];   // (h, zs, u, v) at x, y in [-2, 17], corners unused
     Q4<T> gx[16 * GXP]; // slopes along x of (h, zs, u, v) at x in [-1,16], y in [0,15]
@@ -395,9 +364,8 @@ __device__ __forceinline__ bool yface(Tile<T>& s, const Cst<T>& C, int x, int yf
 // dtmax: its X face then its Y face, a dry face storing 1/epsi outright
 // (src/Kurganov.cu:102-105, 159, 1488-1491, 1537).
 template <class T, bool FAST, bool WANT_DT, bool DEV>
-__device__ __forceinline__ T tile_fluxes(Tile<T>& s, const Cst<T>& C, const KParams& P, int t, const Bnd<T>& bd,
+__device__ __forceinline__ T tile_fluxes(Tile<T>& s, const Cst<T>& C, int t, bool wl, bool wr, bool wb, bool wt,
                                          T& dh, T& dhu, T& dhv) {
-    const bool wl = bd.ty[0] == 0, wr = bd.ty[1] == 0, wb = bd.ty[2] == 0, wt = bd.ty[3] == 0;
     tile_gradients<T, FAST, DEV>(s, C, t);
     __syncthreads();
     const int x = t & 15, y = t >> 4;
@@ -418,35 +386,62 @@ __device__ __forceinline__ T tile_fluxes(Tile<T>& s, const Cst<T>& C, const KPar
     if (wr && x == BW - 1) { Fhr = T(0.0); Fql = Sr; }
     if (wb && y == 0) { Fhb = T(0.0); St = Fqb; }
     if (wt && y == BW - 1) { Fht = T(0.0); Fqb = St; }
-    if (bd.any) {
-        // boundary types 2 and 3 (bndFluxGPUSideCPU): the face's mass flux from
-        // the step-start state, or the no-slip patch on a dry face
-        auto open = [&](int sd, int ci, int cin, T& Fh) {
-            const T hi = bd.h[ci], hin = bd.h[cin];
-            T q = Fh;
-            const bool ok = bnd_flux<T>(P, sd, bd.ty[sd], bd.zs[sd], bd.zb[ci] + hi, hi, bd.zb[cin] + hin, hin,
-                                        (sd < 2 ? bd.u : bd.v)[cin], q);
-            if (ok) Fh = q;
-            return ok;
-        };
-        const int ib = bd.ib;
-        if (bd.ty[0] >= 2 && x == 0 && !open(0, memloc(0, y, ib), memloc(1, y, ib), Fhl)) { Fhl = T(0.0); Sr = Fql; }
-        if (bd.ty[1] >= 2 && x == BW - 1 && !open(1, memloc(BW - 1, y, ib), memloc(BW - 1, y, ib), Fhr)) {
-            Fhr = T(0.0);
-            Fql = Sr;
-        }
-        if (bd.ty[2] >= 2 && y == 0 && !open(2, memloc(x, 0, ib), memloc(x, 1, ib), Fhb)) { Fhb = T(0.0); St = Fqb; }
-        if (bd.ty[3] >= 2 && y == BW - 1 && !open(3, memloc(x, BW - 1, ib), memloc(x, BW - 1, ib), Fht)) {
-            Fht = T(0.0);
-            Fqb = St;
-        }
-    }
     // src/Advection.cu:236-250 (Cartesian: the metric terms vanish)
     dh = T(-1.0) * (Fhr - Fhl + Fht - Fhb) * C.cmdinv;
-    dh = river_inflow<T>(P, bd.ib, x, y, dh);
     dhu = (Fql + Fb.d - Sr - Ft.d) * C.cmdinv + C.fc * q.a * q.d;
     dhv = (Fqb + Fl.d - St - Fr.d) * C.cmdinv - C.fc * q.a * q.c;
     return dtc;
+}
+
+// Tendencies of an edge cell of a block on a side of type 2 or 3, recomputed
+// from the tile's faces (still in LDS after tile_fluxes) with the boundary
+// face's mass flux from bnd_flux, or the no-slip patch on a dry face.  Runs
+// only for blocks with such a side, after tile_fluxes; it reads the four mass
+// fluxes of the cell, and the momentum terms only where a dry face falls back
+// to the no-slip patch, so that it adds few registers to the kernel.
+template <class T>
+__device__ void open_sides(const Tile<T>& s, const Cst<T>& C, const KParams& P, int t, int ib, int nL, int nR, int nB,
+                           int nT, double tstage, const T* h, const T* u, const T* v, const T* zb, T& dh, T& dhu,
+                           T& dhv) {
+    const int x = t & 15, y = t >> 4;
+    const int side[4] = {nL == ib && x == 0 ? P.btype[0] : -1, nR == ib && x == BW - 1 ? P.btype[1] : -1,
+                         nB == ib && y == 0 ? P.btype[2] : -1, nT == ib && y == BW - 1 ? P.btype[3] : -1};
+    if (side[0] < 2 && side[1] < 2 && side[2] < 2 && side[3] < 2) return;
+    T F[4] = {s.fx[fxi(x, y)].a, s.fx[fxi(x + 1, y)].a, s.fy[fyi(x, y)].a, s.fy[fyi(x, y + 1)].a};
+    bool slip[4];
+#pragma unroll
+    for (int sd = 0; sd < 4; sd++) {
+        slip[sd] = side[sd] == 0;
+        if (side[sd] >= 2) {
+            const int ci = sd == 0 ? memloc(0, y, ib) : sd == 1 ? memloc(BW - 1, y, ib) : sd == 2 ? memloc(x, 0, ib)
+                                                                                            : memloc(x, BW - 1, ib);
+            const int cin = sd == 0 ? memloc(1, y, ib) : sd == 2 ? memloc(x, 1, ib) : ci;
+            const T hi = h[ci], hin = h[cin];
+            T q = F[sd];
+            if (bnd_flux<T>(P, sd, side[sd], bnd_level(P, sd, tstage), zb[ci] + hi, hi, zb[cin] + hin, hin,
+                            (sd < 2 ? u : v)[cin], q))
+                F[sd] = q;
+            else
+                slip[sd] = true;
+        }
+        if (slip[sd]) F[sd] = T(0.0);
+    }
+    dh = T(-1.0) * (F[1] - F[0] + F[3] - F[2]) * C.cmdinv;
+    const Q4<T> q = s.q[tix(x, y)];
+    if ((side[0] >= 2 && slip[0]) || (side[1] >= 2 && slip[1])) {
+        const Q4<T> Fl = s.fx[fxi(x, y)], Fr = s.fx[fxi(x + 1, y)];
+        T Fql = Fl.b, Sr = Fr.c;
+        if (slip[0]) Sr = Fql;
+        if (slip[1]) Fql = Sr;
+        dhu = (Fql + s.fy[fyi(x, y)].d - Sr - s.fy[fyi(x, y + 1)].d) * C.cmdinv + C.fc * q.a * q.d;
+    }
+    if ((side[2] >= 2 && slip[2]) || (side[3] >= 2 && slip[3])) {
+        const Q4<T> Fb = s.fy[fyi(x, y)], Ft = s.fy[fyi(x, y + 1)];
+        T Fqb = Fb.b, St = Ft.c;
+        if (slip[2]) St = Fqb;
+        if (slip[3]) Fqb = St;
+        dhv = (Fqb + s.fx[fxi(x, y)].d - St - s.fx[fxi(x + 1, y)].d) * C.cmdinv - C.fc * q.a * q.c;
+    }
 }
 
 // src/Advection.cu:332-385.  The drying limiter evaluates
@@ -540,7 +535,10 @@ __device__ __forceinline__ Sched schedule(int w, int nwg, int n) {
 // previous one.  One atomic min per workgroup (per member and workgroup in a
 // batch, whose processing order keeps each member's blocks together; blocks
 // of members that reached their end time are skipped).
-template <class T, bool FAST, bool BATCH, bool DEV>
+// EXT kernels also take the open boundaries (types 2, 3), rivers and rain maps;
+// the plain ones leave that code out, which keeps their register count (and
+// so their occupancy) down.
+template <class T, bool FAST, bool BATCH, bool DEV, bool EXT = false>
 __global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuffers<T> B, DevLoop* L) {
     if (!BATCH && L->done) return;
     __shared__ Tile<T> s;
@@ -590,10 +588,13 @@ __global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuff
             if (ldu(&L[n.sid].done, 0)) continue;
         }
         T dh, dhu, dhv;
-        // boundary values at the predictor's time, totaltime + dt/2 with the previous dt (src/FlowCPU.cu:94)
-        const Bnd<T> bd = block_bnd<T>(P, n.ib, n.nL, n.nR, n.nB, n.nT, BATCH ? 0.0 : L->totaltime + L->dt * 0.5, B.h,
-                                       B.u, B.v, B.zb);
-        T dtc = tile_fluxes<T, FAST, true, DEV>(s, C, P, t, bd, dh, dhu, dhv);
+        T dtc = tile_fluxes<T, FAST, true, DEV>(s, C, t, P.btype[0] == 0 && n.nL == n.ib, P.btype[1] == 0 && n.nR == n.ib,
+                                           P.btype[2] == 0 && n.nB == n.ib, P.btype[3] == 0 && n.nT == n.ib, dh, dhu, dhv);
+        if (EXT && P.bopen && (n.nL == n.ib || n.nR == n.ib || n.nB == n.ib || n.nT == n.ib))
+            // boundary values at the predictor's time, totaltime + dt/2 with the previous dt (src/FlowCPU.cu:94)
+            open_sides<T>(s, C, P, t, n.ib, n.nL, n.nR, n.nB, n.nT, L->totaltime + L->dt * 0.5, B.h, B.u, B.v, B.zb, dh,
+                          dhu, dhv);
+        if (EXT && P.nriver) dh = river_inflow<T>(P, n.ib, x, y, dh);
         const int i = memloc(x, y, n.ib);
         B.dh[i] = dh;
         B.dhu[i] = dhu;
@@ -619,7 +620,7 @@ template <class T> __device__ __forceinline__ T kahan_add(T h, T d, T& c) {
 // Persistent corrector with the same schedule and prefetch.  COMP: the depth
 // update accumulates with a compensated (Kahan) sum, so the small increments
 // of rain and of slow flows are not lost to FP32 rounding of h.
-template <class T, bool FAST, bool BATCH, bool COMP>
+template <class T, bool FAST, bool BATCH, bool COMP, bool EXT = false>
 __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuffers<T> B, DevLoop* L) {
     if (!BATCH && !L->go) return;
     __shared__ Tile<T> s;
@@ -682,9 +683,12 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
         }
         const int ib = n.ib, nL = n.nL, nR = n.nR, nB = n.nB, nT = n.nT;
         T dh, dhu, dhv;
-        // boundary values at totaltime + dt, from the step-start state (src/FlowCPU.cu:187)
-        const Bnd<T> bd = block_bnd<T>(P, ib, nL, nR, nB, nT, BATCH ? 0.0 : L->tcorr, B.h, B.u, B.v, B.zb);
-        (void)tile_fluxes<T, FAST, false, COMP>(s, C, P, t, bd, dh, dhu, dhv);
+        (void)tile_fluxes<T, FAST, false, COMP>(s, C, t, P.btype[0] == 0 && nL == ib, P.btype[1] == 0 && nR == ib,
+                                                P.btype[2] == 0 && nB == ib, P.btype[3] == 0 && nT == ib, dh, dhu, dhv);
+        if (EXT && P.bopen && (nL == ib || nR == ib || nB == ib || nT == ib))
+            // boundary values at totaltime + dt, from the step-start state (src/FlowCPU.cu:187)
+            open_sides<T>(s, C, P, t, ib, nL, nR, nB, nT, L->tcorr, B.h, B.u, B.v, B.zb, dh, dhu, dhv);
+        if (EXT && P.nriver) dh = river_inflow<T>(P, ib, x, y, dh);
         T h1, u1, v1;
         const int i = memloc(x, y, ib);
         T comp = COMP ? B.ch[i] : T(0);
@@ -728,7 +732,13 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
         // implicit uniform rain after cleanup, src/Updateforcing.cu:530-581
         if (P.rain_on) {
             T act = (double)zb0 < C.mask_d ? T(1.0) : T(0.0);
-            T rainhh = rain_depth<T>(P, Lm, ib, x, y, dt) * act;
+            T rainhh;
+            if (EXT) {
+                rainhh = rain_depth<T>(P, Lm, ib, x, y, dt) * act;
+            } else {
+                const T a = (sizeof(T) == 8 ? T(Lm->rate64) : T(Lm->rate32)) * dt;
+                rainhh = (a < T(0.0) ? T(0.0) : a) * act;  // std::max(a, 0)
+            }
             T qvol = FAST ? h1 * Ar<T, FAST>::rcp(h1 + rainhh) : h1 / (h1 + rainhh);
             bool wetold = (double)h1 > C.eps_d;
             h1 = COMP ? kahan_add(h1, rainhh, comp) : h1 + rainhh;
@@ -961,11 +971,16 @@ void launch_step_t(const KParams& P, const StepBuffers<T>& B, DevLoop* L, int ad
     if (!BATCH) launch_rain_now(P, L, st);
     if (ev) (void)hipEventRecord(ev[0], st);
     const Cst<T> C = make_cst<T, FAST>(P);
-    hipLaunchKernelGGL((k_predictor<T, FAST, BATCH, COMP>), dim3(P.grid_pred), dim3(256), 0, st, P, C, B, L);
+    // batches never take the extra forcings: only one-level handles get the EXT kernels
+    constexpr bool EXT_OK = !BATCH;
+    const bool ext = EXT_OK && (P.bopen || P.nriver || P.rnow);
+    if (ext) hipLaunchKernelGGL((k_predictor<T, FAST, BATCH, COMP, EXT_OK>), dim3(P.grid_pred), dim3(256), 0, st, P, C, B, L);
+    else hipLaunchKernelGGL((k_predictor<T, FAST, BATCH, COMP>), dim3(P.grid_pred), dim3(256), 0, st, P, C, B, L);
     if (ev) (void)hipEventRecord(ev[1], st);
     hipLaunchKernelGGL((k_reduce_dt<T>), dim3(BATCH ? P.nmember : 1), dim3(64), 0, st, L, advance_time);
     if (ev) (void)hipEventRecord(ev[2], st);
-    hipLaunchKernelGGL((k_corrector<T, FAST, BATCH, COMP>), dim3(P.grid_corr), dim3(256), 0, st, P, C, B, L);
+    if (ext) hipLaunchKernelGGL((k_corrector<T, FAST, BATCH, COMP, EXT_OK>), dim3(P.grid_corr), dim3(256), 0, st, P, C, B, L);
+    else hipLaunchKernelGGL((k_corrector<T, FAST, BATCH, COMP>), dim3(P.grid_corr), dim3(256), 0, st, P, C, B, L);
     if (ev) (void)hipEventRecord(ev[3], st);
 }
 
