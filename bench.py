@@ -395,6 +395,7 @@ def main():
             cpu = dict(value=None, unit="Gcell-updates/s", cores=4, kind="port", sample="failed: %r" % (e,))
 
     d.close()
+    nblk_main = sc["param"]["nblk"]
     del sc
     ens = None
     if not args.no_ensemble:
@@ -430,7 +431,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "rain on terrain %dx%d cells (%d blocks of 16x16), dx=5 m, seeded 6-octave "
                                "value-noise bed 20 m + 0.002 slope, rain 50 mm/h, Manning n=0.03, outflow "
-                               "boundaries, initially dry" % (args.n, args.n, sc["param"]["nblk"]),
+                               "boundaries, initially dry" % (args.n, args.n, nblk_main),
                    "precision": args.precision, "simulated_time_s": round(loop.totaltime, 3),
                    "kernels_ms": {k: round(v, 4) for k, v in r["kt"].items()}},
         "roofline": roof,
