@@ -29,6 +29,7 @@ struct bgf_handle_s {
     size_t S;   // elements per array
     std::vector<int> active;
     int *d_active = nullptr, *d_order = nullptr, *d_level = nullptr;
+    int* d_queue = nullptr;  // work-queue counters of the persistent kernels
     int *d_nb[8] = {nullptr};  // LeftBot LeftTop RightBot RightTop TopLeft TopRight BotLeft BotRight
     void *d_h[2] = {nullptr, nullptr}, *d_u[2] = {nullptr, nullptr}, *d_v[2] = {nullptr, nullptr};
     void *d_zb = nullptr, *d_cf = nullptr, *d_dh = nullptr, *d_dhu = nullptr, *d_dhv = nullptr;
@@ -90,7 +91,7 @@ namespace {
 
 void free_handle(bgf_handle h) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
-    F(h->d_active); F(h->d_order); F(h->d_level);
+    F(h->d_active); F(h->d_order); F(h->d_level); F(h->d_queue);
     for (int k = 0; k < 8; k++) F(h->d_nb[k]);
     for (int k = 0; k < 2; k++) { F(h->d_h[k]); F(h->d_u[k]); F(h->d_v[k]); }
     F(h->d_zb); F(h->d_cf); F(h->d_dh); F(h->d_dhu); F(h->d_dhv);
@@ -432,6 +433,7 @@ bgf_status create_handle(const bgf_param* p, const bgf_blocks* b, int cap, bgf_h
     bool ok = true;
     ok = ok && A((void**)&h->d_active, sizeof(int) * cap);
     ok = ok && A((void**)&h->d_order, sizeof(int) * cap);
+    ok = ok && A((void**)&h->d_queue, sizeof(int) * 2 * bgf::NXCD * bgf::QSTRIDE);
     ok = ok && A((void**)&h->d_level, sizeof(int) * cap);
     for (int k = 0; k < 8; k++) ok = ok && A((void**)&h->d_nb[k], sizeof(int) * cap);
     for (int k = 0; k < 2; k++) {
@@ -502,6 +504,7 @@ bgf_status create_handle(const bgf_param* p, const bgf_blocks* b, int cap, bgf_h
     }
     kp.active = h->d_active;
     kp.order = h->d_order;
+    kp.queue = h->d_queue;
     kp.LeftBot = h->d_nb[0];
     kp.RightBot = h->d_nb[2];
     kp.TopLeft = h->d_nb[4];

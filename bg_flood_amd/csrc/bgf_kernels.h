@@ -54,6 +54,7 @@ struct KParams {
     double fc, mask, rain_mmh;
     int frictionmodel, aoibnd, ForceMassConserve, rain_on, nblk, wetdryfix;
     int grid_pred, grid_corr;  // persistent workgroup counts (multiples of 8)
+    int* queue;                // the persistent kernels' 16 work-queue counters, QSTRIDE apart (zeroed at allocation)
     const int* active;  // active blocks in the reference order
     const int* order;   // the same blocks in the persistent kernels' processing order
     const int* LeftBot;
@@ -113,6 +114,9 @@ struct KParams {
 // the tendencies cm, fm of the faces below and above, and the Coriolis factor.
 enum { MET_CMX = 0, MET_CMY = 16, MET_FMY = 33, MET_CME = 50, MET_FMV = 66, MET_FMVP = 82, MET_FC = 98,
        METRIC_STRIDE = 128 };
+
+constexpr int NXCD = 8;     // XCDs of an MI355X, each with its own L2
+constexpr int QSTRIDE = 64; // ints between work-queue counters (a 256-byte line each)
 
 template <class T> struct StepBuffers {
     const T* h;

@@ -101,18 +101,6 @@ template <class T> __device__ __forceinline__ T minmod2c(T th, T half, T s0, T s
     return dmax(mn, T(0.0)) + dmin(mx, T(0.0));
 }
 
-// minmod2c of the sums z + h from the differences of each part: the slope of
-// zs = zb + h without forming zs, whose FP32 rounding (ulp of the bed
-// elevation) would swamp millimetre depths (deviation form).
-template <class T> __device__ __forceinline__ T minmod2s(T th, T half, T z0, T z1, T z2, T h0, T h1, T h2) {
-    T d1 = th * ((z1 - z0) + (h1 - h0));
-    T d2 = ((z2 - z0) + (h2 - h0)) * half;
-    T d3 = th * ((z2 - z1) + (h2 - h1));
-    T mn = dmin(dmin(d1, d2), d3);
-    T mx = dmax(dmax(d1, d2), d3);
-    return dmax(mn, T(0.0)) + dmin(mx, T(0.0));
-}
-
 // Constants of the step in T, computed once on the host (make_cst) and
 // passed by value, so kernels read them from scalar registers.
 template <class T> struct Cst {
@@ -246,6 +234,7 @@ template <class T> struct Tile {
     Q4<T> fx[16 * FP];  // (Fhu, Fqux, Su, Fqvx) at X faces
     Q4<T> fy[17 * FP];  // (Fhv, Fqvy, Sv, Fquy) at Y faces
     T red[4];
+    int next[4];  // queue positions, taken two iterations ahead by thread 0
 };
 
 // Strip slot k in [0, 128): waves 0 and 1 fetch the two-deep x strips (left,
@@ -273,16 +262,35 @@ __device__ __forceinline__ int strip_src(int k, int ib, int nL, int nR, int nB, 
     return memloc(j, sy, nb);
 }
 
+// FAST minmod of three: minmod(d1, d2, d3) = med3(d1, med3(d2, d3, 0), 0)
+// (two operands of one sign give their smaller magnitude, mixed signs give
+// 0), two v_med3_f32 instead of min3, max3, max, min and add.  Equal to
+// minmod2c for every non-NaN input up to the sign of a zero result.
+__device__ __forceinline__ float minmod3f(float d1, float d2, float d3) {
+    return __builtin_amdgcn_fmed3f(d1, __builtin_amdgcn_fmed3f(d2, d3, 0.0f), 0.0f);
+}
+__device__ __forceinline__ float minmod2f(float th, float half, float s0, float s1, float s2) {
+    return minmod3f(th * (s1 - s0), (s2 - s0) * half, th * (s2 - s1));
+}
+// minmod2f of the sums z + h from the differences of each part: the slope of
+// zs = zb + h without forming zs, whose FP32 rounding (ulp of the bed
+// elevation) would swamp millimetre depths (deviation form).
+__device__ __forceinline__ float minmod2sf(float th, float half, float z0, float z1, float z2, float h0, float h1,
+                                          float h2) {
+    return minmod3f(th * ((z1 - z0) + (h1 - h0)), ((z2 - z0) + (h2 - h0)) * half, th * ((z2 - z1) + (h2 - h1)));
+}
+
 template <class T, bool FAST, bool DEV = false>
 __device__ __forceinline__ Q4<T> slopes(const Cst<T>& C, const Q4<T>& m, const Q4<T>& c, const Q4<T>& p) {
     Q4<T> g;
-    if (FAST) {
+    if constexpr (FAST) {
+        static_assert(sizeof(T) == 4, "FAST kernels are FP32");
         // half-cell increments 0.5*minmod2: coefficients theta/2 and 1/4
-        g.a = minmod2c(C.th_s, C.half_s, m.a, c.a, p.a);
-        g.b = DEV ? minmod2s(C.th_s, C.half_s, m.b, c.b, p.b, m.a, c.a, p.a)
-                  : minmod2c(C.th_s, C.half_s, m.b, c.b, p.b);
-        g.c = minmod2c(C.th_s, C.half_s, m.c, c.c, p.c);
-        g.d = minmod2c(C.th_s, C.half_s, m.d, c.d, p.d);
+        g.a = minmod2f(C.th_s, C.half_s, m.a, c.a, p.a);
+        g.b = DEV ? minmod2sf(C.th_s, C.half_s, m.b, c.b, p.b, m.a, c.a, p.a)
+                  : minmod2f(C.th_s, C.half_s, m.b, c.b, p.b);
+        g.c = minmod2f(C.th_s, C.half_s, m.c, c.c, p.c);
+        g.d = minmod2f(C.th_s, C.half_s, m.d, c.d, p.d);
     } else {
         g.a = minmod2c(C.theta, C.half, m.a, c.a, p.a) / C.delta;
         g.b = minmod2c(C.theta, C.half, m.b, c.b, p.b) / C.delta;
@@ -292,14 +300,14 @@ __device__ __forceinline__ Q4<T> slopes(const Cst<T>& C, const Q4<T>& m, const Q
     return g;
 }
 
-// Extra work of a tile beyond one cell per thread goes to distinct waves, so
-// that no wave runs more than one extra section between two barriers: the x
-// halo-ring slopes to wave 0, the y halo-ring slopes to wave 1, the 17th X
-// faces to wave 2 and the 17th Y faces to wave 3.
-__device__ __forceinline__ int ring_slot(int t) { return t < 32 ? t : (t >= 64 && t < 96 ? t - 32 : -1); }
+// Extra work of a tile beyond one cell per thread goes to distinct waves, each
+// as one section with its lanes filled: the 64 halo-ring slopes to wave 2
+// and the 32 seventeenth faces to wave 3 (waves 0 and 1 fetch the strips, and
+// in the corrector also advance them).
 
 // Slopes of the tile: every thread its own cell along x and y; the halo ring
-// (x slopes at x = -1, 16 and y slopes at y = -1, 16) on waves 0 and 1.
+// on wave 2, lanes 0-31 the x slopes at x = -1, 16 and lanes 32-63 the y
+// slopes at y = -1, 16.
 template <class T, bool FAST, bool DEV>
 __device__ __forceinline__ void tile_gradients(Tile<T>& s, const Cst<T>& C, int t) {
     {
@@ -313,21 +321,16 @@ __device__ __forceinline__ void tile_gradients(Tile<T>& s, const Cst<T>& C, int 
         gs.a = g.a; gs.b = g.b; gs.c = g.d; gs.d = g.c;
         s.gy[gyi(x, y)] = gs;
     }
-    const int r = ring_slot(t);
-    if (r >= 0) {
-        const int j = r & 15, side = r >> 4;
-        if (side < 2) {
-            const int x = side == 0 ? -1 : BW;
-            const int c = tix(x, j);
-            s.gx[gxi(x, j)] = slopes<T, FAST, DEV>(C, s.q[c - 1], s.q[c], s.q[c + 1]);
-        } else {
-            const int y = side == 2 ? -1 : BW;
-            const int c = tix(j, y);
-            Q4<T> g = slopes<T, FAST, DEV>(C, s.q[c - TP], s.q[c], s.q[c + TP]);
-            Q4<T> gs;
-            gs.a = g.a; gs.b = g.b; gs.c = g.d; gs.d = g.c;
-            s.gy[gyi(j, y)] = gs;
-        }
+    if ((t >> 6) == 2) {
+        const int r = t & 63, j = r & 15;
+        const bool isx = r < 32;
+        const int o = (r & 16) ? BW : -1;
+        const int c = isx ? tix(o, j) : tix(j, o);
+        const int d = isx ? 1 : TP;
+        const Q4<T> g = slopes<T, FAST, DEV>(C, s.q[c - d], s.q[c], s.q[c + d]);
+        Q4<T> gs;
+        gs.a = g.a; gs.b = g.b; gs.c = isx ? g.c : g.d; gs.d = isx ? g.d : g.c;
+        *(isx ? &s.gx[gxi(o, j)] : &s.gy[gyi(j, o)]) = gs;
     }
 }
 
@@ -372,8 +375,20 @@ __device__ __forceinline__ T tile_fluxes(Tile<T>& s, const Cst<T>& C, int t, boo
     T dtx = T(0.0), dty = T(0.0), dtd;
     xface<T, FAST, WANT_DT, DEV>(s, C, x, y, dtx);
     bool wy = yface<T, FAST, WANT_DT, DEV>(s, C, x, y, dty);
-    if (t >= 128 && t < 144) xface<T, FAST, false, DEV>(s, C, BW, t - 128, dtd);
-    else if (t >= 192 && t < 208) yface<T, FAST, false, DEV>(s, C, t - 192, BW, dtd);
+    if (t >= 192 && t < 224) {
+        // the seventeenth faces on wave 3: lanes 0-15 the X faces at xf = 16,
+        // lanes 16-31 the Y faces at yf = 16
+        const int j = t & 15;
+        const bool isx = t < 208;
+        const int ci = isx ? tix(BW, j) : tix(j, BW);
+        const Q4<T> qi = s.q[ci], qn = s.q[ci - (isx ? 1 : TP)];
+        const Q4<T>* gi = isx ? &s.gx[gxi(BW, j)] : &s.gy[gyi(j, BW)];
+        const Q4<T> si = *gi, sn = *(gi - (isx ? 1 : GYP));
+        Q4<T> F;
+        kurg_face<T, FAST, false, DEV>(C, qi.a, qn.a, qi.b, qn.b, isx ? qi.c : qi.d, isx ? qn.c : qn.d,
+                                       isx ? qi.d : qi.c, isx ? qn.d : qn.c, si, sn, F, dtd);
+        *(isx ? &s.fx[fxi(BW, j)] : &s.fy[fyi(j, BW)]) = F;
+    }
     T dtc = T(0.0);
     if (WANT_DT) dtc = wy ? (dty < dtx ? dty : dtx) : C.inv_epsi;
     __syncthreads();
@@ -477,10 +492,9 @@ __device__ __forceinline__ void advance(const Cst<T>& C, int fmc, T dt, T hold, 
 }
 
 
-// Workgroups are dispatched round-robin over the eight XCDs, each with its
-// own L2: the persistent kernels give each XCD a contiguous range of blocks so
-// a block's neighbours are mostly fetched into the same L2.
-constexpr int NXCD = 8;
+// Workgroups are dispatched round-robin over the eight XCDs (NXCD), each with
+// its own L2: the persistent kernels give each XCD a contiguous range of
+// blocks so a block's neighbours are mostly fetched into the same L2.
 
 // Block indices are workgroup-uniform: read them through the constant address
 // space so they become scalar loads, which do not wait behind the vector
@@ -488,6 +502,40 @@ constexpr int NXCD = 8;
 __device__ __forceinline__ int ldu(const int* p, int i) {
     return ((const __attribute__((address_space(4))) int*)p)[i];
 }
+// Cell k of block ib of array a, with the block's base formed from the
+// (workgroup-uniform) block index in scalar registers and the cell's byte
+// offset kb = k * sizeof(T) a 32-bit per-thread value: every array's access
+// then takes the same 32-bit offset register, with no 64-bit address
+// arithmetic per array.  The empty asm makes the block base opaque, so the
+// compiler cannot re-associate it into loop-invariant per-thread 64-bit
+// addresses (two VGPRs per array); the global address space is restated
+// after it.
+typedef __attribute__((address_space(1))) char gchar;
+template <class T> __device__ __forceinline__ gchar* blk(const T* a, int ib) {
+    const T* p = a + (size_t)(unsigned)ib * BS;
+    asm("" : "+s"(p));
+    return (gchar*)p;
+}
+__device__ __forceinline__ unsigned vof(unsigned kb) {
+    asm("" : "+v"(kb));  // the 32-bit offset's zero-extension stays beside each access
+    return kb;
+}
+template <class T> __device__ __forceinline__ T ldc(const T* a, int ib, unsigned kb) {
+    return *(const __attribute__((address_space(1))) T*)(blk(a, ib) + vof(kb));
+}
+template <class T> __device__ __forceinline__ void stc(T* a, int ib, unsigned kb, T v) {
+    *(__attribute__((address_space(1))) T*)(blk(a, ib) + vof(kb)) = v;
+}
+// Streaming forms for values no other workgroup of the kernel reads (the
+// outputs, and cf, which is read for the own cells only): non-temporal, so
+// they do not push the neighbour strips out of L2.
+template <class T> __device__ __forceinline__ void stn(T* a, int ib, unsigned kb, T v) {
+    __builtin_nontemporal_store(v, (__attribute__((address_space(1))) T*)(blk(a, ib) + vof(kb)));
+}
+template <class T> __device__ __forceinline__ T ldn(const T* a, int ib, unsigned kb) {
+    return __builtin_nontemporal_load((const __attribute__((address_space(1))) T*)(blk(a, ib) + vof(kb)));
+}
+
 struct Nb {
     int ib, nL, nR, nB, nT, sid;
 };
@@ -512,21 +560,42 @@ template <class T> __device__ __forceinline__ void wave_flush(DevLoop* L, T v, i
     if ((t & 63) == 0) slot_min(L, (blockIdx.x << 2) + (t >> 6), v);
 }
 
-// Block sequence of persistent workgroup w: the contiguous active-list range
-// of its XCD, strided by the number of workgroups on that XCD (nwg is a
-// multiple of NXCD, so every XCD has nwg / NXCD workgroups).
-struct Sched {
-    int first, stride, end;
+// Persistent workgroups take their blocks from a queue per XCD: the XCD's
+// contiguous range of the processing order is handed out one position at a
+// time by an atomic counter, so the blocks in flight on an XCD always form a
+// compact window of consecutive positions (whose neighbour strips are then
+// in that XCD's L2), however unevenly the workgroups progress.  A static
+// stride lets the workgroups drift apart over a launch and the window widen
+// beyond what L2 holds.
+//
+// queue[x * QSTRIDE] (x < 8) are the predictor's counters and
+// queue[(8 + x) * QSTRIDE] the corrector's, each in a line of its own.
+// Each kernel, when it runs, zeroes the other kernel's counters (the other
+// kernel is not running: both are on one stream), so every launch finds its
+// own at zero: the first launch from the allocation, later ones from the
+// other kernel's launch in between (the zeroing comes before a kernel's
+// early return, and a launch that returns early takes no position).
+struct Range {
+    int start, end;
 };
-__device__ __forceinline__ Sched schedule(int w, int nwg, int n) {
-    const int xcd = w % NXCD, k = w / NXCD;
+__device__ __forceinline__ Range xcd_range(int xcd, int n) {
     const int per = n / NXCD, rem = n % NXCD;
-    const int start = xcd * per + (xcd < rem ? xcd : rem);
-    Sched S;
-    S.first = start + k;
-    S.stride = nwg / NXCD;
-    S.end = start + per + (xcd < rem ? 1 : 0);
-    return S;
+    Range R;
+    R.start = xcd * per + (xcd < rem ? xcd : rem);
+    R.end = R.start + per + (xcd < rem ? 1 : 0);
+    return R;
+}
+// Queue position j: thread 0 takes it into slot j & 3 of the tile two
+// iterations before it is needed (so the atomic's latency is hidden behind a
+// whole block), and the workgroup reads it after a barrier.  Position j + 2
+// reuses position j-2's slot, read two barriers earlier.  Each workgroup's
+// positions increase (one thread's atomics on one counter are ordered), so
+// the first one past the range ends the walk and no position is lost.
+__device__ __forceinline__ void queue_take(int* q, int start, int* next, int j, int t) {
+    if (t == 0) next[j & 3] = start + __hip_atomic_fetch_add(q, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int queue_read(const int* next, int j) {
+    return __builtin_amdgcn_readfirstlane(next[j & 3]);
 }
 
 // Persistent predictor: each workgroup walks its block sequence, prefetching
@@ -540,43 +609,53 @@ __device__ __forceinline__ Sched schedule(int w, int nwg, int n) {
 // so their occupancy) down.
 template <class T, bool FAST, bool BATCH, bool DEV, bool EXT = false>
 __global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuffers<T> B, DevLoop* L) {
+    if (blockIdx.x == 0 && threadIdx.x < NXCD) P.queue[(NXCD + threadIdx.x) * QSTRIDE] = 0;  // the corrector's
     if (!BATCH && L->done) return;
     __shared__ Tile<T> s;
     const int t = threadIdx.x;
     const int x = t & 15, y = t >> 4;
-    const Sched S = schedule(blockIdx.x, gridDim.x, P.nblk);
+    const int xcd = blockIdx.x % NXCD;
+    const Range R = xcd_range(xcd, P.nblk);
+    int* const qc = P.queue + xcd * QSTRIDE;
     const int cs = strip_cell(t);
     T bmin = C.inv_epsi;
     int csid = -1;
     T ph = T(0), pz = T(0), pu = T(0), pv = T(0), sh = T(0), sz = T(0), su = T(0), sv = T(0);
+    const unsigned kb = (unsigned)memloc(x, y, 0) * sizeof(T);
     auto fetch = [&](const Nb& n) {
-        const int i = memloc(x, y, n.ib);
-        ph = B.h[i]; pz = B.zb[i]; pu = B.u[i]; pv = B.v[i];
+        ph = ldc(B.h, n.ib, kb); pz = ldc(B.zb, n.ib, kb); pu = ldc(B.u, n.ib, kb); pv = ldc(B.v, n.ib, kb);
         if (t < 128) {
             const int src = strip_src(t, n.ib, n.nL, n.nR, n.nB, n.nT);
             sh = B.h[src]; sz = B.zb[src]; su = B.u[src]; sv = B.v[src];
         }
     };
+    queue_take(qc, R.start, s.next, 0, t);
+    queue_take(qc, R.start, s.next, 1, t);
+    __syncthreads();
+    int pos = queue_read(s.next, 0);
+    int g = 1;  // the next position to read
     Nb cur;
-    if (S.first < S.end) {
-        cur = block_nb<BATCH>(P, S.first);
+    if (pos < R.end) {
+        cur = block_nb<BATCH>(P, pos);
         fetch(cur);
     }
-    for (int ibl = S.first; ibl < S.end; ibl += S.stride) {
+    for (int npos; pos < R.end; pos = npos) {
         {
-            Q4<T> q;
-            q.a = ph; q.b = DEV ? pz : pz + ph; q.c = pu; q.d = pv;
-            s.q[tix(x, y)] = q;
+            Q4<T> qv;
+            qv.a = ph; qv.b = DEV ? pz : pz + ph; qv.c = pu; qv.d = pv;
+            s.q[tix(x, y)] = qv;
             if (t < 128) {
                 Q4<T> r;
                 r.a = sh; r.b = DEV ? sz : sz + sh; r.c = su; r.d = sv;
                 s.q[cs] = r;
             }
         }
+        queue_take(qc, R.start, s.next, g + 1, t);
         __syncthreads();
         const Nb n = cur;
-        if (ibl + S.stride < S.end) {
-            cur = block_nb<BATCH>(P, ibl + S.stride);
+        npos = queue_read(s.next, g++);
+        if (npos < R.end) {
+            cur = block_nb<BATCH>(P, npos);
             fetch(cur);
         }
         if (BATCH) {
@@ -595,10 +674,9 @@ __global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuff
             open_sides<T>(s, C, P, t, n.ib, n.nL, n.nR, n.nB, n.nT, L->totaltime + L->dt * 0.5, B.h, B.u, B.v, B.zb, dh,
                           dhu, dhv);
         if (EXT && P.nriver) dh = river_inflow<T>(P, n.ib, x, y, dh);
-        const int i = memloc(x, y, n.ib);
-        B.dh[i] = dh;
-        B.dhu[i] = dhu;
-        B.dhv[i] = dhv;
+        stn(B.dh, n.ib, kb, dh);
+        stn(B.dhu, n.ib, kb, dhu);
+        stn(B.dhv, n.ib, kb, dhv);
         bmin = dtc < bmin ? dtc : bmin;
     }
     if (BATCH) {
@@ -622,35 +700,48 @@ template <class T> __device__ __forceinline__ T kahan_add(T h, T d, T& c) {
 // of rain and of slow flows are not lost to FP32 rounding of h.
 template <class T, bool FAST, bool BATCH, bool COMP, bool EXT = false>
 __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuffers<T> B, DevLoop* L) {
+    if (blockIdx.x == 0 && threadIdx.x < NXCD) P.queue[threadIdx.x * QSTRIDE] = 0;  // the predictor's
     if (!BATCH && !L->go) return;
     __shared__ Tile<T> s;
     const int t = threadIdx.x;
     const int x = t & 15, y = t >> 4;
-    const Sched S = schedule(blockIdx.x, gridDim.x, P.nblk);
+    const int xcd = blockIdx.x % NXCD;
+    const Range R = xcd_range(xcd, P.nblk);
+    int* const qc = P.queue + (NXCD + xcd) * QSTRIDE;
     const int cs = strip_cell(t);
     const int fmc = P.ForceMassConserve;
     T ph = T(0), pu = T(0), pv = T(0), pdh = T(0), pdhu = T(0), pdhv = T(0), pz = T(0), pcf = T(0);
     T sh = T(0), su = T(0), sv = T(0), sdh = T(0), sdhu = T(0), sdhv = T(0), sz = T(0);
+    const unsigned kb = (unsigned)memloc(x, y, 0) * sizeof(T);
     auto fetch = [&](const Nb& n) {
-        const int i = memloc(x, y, n.ib);
-        ph = B.h[i]; pu = B.u[i]; pv = B.v[i]; pdh = B.dh[i]; pdhu = B.dhu[i]; pdhv = B.dhv[i];
-        pz = B.zb[i]; pcf = B.cf[i];
+        const int ib = n.ib;
+        ph = ldc(B.h, ib, kb); pu = ldc(B.u, ib, kb); pv = ldc(B.v, ib, kb);
+        pdh = ldc(B.dh, ib, kb); pdhu = ldc(B.dhu, ib, kb); pdhv = ldc(B.dhv, ib, kb);
+        pz = ldc(B.zb, ib, kb); pcf = ldn(B.cf, ib, kb);
         if (t < 128) {
             const int src = strip_src(t, n.ib, n.nL, n.nR, n.nB, n.nT);
             sh = B.h[src]; su = B.u[src]; sv = B.v[src]; sdh = B.dh[src]; sdhu = B.dhu[src]; sdhv = B.dhv[src];
             sz = B.zb[src];
         }
     };
+    queue_take(qc, R.start, s.next, 0, t);
+    queue_take(qc, R.start, s.next, 1, t);
+    __syncthreads();
+    int pos = queue_read(s.next, 0);
+    int g = 1;  // the next position to read
     Nb cur;
-    if (S.first < S.end) {
-        cur = block_nb<BATCH>(P, S.first);
+    if (pos < R.end) {
+        cur = block_nb<BATCH>(P, pos);
         fetch(cur);
     }
-    for (int ibl = S.first; ibl < S.end; ibl += S.stride) {
+    for (int npos; pos < R.end; pos = npos) {
         const DevLoop* Lm = L + cur.sid;
         if (BATCH && !ldu(&Lm->go, 0)) {
-            if (ibl + S.stride < S.end) {
-                cur = block_nb<BATCH>(P, ibl + S.stride);
+            queue_take(qc, R.start, s.next, g + 1, t);
+            __syncthreads();
+            npos = queue_read(s.next, g++);
+            if (npos < R.end) {
+                cur = block_nb<BATCH>(P, npos);
                 fetch(cur);
             }
             continue;
@@ -675,10 +766,12 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
                 s.q[cs] = r;
             }
         }
+        queue_take(qc, R.start, s.next, g + 1, t);
         __syncthreads();
         const Nb n = cur;
-        if (ibl + S.stride < S.end) {
-            cur = block_nb<BATCH>(P, ibl + S.stride);
+        npos = queue_read(s.next, g++);
+        if (npos < R.end) {
+            cur = block_nb<BATCH>(P, npos);
             fetch(cur);
         }
         const int ib = n.ib, nL = n.nL, nR = n.nR, nB = n.nB, nT = n.nT;
@@ -690,8 +783,7 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
             open_sides<T>(s, C, P, t, ib, nL, nR, nB, nT, L->tcorr, B.h, B.u, B.v, B.zb, dh, dhu, dhv);
         if (EXT && P.nriver) dh = river_inflow<T>(P, ib, x, y, dh);
         T h1, u1, v1;
-        const int i = memloc(x, y, ib);
-        T comp = COMP ? B.ch[i] : T(0);
+        T comp = COMP ? ldc(B.ch, ib, kb) : T(0);
         if (COMP) {
             // advance() with the depth increment added through the compensated sum
             T edt = dt;
@@ -747,13 +839,14 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
                 v1 = v1 * qvol;
             }
         }
-        B.ho[i] = h1;
-        B.uo[i] = u1;
-        B.vo[i] = v1;
-        if (COMP) B.ch[i] = comp;
+        stn(B.ho, ib, kb, h1);
+        stn(B.uo, ib, kb, u1);
+        stn(B.vo, ib, kb, v1);
+        if (COMP) stc(B.ch, ib, kb, comp);
         if (P.hmax) {
             const float hv = (float)h1;
-            if (hv > P.hmax[i]) P.hmax[i] = hv;
+            const unsigned kf = (unsigned)memloc(x, y, 0) * 4u;
+            if (hv > ldc(P.hmax, ib, kf)) stc(P.hmax, ib, kf, hv);
         }
     }
 }
