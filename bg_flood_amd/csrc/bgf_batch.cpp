@@ -47,6 +47,7 @@ struct bgf_batch_s {
     std::vector<bgf::DevLoop> loops;  // host mirror of the device loop states
     int *d_active = nullptr, *d_order = nullptr, *d_sid = nullptr, *d_level = nullptr;
     int* d_queue = nullptr;  // work-queue counters of the persistent kernels
+    int* d_rec = nullptr;    // per-position block records of the processing order
     int* d_nb[8] = {nullptr};
     void *d_h[2] = {nullptr, nullptr}, *d_u[2] = {nullptr, nullptr}, *d_v[2] = {nullptr, nullptr};
     void *d_zb = nullptr, *d_cf = nullptr, *d_dh = nullptr, *d_dhu = nullptr, *d_dhv = nullptr, *d_hmax = nullptr;
@@ -82,7 +83,7 @@ constexpr int NSAMP = 2048;  // gauge samples kept per member
 
 void free_batch(bgf_batch b) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
-    F(b->d_active); F(b->d_order); F(b->d_sid); F(b->d_level); F(b->d_queue);
+    F(b->d_active); F(b->d_order); F(b->d_sid); F(b->d_level); F(b->d_queue); F(b->d_rec);
     for (int k = 0; k < 8; k++) F(b->d_nb[k]);
     for (int k = 0; k < 2; k++) { F(b->d_h[k]); F(b->d_u[k]); F(b->d_v[k]); }
     F(b->d_zb); F(b->d_cf); F(b->d_dh); F(b->d_dhu); F(b->d_dhv); F(b->d_hmax); F(b->d_ch);
@@ -148,6 +149,8 @@ bgf_status prepare_run(bgf_batch b) {
         b->krun.grid_pred = bgf::persistent_grid(b->prec, 0, b->krun.nblk, b->wg_pred);
         b->krun.grid_corr = bgf::persistent_grid(b->prec, 1, b->krun.nblk, b->wg_corr);
         if (b->krun.grid_pred <= 0 || b->krun.grid_corr <= 0) return fail(BGF_ERR_HIP, "occupancy query failed");
+        bgf::launch_records(b->krun, b->d_rec, b->stream);
+        HIPCHK(hipGetLastError());
     }
     b->run_members = running;
     b->run_valid = true;
@@ -233,7 +236,8 @@ bgf_status bgf_batch_create(const bgf_param* common, int max_blocks, int max_mem
     };
     bool ok = A((void**)&b->d_active, sizeof(int) * max_blocks) && A((void**)&b->d_order, sizeof(int) * max_blocks) &&
               A((void**)&b->d_sid, sizeof(int) * max_blocks) && A((void**)&b->d_level, sizeof(int) * max_blocks) &&
-              A((void**)&b->d_queue, sizeof(int) * 2 * bgf::NXCD * bgf::QSTRIDE);
+              A((void**)&b->d_queue, sizeof(int) * 2 * bgf::NXCD * bgf::QSTRIDE) &&
+              A((void**)&b->d_rec, sizeof(int) * 8 * (size_t)max_blocks);
     for (int k = 0; k < 8; k++) ok = ok && A((void**)&b->d_nb[k], sizeof(int) * max_blocks);
     for (int k = 0; k < 2; k++)
         ok = ok && A(&b->d_h[k], b->es * b->S) && A(&b->d_u[k], b->es * b->S) && A(&b->d_v[k], b->es * b->S);
@@ -270,6 +274,7 @@ bgf_status bgf_batch_create(const bgf_param* common, int max_blocks, int max_mem
     kp.wetdryfix = p.wetdryfix;
     kp.active = b->d_active;
     kp.order = b->d_order;
+    kp.rec = b->d_rec;
     kp.queue = b->d_queue;
     kp.LeftBot = b->d_nb[0]; kp.LeftTop = b->d_nb[1]; kp.RightBot = b->d_nb[2]; kp.RightTop = b->d_nb[3];
     kp.TopLeft = b->d_nb[4]; kp.TopRight = b->d_nb[5]; kp.BotLeft = b->d_nb[6]; kp.BotRight = b->d_nb[7];

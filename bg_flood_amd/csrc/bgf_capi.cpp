@@ -30,6 +30,7 @@ struct bgf_handle_s {
     std::vector<int> active;
     int *d_active = nullptr, *d_order = nullptr, *d_level = nullptr;
     int* d_queue = nullptr;  // work-queue counters of the persistent kernels
+    int* d_rec = nullptr;    // per-position block records of the processing order
     int *d_nb[8] = {nullptr};  // LeftBot LeftTop RightBot RightTop TopLeft TopRight BotLeft BotRight
     void *d_h[2] = {nullptr, nullptr}, *d_u[2] = {nullptr, nullptr}, *d_v[2] = {nullptr, nullptr};
     void *d_zb = nullptr, *d_cf = nullptr, *d_dh = nullptr, *d_dhu = nullptr, *d_dhv = nullptr;
@@ -91,7 +92,7 @@ namespace {
 
 void free_handle(bgf_handle h) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
-    F(h->d_active); F(h->d_order); F(h->d_level); F(h->d_queue);
+    F(h->d_active); F(h->d_order); F(h->d_level); F(h->d_queue); F(h->d_rec);
     for (int k = 0; k < 8; k++) F(h->d_nb[k]);
     for (int k = 0; k < 2; k++) { F(h->d_h[k]); F(h->d_u[k]); F(h->d_v[k]); }
     F(h->d_zb); F(h->d_cf); F(h->d_dh); F(h->d_dhu); F(h->d_dhv);
@@ -433,6 +434,7 @@ bgf_status create_handle(const bgf_param* p, const bgf_blocks* b, int cap, bgf_h
     bool ok = true;
     ok = ok && A((void**)&h->d_active, sizeof(int) * cap);
     ok = ok && A((void**)&h->d_order, sizeof(int) * cap);
+    ok = ok && A((void**)&h->d_rec, sizeof(int) * 8 * (size_t)cap);
     ok = ok && A((void**)&h->d_queue, sizeof(int) * 2 * bgf::NXCD * bgf::QSTRIDE);
     ok = ok && A((void**)&h->d_level, sizeof(int) * cap);
     for (int k = 0; k < 8; k++) ok = ok && A((void**)&h->d_nb[k], sizeof(int) * cap);
@@ -504,6 +506,7 @@ bgf_status create_handle(const bgf_param* p, const bgf_blocks* b, int cap, bgf_h
     }
     kp.active = h->d_active;
     kp.order = h->d_order;
+    kp.rec = h->d_rec;
     kp.queue = h->d_queue;
     kp.LeftBot = h->d_nb[0];
     kp.RightBot = h->d_nb[2];
@@ -523,6 +526,11 @@ bgf_status create_handle(const bgf_param* p, const bgf_blocks* b, int cap, bgf_h
     kp.bxo = h->d_blkxo;
     kp.byo = h->d_blkyo;
     for (int k = 0; k < 4; k++) kp.btype[k] = p->aoibnd;
+    bgf::launch_records(kp, h->d_rec, h->stream);
+    if (hipGetLastError() != hipSuccess) {
+        free_handle(h);
+        return fail(BGF_ERR_HIP, "block records launch failed");
+    }
     *out = h;
     g_err.clear();
     return BGF_OK;
@@ -996,6 +1004,8 @@ bgf_status bgf_remesh(bgf_handle h, const bgf_param* p, const bgf_blocks* b, con
     kp.byo = h->d_blkyo;
     std::vector<int> order = processing_order(h->xy, h->tile_x, h->tile_y);
     ok = hipMemcpyAsync(h->d_order, order.data(), sizeof(int) * q.nblk, hipMemcpyHostToDevice, h->stream) == hipSuccess;
+    bgf::launch_records(kp, h->d_rec, h->stream);
+    ok = ok && hipGetLastError() == hipSuccess;
     if (q.spherical) ok = ok && upload_metric(h, &q, b, nbm);
     if (!ok) return fail(BGF_ERR_HIP, "remesh: upload failed");
     kp.grid_pred = bgf::persistent_grid(h->prec, 0, q.nblk, h->wg_pred);
@@ -1068,6 +1078,8 @@ bgf_status bgf_set_schedule(bgf_handle h, int tile_x, int tile_y, int wg_pred, i
         std::vector<int> order = processing_order(h->xy, tx, ty);
         HIPCHK(hipStreamSynchronize(h->stream));
         HIPCHK(hipMemcpy(h->d_order, order.data(), sizeof(int) * h->nblk, hipMemcpyHostToDevice));
+        bgf::launch_records(h->kp, h->d_rec, h->stream);
+        HIPCHK(hipGetLastError());
         h->tile_x = tx;
         h->tile_y = ty;
     }

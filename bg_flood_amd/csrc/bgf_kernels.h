@@ -57,6 +57,7 @@ struct KParams {
     int* queue;                // the persistent kernels' 16 work-queue counters, QSTRIDE apart (zeroed at allocation)
     const int* active;  // active blocks in the reference order
     const int* order;   // the same blocks in the persistent kernels' processing order
+    const int* rec;     // per position of the order: block, LeftBot, RightBot, BotLeft, TopLeft, member, 0, 0
     const int* LeftBot;
     const int* RightBot;
     const int* BotLeft;
@@ -141,6 +142,9 @@ void launch_step(int precision, const KParams& P, const void* buffers, DevLoop* 
 // Workgroups of a persistent launch; wg_per_cu > 0 caps the resident workgroups per CU.
 int persistent_grid(int precision, int corrector, int nblk, int wg_per_cu);
 void launch_init_dt(int precision, const KParams& P, const void* h, DevLoop* L, hipStream_t st);
+// Fills rec (8 ints per position, 32-byte aligned) from P.order, the neighbour arrays and P.sid;
+// called whenever the order, the mesh or the membership changes.
+void launch_records(const KParams& P, int* rec, hipStream_t st);
 // Initial dt of one member of a batch: P restricted to the member's blocks.
 void launch_init_dt_member(int precision, const KParams& P, const void* h, DevLoop* L, int member, hipStream_t st);
 // hmax = max(hmax, h) over the interior cells of P's blocks.

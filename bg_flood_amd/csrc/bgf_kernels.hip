@@ -539,15 +539,32 @@ template <class T> __device__ __forceinline__ T ldn(const T* a, int ib, unsigned
 struct Nb {
     int ib, nL, nR, nB, nT, sid;
 };
-template <bool BATCH> __device__ __forceinline__ Nb block_nb(const KParams& P, int bl) {
+// The block at position pos of the processing order with its neighbours and
+// member, one 32-byte scalar load of the position's record (launch_records).
+typedef int Rec8 __attribute__((ext_vector_type(8)));
+template <bool BATCH> __device__ __forceinline__ Nb block_nb(const KParams& P, int pos) {
+    const Rec8 r = ((const __attribute__((address_space(4))) Rec8*)P.rec)[pos];
     Nb n;
-    n.ib = ldu(P.order, bl);
-    n.sid = BATCH ? ldu(P.sid, n.ib) : 0;
-    n.nL = ldu(P.LeftBot, n.ib);
-    n.nR = ldu(P.RightBot, n.ib);
-    n.nB = ldu(P.BotLeft, n.ib);
-    n.nT = ldu(P.TopLeft, n.ib);
+    n.ib = r[0];
+    n.nL = r[1];
+    n.nR = r[2];
+    n.nB = r[3];
+    n.nT = r[4];
+    n.sid = BATCH ? r[5] : 0;
     return n;
+}
+
+__global__ __launch_bounds__(256) void k_records(KParams P, int* rec) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= P.nblk) return;
+    const int ib = P.order[k];
+    int4* r = (int4*)(rec + 8 * (size_t)k);
+    r[0] = make_int4(ib, P.LeftBot[ib], P.RightBot[ib], P.BotLeft[ib]);
+    r[1] = make_int4(P.TopLeft[ib], P.sid ? P.sid[ib] : 0, 0, 0);
+}
+
+void launch_records(const KParams& P, int* rec, hipStream_t st) {
+    if (P.nblk > 0) hipLaunchKernelGGL(k_records, dim3((P.nblk + 255) / 256), dim3(256), 0, st, P, rec);
 }
 
 // Merges a member's partial minimum into its slots: one atomic per wave, no
