@@ -1,0 +1,796 @@
+// bgf_ncio.cpp -- NetCDF files without libnetcdf: the BUQ map output of
+// BG_Flood and the reading of gridded NetCDF variables (bathymetry, hot
+// start), in the classic format with 64-bit offsets (CDF-2).
+//
+// The reference writes NETCDF4 (HDF5) files through libnetcdf
+// (src/Write_netcdf.cu:128), which is not available here; the classic format
+// carries the same data model for these files: the same dimensions,
+// variables, types, attributes and values (only the container differs, and
+// the reference's deflate setting, src/Write_netcdf.cu:780-784, has no
+// classic-format counterpart).  The output follows:
+//   creatncfileBUQ (src/Write_netcdf.cu:118-500): global attributes maxlevel,
+//     minlevel, xmin, xmax, ymin, ymax, Conventions, BG_Flood; dimensions
+//     time (unlimited) and blockid; variables time, crs, BGFlood (parameters
+//     as attributes, src/Write_txtlog.cpp:242-330), blockid, blockxo, blockyo,
+//     blockwidth, blocklevel, blockstatus; per level l the dimensions and
+//     coordinate variables xx_P<l>, yy_P<l> (N<l> for negative levels) with
+//     nx = ftoi((xmax - xo) / dx_l) cells centred at xo + dx_l/2 + i dx_l
+//     (Calcnxnyzone, :55-66 and :466-470);
+//   defncvarBUQ (:560-840): per output variable and level a variable
+//     <var>_P<l>(time, yy_P<l>, xx_P<l>), float, or short with scale_factor /
+//     add_offset when smallnc (values round((v - offset) / scale)), with
+//     _FillValue and missingvalue 9.9692e36 (or its short image), and
+//     standard_name, long_name, units, grid_mapping attributes; each block's
+//     16x16 interior goes to start (round((yo + blockyo - yymin) / dx_l),
+//     round((xo + blockxo - xxmin) / dx_l)) with xxmin = xo + dx_l / 2, and
+//     cells no block of that level covers keep the fill value;
+//   writenctimestep / writencvarstepBUQ (:1215-1400): each output appends a
+//     record: time (stored as float) and every variable's grids.
+//
+// Format: NetCDF classic file format specification (Unidata), version 2:
+// big-endian header of dimensions, attributes and variables with 64-bit
+// begin offsets, fixed-size variables first, then the records, each holding
+// one slab of every record variable padded to 4 bytes.
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "bgf_host.h"
+
+namespace {
+using namespace bgf_host;
+
+enum { T_BYTE = 1, T_CHAR = 2, T_SHORT = 3, T_INT = 4, T_FLOAT = 5, T_DOUBLE = 6 };
+int tsize(int t) { return t == T_SHORT ? 2 : t == T_INT || t == T_FLOAT ? 4 : t == T_DOUBLE ? 8 : 1; }
+
+// default fill values of the netCDF library (netcdf.h NC_FILL_*)
+constexpr float kFillF = 9.9692099683868690e+36f;
+constexpr double kFillD = 9.9692099683868690e+36;
+
+struct Att {
+    std::string name;
+    int type = T_INT;
+    size_t n = 0;
+    std::vector<uint8_t> be;  // values in file (big-endian) order
+};
+struct Dim {
+    std::string name;
+    uint64_t len = 0;  // 0: the unlimited (record) dimension
+};
+struct Var {
+    std::string name;
+    std::vector<int> dims;
+    std::vector<Att> atts;
+    int type = T_FLOAT;
+    bool rec = false;
+    uint64_t nelem = 0;  // elements of the whole variable, or of one record's slab
+    uint64_t vsize = 0;  // bytes, padded to 4
+    uint64_t begin = 0;
+};
+struct Schema {
+    std::vector<Dim> dims;
+    std::vector<Att> gatts;
+    std::vector<Var> vars;
+    uint64_t numrecs = 0;
+    uint64_t recsize = 0;
+    uint64_t recbegin = 0;
+    int find(const std::string& name) const {
+        for (size_t k = 0; k < vars.size(); k++)
+            if (vars[k].name == name) return (int)k;
+        return -1;
+    }
+};
+
+// ---- big-endian encoding ----------------------------------------------------
+void be32(std::vector<uint8_t>& o, uint32_t v) {
+    for (int s = 24; s >= 0; s -= 8) o.push_back(uint8_t(v >> s));
+}
+void be64(std::vector<uint8_t>& o, uint64_t v) {
+    for (int s = 56; s >= 0; s -= 8) o.push_back(uint8_t(v >> s));
+}
+void pad4(std::vector<uint8_t>& o) {
+    while (o.size() % 4) o.push_back(0);
+}
+void bename(std::vector<uint8_t>& o, const std::string& s) {
+    be32(o, (uint32_t)s.size());
+    o.insert(o.end(), s.begin(), s.end());
+    pad4(o);
+}
+template <class T> void put_be(uint8_t* p, T v) {
+    uint8_t b[sizeof(T)];
+    std::memcpy(b, &v, sizeof(T));
+    for (size_t k = 0; k < sizeof(T); k++) p[k] = b[sizeof(T) - 1 - k];
+}
+template <class T> T get_be(const uint8_t* p) {
+    uint8_t b[sizeof(T)];
+    for (size_t k = 0; k < sizeof(T); k++) b[k] = p[sizeof(T) - 1 - k];
+    T v;
+    std::memcpy(&v, b, sizeof(T));
+    return v;
+}
+uint32_t rd32(const uint8_t* p) { return get_be<uint32_t>(p); }
+uint64_t rd64(const uint8_t* p) { return get_be<uint64_t>(p); }
+
+Att att_text(const std::string& name, const std::string& v) {
+    Att a;
+    a.name = name;
+    a.type = T_CHAR;
+    a.n = v.size();
+    a.be.assign(v.begin(), v.end());
+    return a;
+}
+template <class T> Att att_num(const std::string& name, int type, T v) {
+    Att a;
+    a.name = name;
+    a.type = type;
+    a.n = 1;
+    a.be.resize(sizeof(T));
+    put_be<T>(a.be.data(), v);
+    return a;
+}
+Att att_int(const std::string& n, int v) { return att_num<int32_t>(n, T_INT, v); }
+Att att_float(const std::string& n, float v) { return att_num<float>(n, T_FLOAT, v); }
+Att att_double(const std::string& n, double v) { return att_num<double>(n, T_DOUBLE, v); }
+Att att_short(const std::string& n, int16_t v) { return att_num<int16_t>(n, T_SHORT, v); }
+
+void beatts(std::vector<uint8_t>& o, const std::vector<Att>& atts) {
+    if (atts.empty()) {
+        be32(o, 0);
+        be32(o, 0);
+        return;
+    }
+    be32(o, 0x0C);
+    be32(o, (uint32_t)atts.size());
+    for (const Att& a : atts) {
+        bename(o, a.name);
+        be32(o, (uint32_t)a.type);
+        be32(o, (uint32_t)a.n);
+        o.insert(o.end(), a.be.begin(), a.be.end());
+        pad4(o);
+    }
+}
+
+std::vector<uint8_t> header(const Schema& S) {
+    std::vector<uint8_t> o = {'C', 'D', 'F', 2};
+    be32(o, (uint32_t)S.numrecs);
+    if (S.dims.empty()) {
+        be32(o, 0);
+        be32(o, 0);
+    } else {
+        be32(o, 0x0A);
+        be32(o, (uint32_t)S.dims.size());
+        for (const Dim& d : S.dims) {
+            bename(o, d.name);
+            be32(o, (uint32_t)d.len);
+        }
+    }
+    beatts(o, S.gatts);
+    if (S.vars.empty()) {
+        be32(o, 0);
+        be32(o, 0);
+    } else {
+        be32(o, 0x0B);
+        be32(o, (uint32_t)S.vars.size());
+        for (const Var& v : S.vars) {
+            bename(o, v.name);
+            be32(o, (uint32_t)v.dims.size());
+            for (int d : v.dims) be32(o, (uint32_t)d);
+            beatts(o, v.atts);
+            be32(o, (uint32_t)v.type);
+            be32(o, (uint32_t)std::min<uint64_t>(v.vsize, 0xFFFFFFFFull));
+            be64(o, v.begin);
+        }
+    }
+    return o;
+}
+
+// Element counts, sizes and offsets: fixed variables in definition order
+// after the header, then the records (a lone record variable is not padded).
+void layout(Schema& S) {
+    int nrecvars = 0;
+    for (Var& v : S.vars) {
+        v.rec = !v.dims.empty() && S.dims[v.dims[0]].len == 0;
+        v.nelem = 1;
+        for (size_t k = v.rec ? 1 : 0; k < v.dims.size(); k++) v.nelem *= S.dims[v.dims[k]].len;
+        v.vsize = (v.nelem * tsize(v.type) + 3) / 4 * 4;
+        nrecvars += v.rec;
+    }
+    uint64_t off = header(S).size();
+    for (Var& v : S.vars)
+        if (!v.rec) {
+            v.begin = off;
+            off += v.vsize;
+        }
+    S.recbegin = off;
+    S.recsize = 0;
+    for (Var& v : S.vars)
+        if (v.rec) {
+            v.begin = off + S.recsize;
+            S.recsize += nrecvars == 1 ? v.nelem * tsize(v.type) : v.vsize;
+        }
+}
+
+// Fill value of a variable in file order (its _FillValue, else the default).
+std::vector<uint8_t> fill_of(const Var& v) {
+    for (const Att& a : v.atts)
+        if (a.name == "_FillValue" && a.type == v.type && a.n == 1) return a.be;
+    std::vector<uint8_t> f(tsize(v.type));
+    switch (v.type) {
+        case T_SHORT: put_be<int16_t>(f.data(), -32767); break;
+        case T_INT: put_be<int32_t>(f.data(), -2147483647); break;
+        case T_FLOAT: put_be<float>(f.data(), kFillF); break;
+        case T_DOUBLE: put_be<double>(f.data(), kFillD); break;
+        case T_BYTE: f[0] = uint8_t(-127); break;
+        default: f[0] = 0;
+    }
+    return f;
+}
+
+std::vector<uint8_t> filled(const Var& v) {
+    const std::vector<uint8_t> f = fill_of(v);
+    std::vector<uint8_t> o(v.vsize, 0);
+    for (uint64_t k = 0; k < v.nelem; k++) std::memcpy(&o[k * f.size()], f.data(), f.size());
+    return o;
+}
+
+bool write_at(FILE* fp, uint64_t off, const void* p, size_t n) {
+    if (fseeko(fp, (off_t)off, SEEK_SET) != 0) return false;
+    return n == 0 || fwrite(p, 1, n, fp) == n;
+}
+
+// ---- reading ---------------------------------------------------------------
+struct Reader {
+    Schema S;
+    std::string err;
+    std::unique_ptr<FILE, int (*)(FILE*)> fp{nullptr, fclose};
+    int version = 0;
+
+    bool open(const char* path) {
+        fp.reset(fopen(path, "rb"));
+        if (!fp) return fail_("cannot open " + std::string(path));
+        std::vector<uint8_t> buf;
+        {
+            fseeko(fp.get(), 0, SEEK_END);
+            const off_t n = ftello(fp.get());
+            // the header is at most a few MB for the files handled here
+            const size_t want = (size_t)std::min<off_t>(n, (off_t)64 << 20);
+            buf.resize(want);
+            fseeko(fp.get(), 0, SEEK_SET);
+            if (fread(buf.data(), 1, want, fp.get()) != want) return fail_("short read");
+        }
+        size_t p = 0;
+        auto need = [&](size_t k) { return p + k <= buf.size(); };
+        if (!need(8) || buf[0] != 'C' || buf[1] != 'D' || buf[2] != 'F')
+            return fail_("not a NetCDF classic file (HDF5-based NetCDF-4 files are not read)");
+        version = buf[3];
+        if (version != 1 && version != 2 && version != 5) return fail_("unknown NetCDF version");
+        const bool v5 = version == 5;
+        p = 4;
+        auto rdn = [&](uint64_t& v) {
+            if (v5) {
+                if (!need(8)) return false;
+                v = rd64(&buf[p]);
+                p += 8;
+            } else {
+                if (!need(4)) return false;
+                v = rd32(&buf[p]);
+                p += 4;
+            }
+            return true;
+        };
+        auto rdname = [&](std::string& s) {
+            uint64_t n;
+            if (!rdn(n) || !need((n + 3) / 4 * 4)) return false;
+            s.assign((const char*)&buf[p], n);
+            p += (n + 3) / 4 * 4;
+            return true;
+        };
+        auto rdatts = [&](std::vector<Att>& atts) {
+            if (!need(4)) return false;
+            const uint32_t tag = rd32(&buf[p]);
+            p += 4;
+            uint64_t n;
+            if (!rdn(n)) return false;
+            if (tag == 0) return n == 0;
+            if (tag != 0x0C) return false;
+            for (uint64_t k = 0; k < n; k++) {
+                Att a;
+                uint64_t cnt;
+                if (!rdname(a.name) || !need(4)) return false;
+                a.type = (int)rd32(&buf[p]);
+                p += 4;
+                if (!rdn(cnt)) return false;
+                a.n = cnt;
+                const size_t bytes = cnt * tsize(a.type);
+                if (!need((bytes + 3) / 4 * 4)) return false;
+                a.be.assign(&buf[p], &buf[p] + bytes);
+                p += (bytes + 3) / 4 * 4;
+                atts.push_back(a);
+            }
+            return true;
+        };
+        uint64_t nrec;
+        if (!rdn(nrec)) return fail_("bad header");
+        S.numrecs = nrec;
+        {
+            if (!need(4)) return fail_("bad header");
+            const uint32_t tag = rd32(&buf[p]);
+            p += 4;
+            uint64_t n;
+            if (!rdn(n)) return fail_("bad header");
+            if (tag == 0x0A) {
+                for (uint64_t k = 0; k < n; k++) {
+                    Dim d;
+                    if (!rdname(d.name) || !rdn(d.len)) return fail_("bad dimension list");
+                    S.dims.push_back(d);
+                }
+            } else if (tag != 0 || n != 0) {
+                return fail_("bad dimension list");
+            }
+        }
+        if (!rdatts(S.gatts)) return fail_("bad global attributes");
+        {
+            if (!need(4)) return fail_("bad header");
+            const uint32_t tag = rd32(&buf[p]);
+            p += 4;
+            uint64_t n;
+            if (!rdn(n)) return fail_("bad header");
+            if (tag == 0x0B) {
+                for (uint64_t k = 0; k < n; k++) {
+                    Var v;
+                    uint64_t nd;
+                    if (!rdname(v.name) || !rdn(nd)) return fail_("bad variable list");
+                    for (uint64_t d = 0; d < nd; d++) {
+                        uint64_t id;
+                        if (!rdn(id) || id >= S.dims.size()) return fail_("bad variable dimensions");
+                        v.dims.push_back((int)id);
+                    }
+                    if (!rdatts(v.atts) || !need(4)) return fail_("bad variable attributes");
+                    v.type = (int)rd32(&buf[p]);
+                    p += 4;
+                    uint64_t vs;
+                    if (!rdn(vs)) return fail_("bad variable");
+                    if (version == 1) {
+                        if (!need(4)) return fail_("bad variable");
+                        v.begin = rd32(&buf[p]);
+                        p += 4;
+                    } else {
+                        if (!need(8)) return fail_("bad variable");
+                        v.begin = rd64(&buf[p]);
+                        p += 8;
+                    }
+                    S.vars.push_back(v);
+                }
+            } else if (tag != 0 || n != 0) {
+                return fail_("bad variable list");
+            }
+        }
+        int nrecvars = 0;
+        for (Var& v : S.vars) {
+            v.rec = !v.dims.empty() && S.dims[v.dims[0]].len == 0;
+            v.nelem = 1;
+            for (size_t k = v.rec ? 1 : 0; k < v.dims.size(); k++) v.nelem *= S.dims[v.dims[k]].len;
+            v.vsize = (v.nelem * tsize(v.type) + 3) / 4 * 4;
+            nrecvars += v.rec;
+        }
+        S.recsize = 0;
+        for (const Var& v : S.vars)
+            if (v.rec) S.recsize += nrecvars == 1 ? v.nelem * tsize(v.type) : v.vsize;
+        return true;
+    }
+    bool fail_(const std::string& m) {
+        err = m;
+        return false;
+    }
+    // one record's slab of a record variable (rec >= 0) or a whole fixed one, as doubles
+    bool get(int vi, long long rec, double* out) {
+        const Var& v = S.vars[vi];
+        if (v.rec && (rec < 0 || (uint64_t)rec >= S.numrecs)) return fail_("record out of range");
+        const uint64_t off = v.begin + (v.rec ? (uint64_t)rec * S.recsize : 0);
+        const size_t es = tsize(v.type);
+        std::vector<uint8_t> b(v.nelem * es);
+        if (fseeko(fp.get(), (off_t)off, SEEK_SET) != 0 || (b.size() && fread(b.data(), 1, b.size(), fp.get()) != b.size()))
+            return fail_("short read of " + v.name);
+        for (uint64_t k = 0; k < v.nelem; k++) {
+            const uint8_t* q = &b[k * es];
+            switch (v.type) {
+                case T_BYTE: out[k] = (double)(int8_t)q[0]; break;
+                case T_CHAR: out[k] = (double)q[0]; break;
+                case T_SHORT: out[k] = get_be<int16_t>(q); break;
+                case T_INT: out[k] = get_be<int32_t>(q); break;
+                case T_FLOAT: out[k] = get_be<float>(q); break;
+                default: out[k] = get_be<double>(q);
+            }
+        }
+        return true;
+    }
+};
+
+double att_value(const Att& a, size_t k) {
+    const uint8_t* q = &a.be[k * tsize(a.type)];
+    switch (a.type) {
+        case T_BYTE: return (double)(int8_t)q[0];
+        case T_CHAR: return (double)q[0];
+        case T_SHORT: return get_be<int16_t>(q);
+        case T_INT: return get_be<int32_t>(q);
+        case T_FLOAT: return get_be<float>(q);
+        default: return get_be<double>(q);
+    }
+}
+
+int ftoi_(double v) { return (int)v; }  // src/Util_CPU.cu ftoi: truncation toward zero
+
+std::string level_suffix(int lev) { return (lev < 0 ? "_N" : "_P") + std::to_string(lev < 0 ? -lev : lev); }
+
+// Metadata of the output variables (src/InitialConditions.cu:590-700).
+struct VarMeta {
+    const char* std_name;
+    const char* long_name;
+    const char* units;
+};
+VarMeta var_meta(const std::string& n) {
+    static const std::map<std::string, VarMeta> m = {
+        {"zb", {"ground_elevation_above_datum", "Ground elevation above datum", "m"}},
+        {"u", {"u_velocity", "Water velocity in x-direction", "m s-1"}},
+        {"v", {"v_velocity", "Velocity in y-direction", "m s-1"}},
+        {"zs", {"water_surface_elevation", "Water surface elevation above datum", "m"}},
+        {"h", {"water_depth", "Water depth", "m"}},
+        {"hmean", {"mean_water_depth", "Mean water depth since last output", "m"}},
+        {"hmax", {"maximum_water_depth", "Maximum water depth since simulation start", "m"}},
+        {"zsmean", {"mean_water_elevation", "Mean water elevation above datum since last output", "m"}},
+        {"zsmax", {"maximum_water_elevation", "Maximum water elevation above datum since simulation start", "m"}},
+        {"umean", {"mean_u_velocity", "Mean velocity in x-direction since last output", "m s-1"}},
+        {"umax", {"maximum_u_velocity", "Maximum velocity in x-direction since simulation start", "m s-1"}},
+        {"vmean", {"mean_v_velocity", "Mean velocity in y-direction since last output", "m s-1"}},
+        {"vmax", {"maximum_v_velocity", "Maximum velocity in y-direction since simulation start", "m s-1"}},
+        {"cf", {"bottom_friction_coefficient", "Bottom friction coefficient", "-"}},
+    };
+    auto it = m.find(n);
+    if (it != m.end()) return it->second;
+    return {"", "", ""};
+}
+
+}  // namespace
+
+struct bgf_ncout_s {
+    std::string path;
+    std::unique_ptr<FILE, int (*)(FILE*)> fp{nullptr, fclose};
+    Schema S;
+    std::vector<std::string> names;
+    int minlev = 0, maxlev = 0;
+    std::vector<int> nx, ny;           // per level
+    std::vector<int> blk, lev, sx, sy;  // per output block: index, level, grid start
+    int smallnc = 0;
+    float scale = 0.01f, offset = 0.0f;
+    int timevar = -1;
+    std::vector<int> vidx;  // variable index per (output variable, level)
+};
+
+extern "C" {
+
+bgf_status bgf_nc_buq_create(const char* path, const bgf_param* p, const bgf_blocks* b, double xmax, double ymax,
+                             int nvar, const char* const* names, int smallnc, float scalefactor, float addoffset,
+                             bgf_ncout* out) {
+    if (!path || !p || !b || !out || nvar < 0 || (nvar && !names))
+        return fail(BGF_ERR_INVALID_ARG, "bgf_nc_buq_create: null argument");
+    if (p->nblk <= 0 || !(xmax > p->xo) || !(ymax > p->yo) || p->minlevel > p->maxlevel)
+        return fail(BGF_ERR_INVALID_ARG, "bgf_nc_buq_create: empty zone or mesh");
+    if (smallnc && !(scalefactor != 0.0f)) return fail(BGF_ERR_INVALID_ARG, "smallnc needs a non-zero scale factor");
+    *out = nullptr;
+    auto F = std::make_unique<bgf_ncout_s>();
+    F->path = path;
+    F->minlev = p->minlevel;
+    F->maxlev = p->maxlevel;
+    F->smallnc = smallnc;
+    F->scale = scalefactor;
+    F->offset = addoffset;
+    for (int k = 0; k < nvar; k++) {
+        if (!names[k] || !*names[k]) return fail(BGF_ERR_INVALID_ARG, "empty output variable name");
+        F->names.push_back(names[k]);
+    }
+    Schema& S = F->S;
+    const int nb = p->nblk;
+    // global attributes (src/Write_netcdf.cu:188-211)
+    S.gatts.push_back(att_int("maxlevel", p->maxlevel));
+    S.gatts.push_back(att_int("minlevel", p->minlevel));
+    S.gatts.push_back(att_double("xmin", p->xo));
+    S.gatts.push_back(att_double("xmax", xmax));
+    S.gatts.push_back(att_double("ymin", p->yo));
+    S.gatts.push_back(att_double("ymax", ymax));
+    S.gatts.push_back(att_text("Conventions", "CF-1.11-draft"));
+    S.gatts.push_back(att_int("BG_Flood", 0));
+    S.dims.push_back({"time", 0});
+    S.dims.push_back({"blockid", (uint64_t)nb});
+    auto addvar = [&](const std::string& name, int type, std::vector<int> dims) {
+        Var v;
+        v.name = name;
+        v.type = type;
+        v.dims = std::move(dims);
+        S.vars.push_back(v);
+        return (int)S.vars.size() - 1;
+    };
+    F->timevar = addvar("time", T_FLOAT, {0});
+    S.vars[F->timevar].atts = {att_text("standard_name", "time"), att_text("units", "seconds since 2000-01-01T00:00:00"),
+                               att_text("axis", "T")};
+    const int crs = addvar("crs", T_INT, {});
+    if (p->spherical) {
+        S.vars[crs].atts = {att_text("grid_mapping_name", "latitude_longitude"),
+                            att_float("longitude_of_prime_meridian", 0.0f), att_float("semi_major_axis", 6378137.0f),
+                            att_float("inverse_flattening", 298.257223563f)};
+    } else {
+        // Param.crs_ref defaults to "no_crs" (src/Param.h:87)
+        S.vars[crs].atts = {att_text("grid_mapping_name", "projected"), att_text("crs_wkt", "no_crs"),
+                            att_text("spatial_ref", "no_crs")};
+    }
+    const int bgfv = addvar("BGFlood", T_INT, {});
+    S.vars[bgfv].atts = {att_double("g", p->g),           att_double("eps", p->eps),
+                         att_double("CFL", p->CFL),       att_double("theta", p->theta),
+                         att_int("frictionmodel", p->frictionmodel), att_double("lat", p->lat),
+                         att_int("wetdryfix", p->wetdryfix), att_int("doubleprecision", p->precision == BGF_FP64),
+                         att_int("engine", p->engine),    att_double("dx", p->dx),
+                         att_double("delta", p->delta),   att_double("xo", p->xo),
+                         att_double("yo", p->yo),         att_int("minlevel", p->minlevel),
+                         att_int("maxlevel", p->maxlevel), att_int("aoibnd", p->aoibnd),
+                         att_int("ForceMassConserve", p->ForceMassConserve), att_int("spherical", p->spherical),
+                         att_double("Radius", p->Radius), att_double("mask", p->mask)};
+    const int vblkid = addvar("blockid", T_INT, {1});
+    const int vbxo = addvar("blockxo", T_FLOAT, {1});
+    const int vbyo = addvar("blockyo", T_FLOAT, {1});
+    const int vbw = addvar("blockwidth", T_FLOAT, {1});
+    const int vblev = addvar("blocklevel", T_INT, {1});
+    addvar("blockstatus", T_INT, {1});
+    const int nlev = p->maxlevel - p->minlevel + 1;
+    std::vector<int> vxx(nlev), vyy(nlev);
+    for (int l = 0; l < nlev; l++) {
+        const int lev = p->minlevel + l;
+        const double ddx = level_dx(p->dx, lev);
+        F->nx.push_back(ftoi_((xmax - p->xo) / ddx));
+        F->ny.push_back(ftoi_((ymax - p->yo) / ddx));
+        if (F->nx.back() <= 0 || F->ny.back() <= 0) return fail(BGF_ERR_INVALID_ARG, "zone smaller than a cell");
+        S.dims.push_back({"xx" + level_suffix(lev), (uint64_t)F->nx.back()});
+        S.dims.push_back({"yy" + level_suffix(lev), (uint64_t)F->ny.back()});
+        const int dx_id = (int)S.dims.size() - 2, dy_id = (int)S.dims.size() - 1;
+        vxx[l] = addvar("xx" + level_suffix(lev), T_DOUBLE, {dx_id});
+        vyy[l] = addvar("yy" + level_suffix(lev), T_DOUBLE, {dy_id});
+        S.vars[vxx[l]].atts = {att_text("axis", "X")};
+        S.vars[vyy[l]].atts = {att_text("axis", "Y")};
+    }
+    // output variables, level by level (defncvarBUQ).  The reference's short
+    // fill value, (short)round((9.9692e36 - addoffset) / scalefactor)
+    // (src/Write_netcdf.cu:577), converts a value far outside the range of a
+    // short (undefined; 0 on x86-64, which would mark every zero as missing);
+    // the netCDF default short fill -32767 is used instead.
+    const int16_t fill_s = -32767;
+    for (const std::string& name : F->names) {
+        const VarMeta m = var_meta(name);
+        for (int l = 0; l < nlev; l++) {
+            const int lev = p->minlevel + l;
+            const int xd = 2 + 2 * l, yd = 3 + 2 * l;
+            const int vi = addvar(name + level_suffix(lev), smallnc ? T_SHORT : T_FLOAT, {0, yd, xd});
+            Var& v = S.vars[vi];
+            if (smallnc) {
+                v.atts.push_back(att_short("_FillValue", fill_s));
+                v.atts.push_back(att_short("missingvalue", fill_s));
+                v.atts.push_back(att_float("scale_factor", scalefactor));
+                v.atts.push_back(att_float("add_offset", addoffset));
+            } else {
+                v.atts.push_back(att_float("_FillValue", 9.9692e+36f));
+                v.atts.push_back(att_float("missingvalue", 9.9692e+36f));
+            }
+            v.atts.push_back(att_text("standard_name", m.std_name));
+            v.atts.push_back(att_text("long_name", m.long_name));
+            v.atts.push_back(att_text("units", m.units));
+            v.atts.push_back(att_text("grid_mapping", "crs"));
+            F->vidx.push_back(vi);
+        }
+    }
+    layout(S);
+    // block table and placement
+    for (int k = 0; k < nb; k++) {
+        const int ib = b->active[k];
+        if (ib < 0 || ib >= p->nblkmem) return fail(BGF_ERR_INVALID_ARG, "active block index out of range");
+        const int lev = b->level[ib];
+        if (lev < p->minlevel || lev > p->maxlevel) return fail(BGF_ERR_INVALID_ARG, "block level outside the range");
+        const double ddx = level_dx(p->dx, lev);
+        const double xxmin = p->xo + ddx / 2.0, yymin = p->yo + ddx / 2.0;
+        const long sy = std::lround((p->yo + b->yo[ib] - yymin) / ddx);
+        const long sx = std::lround((p->xo + b->xo[ib] - xxmin) / ddx);
+        const int l = lev - p->minlevel;
+        if (sx < 0 || sy < 0 || sx + BW > F->nx[l] || sy + BW > F->ny[l])
+            return fail(BGF_ERR_INVALID_ARG, "block " + std::to_string(ib) + " lies outside the output zone");
+        F->blk.push_back(ib);
+        F->lev.push_back(lev);
+        F->sx.push_back((int)sx);
+        F->sy.push_back((int)sy);
+    }
+    F->fp.reset(fopen(path, "wb+"));
+    if (!F->fp) return fail(BGF_ERR_INVALID_ARG, std::string("cannot create ") + path);
+    const std::vector<uint8_t> hdr = header(S);
+    bool ok = write_at(F->fp.get(), 0, hdr.data(), hdr.size());
+    // fixed variables: fill first, then the values the reference writes
+    for (const Var& v : S.vars)
+        if (!v.rec) {
+            const std::vector<uint8_t> f = filled(v);
+            ok = ok && write_at(F->fp.get(), v.begin, f.data(), f.size());
+        }
+    auto put_ints = [&](int vi, const std::vector<int32_t>& a) {
+        std::vector<uint8_t> o(a.size() * 4);
+        for (size_t k = 0; k < a.size(); k++) put_be<int32_t>(&o[4 * k], a[k]);
+        return write_at(F->fp.get(), S.vars[vi].begin, o.data(), o.size());
+    };
+    auto put_floats = [&](int vi, const std::vector<float>& a) {
+        std::vector<uint8_t> o(a.size() * 4);
+        for (size_t k = 0; k < a.size(); k++) put_be<float>(&o[4 * k], a[k]);
+        return write_at(F->fp.get(), S.vars[vi].begin, o.data(), o.size());
+    };
+    auto put_doubles = [&](int vi, const std::vector<double>& a) {
+        std::vector<uint8_t> o(a.size() * 8);
+        for (size_t k = 0; k < a.size(); k++) put_be<double>(&o[8 * k], a[k]);
+        return write_at(F->fp.get(), S.vars[vi].begin, o.data(), o.size());
+    };
+    std::vector<int32_t> ids(nb), levs(nb);
+    std::vector<float> bw(nb), bx(nb), by(nb);
+    for (int k = 0; k < nb; k++) {
+        const int ib = F->blk[k];
+        ids[k] = ib;
+        levs[k] = F->lev[k];
+        bw[k] = (float)level_dx(p->dx, F->lev[k]);
+        // float(T(XParam.xo) + blockxo[ib]) in the handle's precision
+        if (p->precision == BGF_FP64) {
+            bx[k] = (float)(p->xo + b->xo[ib]);
+            by[k] = (float)(p->yo + b->yo[ib]);
+        } else {
+            bx[k] = (float)((float)p->xo + (float)b->xo[ib]);
+            by[k] = (float)((float)p->yo + (float)b->yo[ib]);
+        }
+    }
+    ok = ok && put_ints(vblkid, ids) && put_floats(vbw, bw) && put_floats(vbxo, bx) && put_ints(vblev, levs) &&
+         put_floats(vbyo, by);
+    for (int l = 0; l < nlev; l++) {
+        const double ddx = level_dx(p->dx, p->minlevel + l), dxp = level_dx(p->dx, p->minlevel + l + 1);
+        std::vector<double> xv(F->nx[l]), yv(F->ny[l]);
+        for (int i = 0; i < F->nx[l]; i++) xv[i] = p->xo + dxp + double(i) * ddx;
+        for (int j = 0; j < F->ny[l]; j++) yv[j] = p->yo + dxp + double(j) * ddx;
+        ok = ok && put_doubles(vxx[l], xv) && put_doubles(vyy[l], yv);
+    }
+    if (!ok || fflush(F->fp.get()) != 0) return fail(BGF_ERR_INVALID_ARG, std::string("write failed: ") + path);
+    *out = F.release();
+    g_err.clear();
+    return BGF_OK;
+}
+
+bgf_status bgf_nc_buq_write(bgf_ncout f, double time, int precision, const void* const* values) {
+    if (!f || !f->fp) return fail(BGF_ERR_INVALID_ARG, "null or closed output file");
+    if (!f->names.empty() && !values) return fail(BGF_ERR_INVALID_ARG, "null values");
+    if (precision < 0 || precision > 3) return fail(BGF_ERR_INVALID_ARG, "bad precision");
+    Schema& S = f->S;
+    const bool dbl = precision == BGF_FP64;
+    const int nlev = f->maxlev - f->minlev + 1;
+    std::vector<uint8_t> rec(S.recsize, 0);
+    const uint64_t base = S.recbegin;
+    put_be<float>(&rec[S.vars[f->timevar].begin - base], (float)time);
+    for (size_t n = 0; n < f->names.size(); n++) {
+        if (!values[n]) return fail(BGF_ERR_INVALID_ARG, "null values for " + f->names[n]);
+        for (int l = 0; l < nlev; l++) {
+            const Var& v = S.vars[f->vidx[n * nlev + l]];
+            const std::vector<uint8_t> fill = fill_of(v);
+            uint8_t* g = &rec[v.begin - base];
+            for (uint64_t k = 0; k < v.nelem; k++) std::memcpy(g + k * fill.size(), fill.data(), fill.size());
+        }
+        for (size_t k = 0; k < f->blk.size(); k++) {
+            const int l = f->lev[k] - f->minlev;
+            const Var& v = S.vars[f->vidx[n * nlev + l]];
+            uint8_t* g = &rec[v.begin - base];
+            const int nx = f->nx[l];
+            for (int j = 0; j < BW; j++)
+                for (int i = 0; i < BW; i++) {
+                    const int src = memloc(i, j, f->blk[k]);
+                    const double x = dbl ? ((const double*)values[n])[src] : (double)((const float*)values[n])[src];
+                    const size_t dst = (size_t)(f->sy[k] + j) * nx + (f->sx[k] + i);
+                    if (f->smallnc) {
+                        // (short)round((var - addoffset) / scalefactor) in T (src/Write_netcdf.cu:1289)
+                        const double q = dbl ? (x - (double)f->offset) / (double)f->scale
+                                             : (double)(((float)x - f->offset) / f->scale);
+                        const double r = std::round(q);
+                        put_be<int16_t>(g + 2 * dst, (int16_t)(r < -32768.0 ? -32768 : r > 32767.0 ? 32767 : r));
+                    } else {
+                        put_be<float>(g + 4 * dst, (float)x);
+                    }
+                }
+        }
+    }
+    const uint64_t off = S.recbegin + S.numrecs * S.recsize;
+    if (!write_at(f->fp.get(), off, rec.data(), rec.size())) return fail(BGF_ERR_INVALID_ARG, "write failed: " + f->path);
+    S.numrecs++;
+    uint8_t nr[4];
+    put_be<uint32_t>(nr, (uint32_t)S.numrecs);
+    if (!write_at(f->fp.get(), 4, nr, 4) || fflush(f->fp.get()) != 0)
+        return fail(BGF_ERR_INVALID_ARG, "write failed: " + f->path);
+    g_err.clear();
+    return BGF_OK;
+}
+
+bgf_status bgf_nc_buq_close(bgf_ncout f) {
+    if (!f) return BGF_OK;
+    bool ok = true;
+    if (f->fp) ok = fclose(f->fp.release()) == 0;
+    delete f;
+    return ok ? BGF_OK : fail(BGF_ERR_INVALID_ARG, "close failed");
+}
+
+bgf_status bgf_nc_inq(const char* path, const char* var, int* ndim, long long* shape, int* is_record) {
+    if (!path || !var || !ndim) return fail(BGF_ERR_INVALID_ARG, "bgf_nc_inq: null argument");
+    Reader R;
+    if (!R.open(path)) return fail(BGF_ERR_INVALID_ARG, R.err);
+    const int vi = R.S.find(var);
+    if (vi < 0) return fail(BGF_ERR_INVALID_ARG, std::string("no variable ") + var + " in " + path);
+    const Var& v = R.S.vars[vi];
+    *ndim = (int)v.dims.size();
+    if (shape)
+        for (size_t k = 0; k < v.dims.size() && k < 8; k++) {
+            const Dim& d = R.S.dims[v.dims[k]];
+            shape[k] = d.len == 0 ? (long long)R.S.numrecs : (long long)d.len;
+        }
+    if (is_record) *is_record = v.rec;
+    g_err.clear();
+    return BGF_OK;
+}
+
+bgf_status bgf_nc_dim_names(const char* path, const char* var, char* names, int cap) {
+    if (!path || !var || !names || cap <= 0) return fail(BGF_ERR_INVALID_ARG, "bgf_nc_dim_names: null argument");
+    Reader R;
+    if (!R.open(path)) return fail(BGF_ERR_INVALID_ARG, R.err);
+    const int vi = R.S.find(var);
+    if (vi < 0) return fail(BGF_ERR_INVALID_ARG, std::string("no variable ") + var + " in " + path);
+    std::string s;
+    for (size_t k = 0; k < R.S.vars[vi].dims.size(); k++) {
+        if (k) s += ",";
+        s += R.S.dims[R.S.vars[vi].dims[k]].name;
+    }
+    if ((int)s.size() >= cap) return fail(BGF_ERR_INVALID_ARG, "name buffer too small");
+    std::memcpy(names, s.c_str(), s.size() + 1);
+    g_err.clear();
+    return BGF_OK;
+}
+
+bgf_status bgf_nc_get(const char* path, const char* var, long long rec, double* out) {
+    if (!path || !var || !out) return fail(BGF_ERR_INVALID_ARG, "bgf_nc_get: null argument");
+    Reader R;
+    if (!R.open(path)) return fail(BGF_ERR_INVALID_ARG, R.err);
+    const int vi = R.S.find(var);
+    if (vi < 0) return fail(BGF_ERR_INVALID_ARG, std::string("no variable ") + var + " in " + path);
+    if (!R.get(vi, R.S.vars[vi].rec ? rec : -1, out)) return fail(BGF_ERR_INVALID_ARG, R.err);
+    g_err.clear();
+    return BGF_OK;
+}
+
+bgf_status bgf_nc_get_att(const char* path, const char* var, const char* att, double* out, int* n) {
+    if (!path || !att || !n) return fail(BGF_ERR_INVALID_ARG, "bgf_nc_get_att: null argument");
+    Reader R;
+    if (!R.open(path)) return fail(BGF_ERR_INVALID_ARG, R.err);
+    const std::vector<Att>* atts = &R.S.gatts;
+    if (var && *var) {
+        const int vi = R.S.find(var);
+        if (vi < 0) return fail(BGF_ERR_INVALID_ARG, std::string("no variable ") + var + " in " + path);
+        atts = &R.S.vars[vi].atts;
+    }
+    for (const Att& a : *atts)
+        if (a.name == att) {
+            if (a.type == T_CHAR) return fail(BGF_ERR_INVALID_ARG, std::string("attribute ") + att + " is text");
+            const int cap = *n;
+            *n = (int)a.n;
+            if (out)
+                for (int k = 0; k < cap && k < (int)a.n; k++) out[k] = att_value(a, k);
+            g_err.clear();
+            return BGF_OK;
+        }
+    *n = 0;
+    return fail(BGF_ERR_INVALID_ARG, std::string("no attribute ") + att);
+}
+
+}  // extern "C"

@@ -376,6 +376,37 @@ bgf_status bgf_batch_stats(bgf_batch b, long long* cell_updates, int* nmembers, 
 bgf_status bgf_batch_set_schedule(bgf_batch b, int tile_x, int tile_y, int wg_pred, int wg_corr);
 bgf_status bgf_batch_get_schedule(bgf_batch b, int* tile_x, int* tile_y, int* wg_pred, int* wg_corr);
 
+/* ---- NetCDF files (no libnetcdf needed) ------------------------------------
+ * BUQ map output as BG_Flood writes it (creatncfileBUQ, defncvarBUQ,
+ * writencvarstepBUQ, writenctimestep: src/Write_netcdf.cu:118-500, 560-840,
+ * 1215-1400), in the NetCDF classic format with 64-bit offsets (the reference
+ * writes the same dimensions, variables and attributes into a NETCDF4 file).
+ * The zone is the rectangle xo..xmax, yo..ymax of p (Param.xmax / ymax for
+ * the whole domain, src/InitialConditions.cu:1105-1120) with levels
+ * p->minlevel..p->maxlevel; every active block of b must lie in it.  names
+ * are the output variables (Param.outvars); smallnc, scalefactor and
+ * addoffset as Param.smallnc / scalefactor / addoffset. */
+typedef struct bgf_ncout_s* bgf_ncout;
+bgf_status bgf_nc_buq_create(const char* path, const bgf_param* p, const bgf_blocks* b, double xmax, double ymax,
+                             int nvar, const char* const* names, int smallnc, float scalefactor, float addoffset,
+                             bgf_ncout* out);
+/* Appends one output (Save2Netcdf, src/Write_netcdf.cu:1703-1754): the time
+ * and, for each variable, its host array in BUQ layout (nblkmem*324 values of
+ * double when precision is BGF_FP64, else float). */
+bgf_status bgf_nc_buq_write(bgf_ncout f, double time, int precision, const void* const* values);
+bgf_status bgf_nc_buq_close(bgf_ncout f);
+
+/* Reading NetCDF classic files (versions 1, 2 and 5; the reference reads any
+ * NetCDF through libnetcdf, src/Read_netcdf.cu): a variable's shape (the
+ * record dimension counts the records), its dimension names ("a,b,c"), one
+ * record of a record variable (rec) or a whole fixed variable as doubles, and
+ * a numeric attribute (var NULL or "" for a global one; n: capacity in,
+ * count out). */
+bgf_status bgf_nc_inq(const char* path, const char* var, int* ndim, long long* shape, int* is_record);
+bgf_status bgf_nc_dim_names(const char* path, const char* var, char* names, int cap);
+bgf_status bgf_nc_get(const char* path, const char* var, long long rec, double* out);
+bgf_status bgf_nc_get_att(const char* path, const char* var, const char* att, double* out, int* n);
+
 #ifdef __cplusplus
 }
 #endif

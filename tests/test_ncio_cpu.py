@@ -1,0 +1,246 @@
+"""NetCDF BUQ map output and NetCDF reading (host code of the library, no GPU).
+
+The files are read back with scipy.io.netcdf_file, an independent NetCDF classic implementation,
+and checked against the layout of src/Write_netcdf.cu (creatncfileBUQ :118-500, defncvarBUQ
+:560-840, writencvarstepBUQ :1215-1360, writenctimestep :1369-1400): names, dimensions, types and
+attributes, every block's 16x16 interior at start (round((yo + blockyo - yymin)/dx_l),
+round((xo + blockxo - xxmin)/dx_l)), the fill value where no block of a level lies, and the
+records.  The reader is checked on files scipy wrote.
+"""
+import os
+
+import numpy as np
+import pytest
+from scipy.io import netcdf_file
+
+import bg_flood_amd as bgf
+from bg_flood_amd import ncio, scenarios
+
+MW, BS = 18, 324
+
+
+def _fields(param, blocks, seed=0, T=np.float32):
+    rng = np.random.default_rng(seed)
+    S = param["nblkmem"] * BS
+    return {n: (rng.standard_normal(S) * 3).astype(T) for n in ("zb", "h", "u", "v", "zs", "hmax")}
+
+
+def _expected_grid(param, blocks, a, lev, nx, ny, xmin, ymin):
+    """The level's grid built cell by cell from the block table (not from the writer's code)."""
+    g = np.full((ny, nx), ncio.FILL, np.float32)
+    dx = param["dx"] / 2.0 ** lev if lev >= 0 else param["dx"] * 2.0 ** (-lev)
+    for k in range(param["nblk"]):
+        ib = blocks["active"][k]
+        if blocks["level"][ib] != lev:
+            continue
+        x0 = int(round((param["xo"] + blocks["xo"][ib] - (xmin + dx / 2)) / dx))
+        y0 = int(round((param["yo"] + blocks["yo"][ib] - (ymin + dx / 2)) / dx))
+        for j in range(16):
+            for i in range(16):
+                g[y0 + j, x0 + i] = a[(i + 1) + (j + 1) * MW + ib * BS]
+    return g
+
+
+def _uniform(nbx=4, nby=3, dx=2.0, xo=100.0, yo=-50.0, precision="fp32"):
+    p, blocks, _, _ = scenarios.uniform(16 * nbx, 16 * nby, dx, precision, xo=xo, yo=yo)
+    return p, blocks
+
+
+def _multilevel():
+    # 2 x 2 level-0 blocks of dx 4 m; the bottom-right one refined to four level-1 blocks
+    leaves = [(0, 0, 0), (1, 2, 0), (1, 3, 0), (1, 2, 1), (1, 3, 1), (0, 0, 1), (0, 1, 1)]
+    blocks = bgf.mesh_from_leaves(leaves, 2, 2, 4.0)
+    n = len(leaves)
+    p = bgf.default_param(dx=4.0, delta=4.0, xo=10.0, yo=20.0, nblk=n, nblkmem=n, minlevel=0, maxlevel=1,
+                          precision="fp32")
+    return p, blocks
+
+
+def test_uniform_structure_and_values(tmp_path):
+    p, blocks = _uniform()
+    f = _fields(p, blocks)
+    path = tmp_path / "out.nc"
+    names = ["zs", "h", "u", "v", "zb"]
+    with ncio.BuqOutput(path, p, blocks, names) as o:
+        o.write(0.0, f)
+        f2 = {k: v * 2 for k, v in f.items()}
+        o.write(12.5, f2)
+    xmax, ymax = 100.0 + 64 * 2.0, -50.0 + 48 * 2.0
+    with netcdf_file(str(path), "r", mmap=False) as nc:
+        assert nc.version_byte == 2
+        a = nc._attributes
+        assert a["maxlevel"] == 0 and a["minlevel"] == 0
+        assert a["xmin"] == 100.0 and a["xmax"] == xmax and a["ymin"] == -50.0 and a["ymax"] == ymax
+        assert a["Conventions"] == b"CF-1.11-draft" and a["BG_Flood"] == 0
+        assert nc.dimensions["time"] is None
+        assert nc.dimensions["blockid"] == 12
+        assert nc.dimensions["xx_P0"] == 64 and nc.dimensions["yy_P0"] == 48
+        t = nc.variables["time"]
+        assert t.typecode() == "f" and list(t.data) == [0.0, 12.5]
+        assert t.units == b"seconds since 2000-01-01T00:00:00" and t.axis == b"T"
+        assert nc.variables["crs"].grid_mapping_name == b"projected"
+        assert nc.variables["BGFlood"].g == 9.81
+        np.testing.assert_array_equal(nc.variables["blockid"].data, np.arange(12))
+        np.testing.assert_array_equal(nc.variables["blocklevel"].data, np.zeros(12))
+        np.testing.assert_array_equal(nc.variables["blockwidth"].data, np.full(12, 2.0, np.float32))
+        np.testing.assert_array_equal(nc.variables["blockxo"].data,
+                                      (np.float32(100.0) + blocks["xo"][:12].astype(np.float32)))
+        np.testing.assert_array_equal(nc.variables["xx_P0"].data, 100.0 + 1.0 + 2.0 * np.arange(64))
+        np.testing.assert_array_equal(nc.variables["yy_P0"].data, -50.0 + 1.0 + 2.0 * np.arange(48))
+        assert nc.variables["xx_P0"].axis == b"X"
+        for n in names:
+            v = nc.variables[n + "_P0"]
+            assert v.dimensions == ("time", "yy_P0", "xx_P0") and v.typecode() == "f"
+            assert v._FillValue == np.float32(9.9692e36) and v.missingvalue == np.float32(9.9692e36)
+            assert v.grid_mapping == b"crs"
+            for r, src in enumerate((f, f2)):
+                np.testing.assert_array_equal(v.data[r], _expected_grid(p, blocks, src[n], 0, 64, 48, 100.0, -50.0))
+        assert nc.variables["zs_P0"].standard_name == b"water_surface_elevation"
+        assert nc.variables["h_P0"].units == b"m" and nc.variables["u_P0"].units == b"m s-1"
+
+
+def test_multilevel_grids_and_fill(tmp_path):
+    p, blocks = _multilevel()
+    f = _fields(p, blocks, seed=3)
+    path = tmp_path / "ml.nc"
+    with ncio.BuqOutput(path, p, blocks, ["h", "zb"]) as o:
+        o.write(3.0, f)
+    with netcdf_file(str(path), "r", mmap=False) as nc:
+        assert nc.dimensions["xx_P0"] == 32 and nc.dimensions["yy_P0"] == 32
+        assert nc.dimensions["xx_P1"] == 64 and nc.dimensions["yy_P1"] == 64
+        np.testing.assert_array_equal(nc.variables["xx_P1"].data, 10.0 + 1.0 + 2.0 * np.arange(64))
+        np.testing.assert_array_equal(nc.variables["blocklevel"].data, blocks["level"][blocks["active"]])
+        np.testing.assert_array_equal(nc.variables["blockwidth"].data,
+                                      np.where(blocks["level"][blocks["active"]] == 0, 4.0, 2.0).astype(np.float32))
+        for lev, n in ((0, 32), (1, 64)):
+            for name in ("h", "zb"):
+                g = nc.variables["%s_P%d" % (name, lev)].data[0]
+                np.testing.assert_array_equal(g, _expected_grid(p, blocks, f[name], lev, n, n, 10.0, 20.0))
+        # the refined quadrant is fill on level 0 and data on level 1, and the reverse elsewhere
+        g0 = nc.variables["h_P0"].data[0]
+        g1 = nc.variables["h_P1"].data[0]
+        assert np.all(g0[:16, 16:] == ncio.FILL) and np.all(g0[:16, :16] != ncio.FILL)
+        assert np.all(g1[:32, 32:] != ncio.FILL) and np.all(g1[32:, :] == ncio.FILL)
+
+
+def test_smallnc_short_values(tmp_path):
+    p, blocks = _uniform(nbx=2, nby=1)
+    f = _fields(p, blocks, seed=5)
+    path = tmp_path / "s.nc"
+    with ncio.BuqOutput(path, p, blocks, ["zs"], smallnc=1, scalefactor=0.01, addoffset=1.5) as o:
+        o.write(1.0, f)
+    with netcdf_file(str(path), "r", mmap=False, maskandscale=False) as nc:
+        v = nc.variables["zs_P0"]
+        assert v.typecode() == "h"
+        assert v.scale_factor == np.float32(0.01) and v.add_offset == np.float32(1.5)
+        assert v._FillValue == -32767
+        raw = v.data[0]
+        src = f["zs"]
+        for k in range(2):
+            ib = blocks["active"][k]
+            x0 = int(round((blocks["xo"][ib] - 1.0) / 2.0))
+            for j in range(16):
+                for i in range(16):
+                    x = src[(i + 1) + (j + 1) * MW + ib * BS]
+                    q = (np.float32(x) - np.float32(1.5)) / np.float32(0.01)
+                    assert raw[j, x0 + i] == int(np.floor(abs(q) + 0.5) * np.sign(q))
+
+
+def test_fp64_values_and_reader_roundtrip(tmp_path):
+    p, blocks = _uniform(nbx=2, nby=2, precision="fp64")
+    f = _fields(p, blocks, seed=9, T=np.float64)
+    path = tmp_path / "d.nc"
+    with ncio.BuqOutput(path, p, blocks, ["h", "u"]) as o:
+        for r in range(3):
+            o.write(10.0 * r, {k: v + r for k, v in f.items()})
+    with netcdf_file(str(path), "r", mmap=False) as nc:
+        ref_h = nc.variables["h_P0"].data.copy()
+        ref_t = nc.variables["time"].data.copy()
+    # the library's reader returns what scipy reads
+    np.testing.assert_array_equal(ncio.read_var(path, "h_P0"), ref_h)
+    np.testing.assert_array_equal(ncio.read_var(path, "h_P0", rec=2), ref_h[2])
+    np.testing.assert_array_equal(ncio.read_var(path, "time"), ref_t)
+    assert ncio.var_shape(path, "h_P0") == ((3, 32, 32), True)
+    assert ncio.var_dims(path, "u_P0") == ["time", "yy_P0", "xx_P0"]
+    assert ncio.read_att(path, "maxlevel") == 0
+    assert ncio.read_att(path, "g", var="BGFlood") == 9.81
+    # values are the FP64 state rounded to float once
+    g = ref_h[1]
+    ib = blocks["active"][3]
+    assert g[16, 16] == np.float32(f["h"][1 + MW + ib * BS] + 1)
+    x, y, z = ncio.read_grid(path, "h_P0", step=1)
+    np.testing.assert_array_equal(x, 101.0 + 2.0 * np.arange(32))
+    np.testing.assert_array_equal(z, ref_h[1])
+
+
+@pytest.mark.parametrize("version", [1, 2])
+def test_reader_on_scipy_files(tmp_path, version):
+    path = str(tmp_path / "g.nc")
+    rng = np.random.default_rng(1)
+    x = np.linspace(0.0, 5.0, 7)
+    y = np.linspace(-1.0, 1.0, 5)
+    z = rng.standard_normal((5, 7))
+    zt = rng.standard_normal((3, 5, 7)).astype(np.float32)
+    s = rng.integers(-300, 300, (5, 7)).astype(np.int16)
+    with netcdf_file(path, "w", version=version) as nc:
+        nc.createDimension("time", None)
+        nc.createDimension("y", 5)
+        nc.createDimension("x", 7)
+        vx = nc.createVariable("x", "d", ("x",))
+        vx[:] = x
+        vy = nc.createVariable("y", "d", ("y",))
+        vy[:] = y
+        vz = nc.createVariable("z", "d", ("y", "x"))
+        vz[:] = z
+        vz.scale = 2.5
+        vs = nc.createVariable("s", "h", ("y", "x"))
+        vs[:] = s
+        vt = nc.createVariable("time", "f", ("time",))
+        vt[:] = [0.0, 1.0, 2.0]
+        vzt = nc.createVariable("zt", "f", ("time", "y", "x"))
+        vzt[:] = zt
+        nc.title = "grid"
+        nc.level = np.int32(3)
+    gx, gy, gz = ncio.read_grid(path, "z")
+    np.testing.assert_array_equal(gx, x)
+    np.testing.assert_array_equal(gy, y)
+    np.testing.assert_array_equal(gz, z)
+    np.testing.assert_array_equal(ncio.read_var(path, "s"), s)
+    for k in range(3):
+        np.testing.assert_array_equal(ncio.read_grid(path, "zt", step=k)[2], zt[k])
+    assert ncio.read_att(path, "scale", var="z") == 2.5
+    assert ncio.read_att(path, "level") == 3
+    assert ncio.check_var_name(path, "zb", "z", "ZB") == "z"
+    assert ncio.check_var_name(path, "nothere") == ""
+
+
+def test_errors(tmp_path):
+    p, blocks = _uniform(nbx=1, nby=1)
+    with pytest.raises(bgf.BgfError):
+        ncio.BuqOutput(tmp_path / "e.nc", p, blocks, ["h"], xmax=p["xo"] + 8.0)  # zone smaller than the block
+    with ncio.BuqOutput(tmp_path / "e2.nc", p, blocks, ["h"]) as o:
+        with pytest.raises(bgf.BgfError):
+            o.write(0.0, {"u": np.zeros(p["nblkmem"] * BS, np.float32)})
+        with pytest.raises(bgf.BgfError):
+            o.write(0.0, {"h": np.zeros(10, np.float32)})
+    with pytest.raises(bgf.BgfError):
+        ncio.read_var(tmp_path / "e2.nc", "nothere")
+    bad = tmp_path / "bad.nc"
+    bad.write_bytes(b"\x89HDF\r\n\x1a\n" + b"\0" * 64)
+    with pytest.raises(bgf.BgfError, match="classic"):
+        ncio.read_var(bad, "z")
+
+
+MONAI = "/root/reference/Examples/Monai/Bathy_D.nc"
+
+
+@pytest.mark.skipif(not os.path.exists(MONAI), reason="the reference's example data is not present")
+def test_reference_monai_bathymetry():
+    """The reference's Monai valley bathymetry (a NetCDF classic file written by GMT) read by the
+    library matches scipy's reading."""
+    x, y, z = ncio.read_grid(MONAI, "z")
+    with netcdf_file(MONAI, "r", mmap=False) as nc:
+        np.testing.assert_array_equal(x, nc.variables["x"].data)
+        np.testing.assert_array_equal(y, nc.variables["y"].data)
+        np.testing.assert_array_equal(z, nc.variables["z"].data)
+    assert z.shape == (246, 394)
