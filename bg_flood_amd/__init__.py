@@ -131,6 +131,9 @@ def lib():
         L.bgf_adapt_mesh.argtypes = [ctypes.POINTER(BgfParam), ctypes.POINTER(BgfBlocks), vp, vp, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int, ip, ip, ctypes.POINTER(BgfBlocksOut), ip]
         L.bgf_remesh.argtypes = [vp, ctypes.POINTER(BgfParam), ctypes.POINTER(BgfBlocks), ip]
+        L.bgf_sample_raster.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, vp, vp]
+        L.bgf_sample_raster.restype = ctypes.c_int
         L.bgf_set_bed_source.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, vp]
         L.bgf_set_boundary.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp]
@@ -438,6 +441,17 @@ class Domain:
         a = np.ascontiguousarray(values, np.float32)
         ny, nx = a.shape
         _check(lib().bgf_set_bed_source(self.h, nx, ny, float(xo), float(yo), float(dx), a.ctypes.data))
+
+    def sample_raster(self, values, xo, yo, dx, out=None):
+        """A raster (ny, nx) at xo + i*dx, yo + j*dx sampled onto the interior cells of the mesh
+        (interp2BUQ, bgf_sample_raster on the GPU): an nblkmem*324 array of the domain's T, out's
+        entries (zero if out is None) elsewhere."""
+        a = np.ascontiguousarray(values, np.float32)
+        ny, nx = a.shape
+        T = np.float64 if self.param["precision"] == "fp64" else np.float32
+        o = np.zeros(self.param["nblkmem"] * BS, T) if out is None else np.ascontiguousarray(out, T).copy()
+        _check(lib().bgf_sample_raster(self.h, nx, ny, float(xo), float(yo), float(dx), a.ctypes.data, o.ctypes.data))
+        return o
 
     def remesh(self, blocks, nblk, nblkmem, remap):
         """Moves the state onto a mesh from adapt_mesh (bgf_remesh)."""

@@ -842,6 +842,43 @@ bgf_status bgf_set_bed_source(bgf_handle h, int nx, int ny, double xo, double yo
     return BGF_OK;
 }
 
+bgf_status bgf_sample_raster(bgf_handle h, int nx, int ny, double xo, double yo, double dx, const float* values,
+                             void* out) {
+    if (!h || !values || !out) return fail(BGF_ERR_INVALID_ARG, "null argument");
+    if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
+    if (nx < 2 || ny < 2 || !(dx > 0)) return fail(BGF_ERR_INVALID_ARG, "raster needs nx, ny >= 2 and dx > 0");
+    const size_t rbytes = sizeof(float) * (size_t)nx * ny;
+    const size_t obytes = h->es * h->S;
+    float* d_r = nullptr;
+    void* d_o = nullptr;
+    HIPCHK(hipMalloc((void**)&d_r, rbytes));
+    if (hipMalloc(&d_o, obytes) != hipSuccess) {
+        (void)hipFree(d_r);
+        return fail(BGF_ERR_OOM, "sample output allocation failed");
+    }
+    bool ok = hipMemcpy(d_r, values, rbytes, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d_o, out, obytes, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+        bgf::RemeshLaunch R{};
+        R.nblk = h->nblk;
+        R.active = h->d_active;
+        R.level = h->d_level;
+        R.xo = h->d_blkxo;
+        R.yo = h->d_blkyo;
+        R.nzb = d_o;
+        R.bed = d_r;
+        R.bnx = nx; R.bny = ny; R.bxo = xo; R.byo = yo; R.bdx = dx;
+        R.pxo = h->p.xo; R.pyo = h->p.yo; R.dx = h->p.dx;
+        bgf::launch_sample_raster(h->prec == BGF_FP64 ? 0 : 1, R, h->stream);
+        ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(h->stream) == hipSuccess &&
+             hipMemcpy(out, d_o, obytes, hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    (void)hipFree(d_r);
+    (void)hipFree(d_o);
+    if (!ok) return fail(BGF_ERR_HIP, "raster sampling failed");
+    return BGF_OK;
+}
+
 bgf_status bgf_remesh(bgf_handle h, const bgf_param* p, const bgf_blocks* b, const int* remap) {
     if (!h || !p || !b || !remap) return fail(BGF_ERR_INVALID_ARG, "null argument");
     if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");

@@ -204,5 +204,8 @@ struct RemeshLaunch {
     double bxo, byo, bdx, pxo, pyo, dx, eps;
 };
 void launch_remesh(int precision, const KParams& Po, const RemeshLaunch& R, hipStream_t st);
+// R.bed sampled onto the interior cells of R's mesh (R.active, level, xo, yo): R.nzb = interp2BUQ
+// of the raster at every cell (only the raster, mesh and output fields of R are read).
+void launch_sample_raster(int precision, const RemeshLaunch& R, hipStream_t st);
 
 }  // namespace bgf

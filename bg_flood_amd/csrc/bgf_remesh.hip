@@ -200,7 +200,35 @@ template <class T> __global__ __launch_bounds__(256) void k_remesh(KParams Po, R
     A.ncf[o] = cf;
 }
 
+// A raster sampled onto the cells of a mesh, as interp2BUQ(XParam, XBlock,
+// forcing, z) (src/GridManip.cu:333-560) at the cell centres
+// xo + blockxo + i dx_l, yo + blockyo + j dx_l.
+template <class T> __global__ __launch_bounds__(256) void k_sample_raster(RemeshArgs<T> A) {
+    const int ib = A.active[blockIdx.x];
+    const int x = threadIdx.x & 15, y = threadIdx.x >> 4;
+    const double blkdx = calcres(A.dx, A.level[ib]);
+    const T px = T(A.pxo + A.xo[ib] + x * blkdx), py = T(A.pyo + A.yo[ib] + y * blkdx);
+    A.nzb[memloc(x, y, ib)] = bed_sample(A, px, py, T(blkdx));
+}
+
 }  // namespace
+
+void launch_sample_raster(int precision, const RemeshLaunch& R, hipStream_t st) {
+    if (R.nblk <= 0 || !R.bed) return;
+    if (precision == 0) {
+        RemeshArgs<double> A{};
+        A.active = R.active; A.level = R.level; A.xo = R.xo; A.yo = R.yo; A.nzb = (double*)R.nzb;
+        A.bed = R.bed; A.bnx = R.bnx; A.bny = R.bny; A.bxo = R.bxo; A.byo = R.byo; A.bdx = R.bdx;
+        A.pxo = R.pxo; A.pyo = R.pyo; A.dx = R.dx;
+        hipLaunchKernelGGL(k_sample_raster<double>, dim3(R.nblk), dim3(256), 0, st, A);
+    } else {
+        RemeshArgs<float> A{};
+        A.active = R.active; A.level = R.level; A.xo = R.xo; A.yo = R.yo; A.nzb = (float*)R.nzb;
+        A.bed = R.bed; A.bnx = R.bnx; A.bny = R.bny; A.bxo = R.bxo; A.byo = R.byo; A.bdx = R.bdx;
+        A.pxo = R.pxo; A.pyo = R.pyo; A.dx = R.dx;
+        hipLaunchKernelGGL(k_sample_raster<float>, dim3(R.nblk), dim3(256), 0, st, A);
+    }
+}
 
 void launch_remesh(int precision, const KParams& Po, const RemeshLaunch& R, hipStream_t st) {
     if (R.nblk <= 0) return;

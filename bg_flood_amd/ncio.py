@@ -194,3 +194,67 @@ def check_var_name(path, *names):
         if has_var(path, n):
             return n
     return ""
+
+
+def _raster(path, name, step):
+    """A grid variable as a float32 raster with increasing x and y: (values, xo, yo, dx)."""
+    x, y, z = read_grid(path, name, step)
+    z = np.asarray(z, np.float32)
+    if len(x) < 2 or len(y) < 2:
+        raise BgfError("%s: a grid needs at least 2 x 2 points" % name)
+    if x[1] < x[0]:
+        x, z = x[::-1], z[:, ::-1]
+    if y[1] < y[0]:
+        y, z = y[::-1], z[::-1, :]
+    dx = float(x[1] - x[0])
+    return np.ascontiguousarray(z), float(x[0]), float(y[0]), dx
+
+
+HOTSTART_NAMES = {  # checkncvarname candidates (src/InitEvolv.cu:636-640)
+    "zb": ("zb", "z", "ZB", "Z", "zb_P0"),
+    "zs": ("zs", "eta", "ZS", "ETA", "zs_P0"),
+    "h": ("h", "hh", "hhh", "hhhh", "h_P0"),
+    "u": ("u", "uu", "uvel", "UVEL", "u_P0"),
+    "v": ("v", "vv", "vvel", "VVEL", "v_P0"),
+}
+
+
+def hotstart(domain, path, step=0, zb=None):
+    """State of a hot start from a NetCDF file (readhotstartfile, src/InitEvolv.cu:598-835): zb, zs,
+    h, u and v are looked up under the reference's names, read at `step` and sampled onto the
+    domain's mesh (interp2BUQ); zs is raised to the bed, h is max(zs - zb, 0) when the file has zs
+    only, zs = zb + h when it has h only; missing u, v are zero.  zb: the domain's bed (BUQ array),
+    used when the file has none.  Returns dict(zb, h, u, v) in BUQ layout (the domain's T), or None
+    when the file has neither zs nor h (the reference then falls back to a cold start).  A file with
+    both zs and h gives h from the file (the state here is zb, h, u, v)."""
+    p = domain.param
+    T = np.float64 if p["precision"] == "fp64" else np.float32
+    S = p["nblkmem"] * 324
+    names = {k: check_var_name(path, *v) for k, v in HOTSTART_NAMES.items()}
+    if not names["zs"] and not names["h"]:
+        return None
+
+    def sample(k):
+        vals, xo, yo, dx = _raster(path, names[k], step)
+        return domain.sample_raster(vals, xo, yo, dx)
+
+    if names["zb"]:
+        zbv = sample("zb")
+    elif zb is not None:
+        zbv = np.asarray(zb, T).copy()
+    else:
+        raise BgfError("%s has no bed (zb) and none was given" % path)
+    zs = None
+    if names["zs"]:
+        zs = np.maximum(sample("zs"), zbv).astype(T)  # utils::max(zs, zb)
+    if names["h"]:
+        h = sample("h")
+    else:
+        h = np.maximum(zs - zbv, T(0.0)).astype(T)
+    u = sample("u") if names["u"] else np.zeros(S, T)
+    v = sample("v") if names["v"] else np.zeros(S, T)
+    from . import interior_mask
+    m = interior_mask(p["nblkmem"], np.asarray(domain.blocks["active"])[:p["nblk"]])
+    for a in (h, u, v):
+        a[~m] = 0
+    return dict(zb=zbv, h=h, u=u, v=v)

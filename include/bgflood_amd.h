@@ -299,6 +299,15 @@ bgf_status bgf_set_rain_map(bgf_handle h, int nx, int ny, double xo, double yo, 
  * NULL values removes it. */
 bgf_status bgf_set_bed_source(bgf_handle h, int nx, int ny, double xo, double yo, double dx, const float* values);
 
+/* A raster (values[i + j*nx] at xo + i*dx, yo + j*dx) sampled onto the
+ * interior cells of the handle's mesh as interp2BUQ (src/GridManip.cu:333-560:
+ * bilinear where the cell is no coarser than the raster, else the mean of the
+ * raster points in the cell), into out (host, nblkmem*324 values of the
+ * handle's T; other entries are left as they are).  Used for the hot start
+ * (readhotstartfile, src/InitEvolv.cu:598-835) and for gridded inputs. */
+bgf_status bgf_sample_raster(bgf_handle h, int nx, int ny, double xo, double yo, double dx, const float* values,
+                             void* out);
+
 /* Refinement flags of the reference's criteria on the GPU state
  * (src/AdaptCriteria.cu:62-160): per active block, mode 0 (Threshold) sets
  * flags[ib] if any interior cell of the field exceeds a; mode 1 (Inrange) if

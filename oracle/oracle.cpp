@@ -2198,6 +2198,29 @@ extern "C" int orc_adapt_mesh(const orc_param* p, const orc_blocks* b, const uns
     return adapt_mesh_t<float>(p, b, flags, cflags, capacity, nblk_out, nblkmem_out, active, level, xo, yo, nb, remap);
 }
 
+// interp2BUQ(XParam, XBlock, forcing, z) (src/GridManip.cu:333-560): the raster
+// at every interior cell of the active blocks, cell centres
+// xo + blockxo + i dx_l in T.
+template <class T> void sample_raster_t(const orc_param* p, const orc_blocks* b, const BedSource& f, T* out) {
+    for (int ibl = 0; ibl < p->nblk; ibl++) {
+        const int ib = b->active[ibl];
+        const double blkdx = calcres(p->dx, b->level[ib]);
+        for (int iy = 0; iy < BW; iy++)
+            for (int ix = 0; ix < BW; ix++) {
+                T x = T(p->xo + b->xo[ib] + ix * blkdx), y = T(p->yo + b->yo[ib] + iy * blkdx);
+                out[memloc(ix, iy, ib)] = bed_interp(x, y, T(blkdx), f);
+            }
+    }
+}
+
+extern "C" int orc_sample_raster(const orc_param* p, const orc_blocks* b, int nx, int ny, double xo, double yo,
+                                 double dx, const float* val, void* out) {
+    BedSource src{nx, ny, xo, yo, dx, val};
+    if (p->doubleprecision) sample_raster_t<double>(p, b, src, (double*)out);
+    else sample_raster_t<float>(p, b, src, (float*)out);
+    return 0;
+}
+
 extern "C" int orc_remesh(orc_model m, const orc_param* p, const orc_blocks* b, const int* remap, int bnx, int bny,
                           double bxo, double byo, double bdx, const float* bed) {
     BedSource src{bnx, bny, bxo, byo, bdx, bed};

@@ -107,6 +107,11 @@ int orc_adapt_mesh(const orc_param* p, const orc_blocks* b, const unsigned char*
  * not NULL). */
 int orc_remesh(orc_model m, const orc_param* p, const orc_blocks* b, const int* remap, int bnx, int bny,
                double bxo, double byo, double bdx, const float* bed);
+/* A float raster val[ny][nx] at (xo, yo) with spacing dx sampled onto the
+ * interior cells of the active blocks (interp2BUQ); out has nblkmem*324
+ * values of T (double when p->doubleprecision). */
+int orc_sample_raster(const orc_param* p, const orc_blocks* b, int nx, int ny, double xo, double yo, double dx,
+                      const float* val, void* out);
 /* Spherical metric terms (src/Spherical.cu:15-64). */
 double orc_calccm_d(double Radius, double delta, double yo, int iy);
 double orc_calcfm_d(double Radius, double delta, double yo, double iy);

@@ -99,6 +99,10 @@ def lib():
         L.orc_remesh.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcParam), ctypes.POINTER(OrcBlocks), ip,
                                  ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_void_p]
+        L.orc_sample_raster.restype = ctypes.c_int
+        L.orc_sample_raster.argtypes = [ctypes.POINTER(OrcParam), ctypes.POINTER(OrcBlocks), ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
+                                        ctypes.c_void_p]
         L.orc_set_rivers.restype = ctypes.c_int
         L.orc_set_rivers.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_void_p, ctypes.c_void_p]
@@ -276,6 +280,19 @@ class Oracle:
         self.p = p
         self._keep = keep
         self.size = p.nblkmem * BS
+
+
+def sample_raster(p, blocks, values, xo, yo, dx):
+    """interp2BUQ of the raster values (ny, nx) at (xo, yo), spacing dx, onto the interior cells of
+    the active blocks (orc_sample_raster); an nblkmem*324 array of T, zero elsewhere."""
+    keep = {}
+    b = _blocks(blocks, p.nblkmem, keep)
+    vals = np.ascontiguousarray(values, np.float32)
+    out = np.zeros(p.nblkmem * BS, np.float64 if p.doubleprecision else np.float32)
+    rc = lib().orc_sample_raster(ctypes.byref(p), ctypes.byref(b), vals.shape[1], vals.shape[0], float(xo), float(yo),
+                                 float(dx), vals.ctypes.data, out.ctypes.data)
+    assert rc == 0
+    return out
 
 
 def _blocks(blocks, nbm, keep):

@@ -244,3 +244,51 @@ def test_reference_monai_bathymetry():
         np.testing.assert_array_equal(y, nc.variables["y"].data)
         np.testing.assert_array_equal(z, nc.variables["z"].data)
     assert z.shape == (246, 394)
+
+
+def _orc_param(p, blocks):
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), ".."))
+    from oracle import pyoracle as po
+    return po, po.make_param(dx=p["dx"], delta=p["delta"], xo=p["xo"], yo=p["yo"], nblk=p["nblk"],
+                             nblkmem=p["nblkmem"], minlevel=p.get("minlevel", 0), maxlevel=p.get("maxlevel", 0),
+                             doubleprecision=int(p["precision"] == "fp64"))
+
+
+def test_oracle_sample_raster_pins():
+    """orc_sample_raster (interp2BUQ over the mesh) against scipy's bilinear interpolation where
+    the cells are finer than the raster, and against the mean of the raster points in each cell
+    where they are coarser (blockmean)."""
+    from scipy.interpolate import RegularGridInterpolator
+    p, blocks = _uniform(nbx=2, nby=2, dx=1.0, xo=0.0, yo=0.0, precision="fp64")
+    po, op = _orc_param(p, blocks)
+    rng = np.random.default_rng(4)
+    # raster of spacing 3 m covering the domain (cells of 1 m: bilinear)
+    xr = -1.0 + 3.0 * np.arange(13)
+    yr = -2.0 + 3.0 * np.arange(13)
+    vals = rng.standard_normal((13, 13)).astype(np.float32)
+    out = po.sample_raster(op, blocks, vals, xr[0], yr[0], 3.0)
+    f = RegularGridInterpolator((yr, xr), vals.astype(np.float64))
+    for k in range(4):
+        ib = blocks["active"][k]
+        for j in range(16):
+            for i in range(16):
+                x = blocks["xo"][ib] + i
+                y = blocks["yo"][ib] + j
+                assert abs(out[(i + 1) + (j + 1) * MW + ib * BS] - f([[y, x]])[0]) < 1e-12
+    # raster of spacing 0.25 m (cells of 1 m: blockmean).  A constant raster gives the constant;
+    # a raster linear in x gives a value between the plane at the cell's left edge less one
+    # raster spacing and at its right edge (the points blockmean averages lie there).
+    xr = -0.125 + 0.25 * np.arange(140)
+    const = np.full((140, 140), np.float32(2.75))
+    out = po.sample_raster(op, blocks, const, xr[0], xr[0], 0.25)
+    m = bgf.interior_mask(p["nblkmem"], blocks["active"][:p["nblk"]])
+    assert np.all(out[m] == 2.75)
+    lin = np.tile((3.0 * xr).astype(np.float32), (140, 1))
+    out = po.sample_raster(op, blocks, lin, xr[0], xr[0], 0.25)
+    for k in range(4):
+        ib = blocks["active"][k]
+        for (i, j) in ((0, 0), (5, 7), (15, 15)):
+            x = blocks["xo"][ib] + i
+            v = out[(i + 1) + (j + 1) * MW + ib * BS]
+            assert 3.0 * (x - 0.75) - 1e-6 <= v <= 3.0 * (x + 0.5) + 1e-6
