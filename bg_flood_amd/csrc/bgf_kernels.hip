@@ -49,18 +49,23 @@
 
 namespace bgf {
 
-// LDS pitches are odd so that column accesses (halo strips, the 17th face)
-// fall into distinct banks.
+// LDS pitches.  128-bit accesses of 16-lane rows are free of bank conflicts
+// on gfx950 only at pitches of 16 or 32 entries (17 to 24 cost 4 conflict
+// cycles per read, measured with SQ_LDS_BANK_CONFLICT), so the arrays that are
+// only read by rows (y slopes, Y faces) are 16 wide; the others, which also
+// take column accesses (halo strips, the ring, the 17th faces), keep odd
+// pitches.
 constexpr int TP = 21;  // tile: x, y in [-2, 17]
 constexpr int GXP = 19; // x slopes: x in [-1, 16], y in [0, 15]
-constexpr int GYP = 17; // y slopes: x in [0, 15], y in [-1, 16]
-constexpr int FP = 17;  // X faces xf in [0, 16]; Y faces x in [0, 15]
+constexpr int GYP = 16; // y slopes: x in [0, 15], y in [-1, 16]
+constexpr int FP = 17;  // X faces xf in [0, 16], y in [0, 15]
+constexpr int FYP = 16; // Y faces x in [0, 15], yf in [0, 16]
 
 __device__ __forceinline__ int tix(int x, int y) { return (y + 2) * TP + (x + 2); }
 __device__ __forceinline__ int gxi(int x, int y) { return y * GXP + (x + 1); }
 __device__ __forceinline__ int gyi(int x, int y) { return (y + 1) * GYP + x; }
 __device__ __forceinline__ int fxi(int xf, int y) { return y * FP + xf; }
-__device__ __forceinline__ int fyi(int x, int yf) { return yf * FP + x; }
+__device__ __forceinline__ int fyi(int x, int yf) { return yf * FYP + x; }
 
 // fmin/fmax (v_min/v_max, fused into v_min3/v_max3).  They differ from the
 // reference's std::min/std::max only for NaN operands and in the sign of a
@@ -232,7 +237,7 @@ template <class T> struct Tile {
     Q4<T> gx[16 * GXP]; // slopes along x of (h, zs, u, v) at x in [-1,16], y in [0,15]
     Q4<T> gy[18 * GYP]; // slopes along y of (h, zs, v, u) at x in [0,15], y in [-1,16]
     Q4<T> fx[16 * FP];  // (Fhu, Fqux, Su, Fqvx) at X faces
-    Q4<T> fy[17 * FP];  // (Fhv, Fqvy, Sv, Fquy) at Y faces
+    Q4<T> fy[17 * FYP]; // (Fhv, Fqvy, Sv, Fquy) at Y faces
     T red[4];
     int next[4];  // queue positions, taken two iterations ahead by thread 0
 };
