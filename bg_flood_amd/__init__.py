@@ -134,6 +134,11 @@ def lib():
         L.bgf_sample_raster.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, vp, vp]
         L.bgf_sample_raster.restype = ctypes.c_int
+        L.bgf_set_stats.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.bgf_reset_stats.argtypes = [vp, ctypes.c_int]
+        L.bgf_get_stats.argtypes = [vp, ctypes.c_char_p, vp]
+        for n in ("bgf_set_stats", "bgf_reset_stats", "bgf_get_stats"):
+            getattr(L, n).restype = ctypes.c_int
         L.bgf_set_bed_source.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, vp]
         L.bgf_set_boundary.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp]
@@ -441,6 +446,26 @@ class Domain:
         a = np.ascontiguousarray(values, np.float32)
         ny, nx = a.shape
         _check(lib().bgf_set_bed_source(self.h, nx, ny, float(xo), float(yo), float(dx), a.ctypes.data))
+
+    STATS = ("hmean", "zsmean", "umean", "vmean", "Umean", "hUmean", "hUUmean", "hmax", "zsmax", "umax", "vmax",
+             "Umax", "hUmax", "hUUmax", "twet")
+
+    def set_stats(self, mean=False, max=False, twet=False, wet_threshold=0.1):
+        """Output statistics updated on the GPU after every step (Calcmeanmax): means (Param.outmean),
+        maxima (outmax) and wet time (outtwet, threshold Param.wet_threshold); starts them afresh."""
+        mask = (1 if mean else 0) | (2 if max else 0) | (4 if twet else 0)
+        _check(lib().bgf_set_stats(self.h, mask, float(wet_threshold)))
+
+    def reset_stats(self, mean=True, max=False, twet=False):
+        """resetmeanmax after an output: the means (and their step count), and with Param.resetmax
+        the maxima and the wet time."""
+        _check(lib().bgf_reset_stats(self.h, (1 if mean else 0) | (2 if max else 0) | (4 if twet else 0)))
+
+    def stat(self, name):
+        """One statistic by its output name (hmean ... hUUmax, twet), in BUQ layout."""
+        out = np.zeros(self.size, self.dtype)
+        _check(lib().bgf_get_stats(self.h, name.encode(), out.ctypes.data))
+        return out
 
     def sample_raster(self, values, xo, yo, dx, out=None):
         """A raster (ny, nx) at xo + i*dx, yo + j*dx sampled onto the interior cells of the mesh

@@ -45,6 +45,9 @@ struct bgf_handle_s {
     int pending_steps = 0;  // steps enqueued by bgf_run_async
     bool state_set = false;
     bgf::DevLoop loop_host;  // staging copy of the loop state for uploads
+    // output statistics (Calcmeanmax): enabled set, arrays, steps since the means were reset
+    bgf::StatsLaunch stats{};
+    int stats_n = 0;
     // multi-level meshes: the step runs the bgf_amr kernels on the reference
     // layout (state stays in buffer 0, buffer 1 is evolv_o)
     bool amr = false;
@@ -93,6 +96,7 @@ namespace {
 void free_handle(bgf_handle h) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
     F(h->d_active); F(h->d_order); F(h->d_level); F(h->d_queue); F(h->d_rec);
+    for (int k = 0; k < 15; k++) F(h->stats.a[k]);
     for (int k = 0; k < 8; k++) F(h->d_nb[k]);
     for (int k = 0; k < 2; k++) { F(h->d_h[k]); F(h->d_u[k]); F(h->d_v[k]); }
     F(h->d_zb); F(h->d_cf); F(h->d_dh); F(h->d_dhu); F(h->d_dhv);
@@ -126,6 +130,7 @@ bgf_status upload_loop(bgf_handle h, const bgf_loop* loop, double endtime) {
     L.rain_v = h->d_rain_v;
     L.rain_n = h->rain_n;
     L.done = loop->totaltime >= endtime ? 1 : 0;
+    L.stats_n = h->stats_n;
     for (int k = 0; k < bgf::NSLOT; k++) {
         L.slot32[k] = 0x7f800000u;
         L.slot64[k] = 0x7ff0000000000000ull;
@@ -138,7 +143,17 @@ bgf_status upload_loop(bgf_handle h, const bgf_loop* loop, double endtime) {
 bgf_status download_loop(bgf_handle h, bgf::DevLoop* L) {
     HIPCHK(hipMemcpyAsync(L, h->d_loop, sizeof(*L), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    h->stats_n = L->stats_n;
     return BGF_OK;
+}
+
+// Output statistics of the state after a step: one-level steps end in buffer
+// `out`, multi-level steps in buffer 0.
+void enqueue_stats(bgf_handle h, int out, int advanced) {
+    if (!h->stats.mask) return;
+    const int b = h->amr ? 0 : out;
+    bgf::launch_stats(h->prec == 0 ? 0 : 1, h->kp, h->stats, h->d_h[b], h->d_u[b], h->d_v[b], h->d_zb, h->d_loop,
+                      advanced, h->stream);
 }
 
 void fill_buffers(bgf_handle h, int in, void* out_storage) {
@@ -183,11 +198,13 @@ void enqueue_period(bgf_handle h, int advance_time) {
     if (h->amr) {
         fill_amr_buffers(h, storage);
         bgf::launch_amr_step(h->prec == 0 ? 0 : 1, h->kp, storage, h->d_loop, advance_time, h->stream, nullptr);
+        enqueue_stats(h, 0, advance_time);
         return;
     }
     for (int k = 0; k < 2; k++) {
         fill_buffers(h, k, storage);
         bgf::launch_step(h->prec, h->kp, storage, h->d_loop, advance_time, h->stream, nullptr);
+        enqueue_stats(h, 1 - k, advance_time);
     }
 }
 
@@ -224,10 +241,12 @@ bgf_status enqueue_steps(bgf_handle h, int nsteps, int advance_time) {
         if (h->amr) {
             bgf::launch_amr_step(h->prec == 0 ? 0 : 1, h->kp, storage, h->d_loop, advance_time, h->stream,
                                  h->timing ? h->ev : nullptr);
+            enqueue_stats(h, 0, advance_time);
         } else {
             fill_buffers(h, k & 1, storage);
             bgf::launch_step(h->prec, h->kp, storage, h->d_loop, advance_time, h->stream,
                              h->timing ? h->ev : nullptr);
+            enqueue_stats(h, 1 - (k & 1), advance_time);
         }
         if (h->timing) {
             HIPCHK(hipEventSynchronize(h->ev[3]));
@@ -842,6 +861,64 @@ bgf_status bgf_set_bed_source(bgf_handle h, int nx, int ny, double xo, double yo
     return BGF_OK;
 }
 
+bgf_status bgf_set_stats(bgf_handle h, int mask, double wet_threshold) {
+    if (!h) return fail(BGF_ERR_INVALID_ARG, "null handle");
+    if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
+    if (mask < 0 || mask > 7) return fail(BGF_ERR_INVALID_ARG, "statistics mask must be within 0..7");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const size_t bytes = h->es * (size_t)h->cap * BS;
+    for (int k = 0; k < 15; k++) {
+        const int bit = k < 7 ? 1 : k < 14 ? 2 : 4;
+        if ((mask & bit) && !h->stats.a[k]) {
+            if (hipMalloc(&h->stats.a[k], bytes) != hipSuccess) {
+                h->stats.a[k] = nullptr;
+                return fail(BGF_ERR_OOM, "statistics allocation failed");
+            }
+            HIPCHK(hipMemset(h->stats.a[k], 0, bytes));
+        } else if (!(mask & bit) && h->stats.a[k]) {
+            (void)hipFree(h->stats.a[k]);
+            h->stats.a[k] = nullptr;
+        }
+    }
+    h->stats.mask = mask;
+    h->stats.wet_threshold = wet_threshold;
+    // Initmeanmax (src/Meanmax.cu:191-199): everything reset at the start
+    bgf::launch_stats_reset(h->prec == 0 ? 0 : 1, h->kp, h->stats, 7, h->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->stats_n = 0;
+    if (h->graph) {  // the captured step gains or loses the statistics kernel
+        (void)hipGraphExecDestroy(h->graph);
+        h->graph = nullptr;
+    }
+    return BGF_OK;
+}
+
+bgf_status bgf_reset_stats(bgf_handle h, int which) {
+    if (!h) return fail(BGF_ERR_INVALID_ARG, "null handle");
+    if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
+    bgf::launch_stats_reset(h->prec == 0 ? 0 : 1, h->kp, h->stats, which, h->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (which & 1) h->stats_n = 0;
+    return BGF_OK;
+}
+
+bgf_status bgf_get_stats(bgf_handle h, const char* name, void* out) {
+    if (!h || !name || !out) return fail(BGF_ERR_INVALID_ARG, "null argument");
+    if (h->pending_steps) return fail(BGF_ERR_STATE, "bgf_run_async pending: call bgf_sync first");
+    static const char* names[15] = {"hmean", "zsmean", "umean", "vmean", "Umean", "hUmean", "hUUmean", "hmax",
+                                    "zsmax", "umax",  "vmax",   "Umax",  "hUmax", "hUUmax", "twet"};
+    for (int k = 0; k < 15; k++)
+        if (std::strcmp(name, names[k]) == 0) {
+            if (!h->stats.a[k]) return fail(BGF_ERR_STATE, std::string(name) + " is not being collected");
+            HIPCHK(hipStreamSynchronize(h->stream));
+            HIPCHK(hipMemcpy(out, h->stats.a[k], h->es * h->S, hipMemcpyDeviceToHost));
+            return BGF_OK;
+        }
+    return fail(BGF_ERR_INVALID_ARG, std::string("unknown statistic ") + name);
+}
+
 bgf_status bgf_sample_raster(bgf_handle h, int nx, int ny, double xo, double yo, double dx, const float* values,
                              void* out) {
     if (!h || !values || !out) return fail(BGF_ERR_INVALID_ARG, "null argument");
@@ -971,7 +1048,13 @@ bgf_status bgf_remesh(bgf_handle h, const bgf_param* p, const bgf_blocks* b, con
         }
         nh->riv_box = h->riv_box; nh->riv_n = h->riv_n; nh->riv_t = h->riv_t; nh->riv_q = h->riv_q;
         std::swap(*h, *nh);  // h takes the new mesh; nh (the old one) is freed
+        const int smask = nh->stats.mask;
+        const double sthr = nh->stats.wet_threshold;
         bgf_destroy(nh);
+        if (smask) {  // the statistics restart on the new mesh
+            s = bgf_set_stats(h, smask, sthr);
+            if (s != BGF_OK) return s;
+        }
         if (!h->riv_n.empty()) {
             s = setup_rivers(h);
             if (s != BGF_OK) return s;
@@ -1057,6 +1140,9 @@ bgf_status bgf_remesh(bgf_handle h, const bgf_param* p, const bgf_blocks* b, con
     }
     // the bed's halo on the new mesh (src/InitialConditions.cu:135-147)
     bgf::launch_amr_fill(prec, kp, h->d_zb, h->stream);
+    // the statistics restart on the new mesh
+    bgf::launch_stats_reset(prec, kp, h->stats, 7, h->stream);
+    h->stats_n = 0;
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return fail(BGF_ERR_HIP, std::string("remesh: ") + hipGetErrorString(e));

@@ -33,6 +33,7 @@ struct DevLoop {
     const double* rain_v;
     int rain_n;
     double tcorr;    // time of the corrector's boundary values: the step's start + dt (set by the dt reduction)
+    int stats_n;     // steps since the output statistics' means were reset (Loop.nstep of Calcmeanmax)
     unsigned int slot32[NSLOT];
     unsigned long long slot64[NSLOT];
 };
@@ -204,6 +205,18 @@ struct RemeshLaunch {
     double bxo, byo, bdx, pxo, pyo, dx, eps;
 };
 void launch_remesh(int precision, const KParams& Po, const RemeshLaunch& R, hipStream_t st);
+
+// Output statistics of Calcmeanmax (src/Meanmax.cu:15-112) over the active blocks after a step.
+struct StatsLaunch {
+    int mask;              // 1 means, 2 maxima, 4 wet time
+    double wet_threshold;  // Param.wet_threshold
+    void* a[15];           // mean h, zs, u, v, U, hU, hUU; max h, zs, u, v, U, hU, hUU; wet time
+};
+// advanced: the step advanced totaltime (else the statistics take totaltime + dt as the step's end).
+void launch_stats(int precision, const KParams& P, const StatsLaunch& S, const void* h, const void* u, const void* v,
+                  const void* zb, const DevLoop* L, int advanced, hipStream_t st);
+// Resets the statistics selected by `which` (1 means to 0, 2 maxima to -max(T), 4 wet time to 0).
+void launch_stats_reset(int precision, const KParams& P, const StatsLaunch& S, int which, hipStream_t st);
 // R.bed sampled onto the interior cells of R's mesh (R.active, level, xo, yo): R.nzb = interp2BUQ
 // of the raster at every cell (only the raster, mesh and output fields of R are read).
 void launch_sample_raster(int precision, const RemeshLaunch& R, hipStream_t st);

@@ -45,6 +45,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <limits>
+
 #include "bgf_common.h"
 
 namespace bgf {
@@ -926,12 +928,81 @@ __global__ __launch_bounds__(64) void k_reduce_dt(DevLoop* L, int mode) {
             L->go = 1;
             L->steps_done = L->steps_done + 1;
             L->nstep = L->nstep + 1;
+            L->stats_n = L->stats_n + 1;
         }
         if (mode == 1) {
             L->totaltime = L->totaltime + L->dt;
             if (L->totaltime >= L->endtime) L->done = 1;
         }
     }
+}
+
+// Output statistics after a step (Calcmeanmax, src/Meanmax.cu:15-112), all
+// in one pass over the state: with mask & 1 the sums of h, zs, u, v
+// (addavg_var, :374-400), U = |(u, v)|, h U and h U U assigned as
+// addUandhU does (:495-525), all seven divided by the step count at an
+// output step, nextoutputtime - totaltime <= dt * T(1e-5) (divavg_var,
+// :434-460); with mask & 2 the maxima of h, zs, u, v, U, h U, h (u^2 + v^2)
+// (max_var, max_Norm, max_hU, max_hUU, :560-660); with mask & 4 the wet time
+// (addwettime, :786-818).  zs is zb + h.  IEEE square roots and divisions in
+// every precision.
+template <class T>
+__global__ __launch_bounds__(256) void k_stats(KParams P, StatsLaunch S, const T* __restrict__ hh,
+                                               const T* __restrict__ uu, const T* __restrict__ vv,
+                                               const T* __restrict__ zb, const DevLoop* L, int advanced) {
+    if (!L->go) return;
+    const int ib = P.active[blockIdx.x];
+    const int i = memloc(threadIdx.x & 15, threadIdx.x >> 4, ib);
+    const T h = hh[i], u = uu[i], v = vv[i], zs = zb[i] + h;
+    if (S.mask & 1) {
+        const double tt = advanced ? L->totaltime : L->totaltime + L->dt;
+        const bool divide = L->nextoutputtime - tt <= L->dt * (double)T(0.00001);
+        const T n = T(L->stats_n);
+        const T U = sqrt((u * u) + (v * v));
+        const T m[7] = {((T*)S.a[0])[i] + h, ((T*)S.a[1])[i] + zs, ((T*)S.a[2])[i] + u, ((T*)S.a[3])[i] + v,
+                        U, h * U, h * U * U};
+#pragma unroll
+        for (int k = 0; k < 7; k++) ((T*)S.a[k])[i] = divide ? m[k] / n : m[k];
+    }
+    if (S.mask & 2) {
+        const T U = sqrt((u * u) + (v * v));
+        const T x[7] = {h, zs, u, v, U, h * U, h * (u * u + v * v)};
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            T* const p = (T*)S.a[7 + k];
+            const T o = p[i];
+            p[i] = o < x[k] ? x[k] : o;
+        }
+    }
+    if ((S.mask & 4) && h > T(S.wet_threshold)) ((T*)S.a[14])[i] = ((T*)S.a[14])[i] + T(L->dt);
+}
+
+template <class T> __global__ __launch_bounds__(256) void k_stats_reset(KParams P, StatsLaunch S, int which) {
+    const int ib = P.active[blockIdx.x];
+    const int i = memloc(threadIdx.x & 15, threadIdx.x >> 4, ib);
+    const T neg = -std::numeric_limits<T>::max();  // Loop.hugenegval (src/Mainloop.cu:233-234)
+#pragma unroll
+    for (int k = 0; k < 15; k++) {
+        const int bit = k < 7 ? 1 : k < 14 ? 2 : 4;
+        if ((which & bit) && (S.mask & bit)) ((T*)S.a[k])[i] = (k >= 7 && k < 14) ? neg : T(0);
+    }
+}
+
+void launch_stats(int precision, const KParams& P, const StatsLaunch& S, const void* h, const void* u, const void* v,
+                  const void* zb, const DevLoop* L, int advanced, hipStream_t st) {
+    if (!S.mask || P.nblk <= 0) return;
+    if (precision == 0)
+        hipLaunchKernelGGL(k_stats<double>, dim3(P.nblk), dim3(256), 0, st, P, S, (const double*)h, (const double*)u,
+                           (const double*)v, (const double*)zb, L, advanced);
+    else
+        hipLaunchKernelGGL(k_stats<float>, dim3(P.nblk), dim3(256), 0, st, P, S, (const float*)h, (const float*)u,
+                           (const float*)v, (const float*)zb, L, advanced);
+}
+
+void launch_stats_reset(int precision, const KParams& P, const StatsLaunch& S, int which, hipStream_t st) {
+    if (!S.mask || P.nblk <= 0) return;
+    if (precision == 0) hipLaunchKernelGGL(k_stats_reset<double>, dim3(P.nblk), dim3(256), 0, st, P, S, which);
+    else hipLaunchKernelGGL(k_stats_reset<float>, dim3(P.nblk), dim3(256), 0, st, P, S, which);
 }
 
 // The rain map of a step, at its start: InterpstepCPU (src/GridManip.cu:602-622)

@@ -299,6 +299,23 @@ bgf_status bgf_set_rain_map(bgf_handle h, int nx, int ny, double xo, double yo, 
  * NULL values removes it. */
 bgf_status bgf_set_bed_source(bgf_handle h, int nx, int ny, double xo, double yo, double dx, const float* values);
 
+/* Output statistics (Calcmeanmax, src/Meanmax.cu:15-112), updated on the GPU
+ * after every step: mask 1 the means of h, zs, u, v (sums divided by the
+ * step count at the step that reaches nextoutputtime) with U = |(u, v)|, h U
+ * and h U U as the reference leaves them (assigned each step, then divided);
+ * mask 2 the maxima of h, zs, u, v, U, h U, h U^2 (Param.outmax); mask 4 the
+ * time each cell spent with h > wet_threshold (Param.outtwet).  zs = zb + h.
+ * bgf_set_stats starts them (Initmeanmax: means and wet time 0, maxima
+ * -max(T)); bgf_reset_stats resets those in `which` after an output
+ * (resetmeanmax; the means also restart their step count); bgf_get_stats
+ * copies one, by its output name ("hmean", "zsmean", "umean", "vmean",
+ * "Umean", "hUmean", "hUUmean", "hmax", "zsmax", "umax", "vmax", "Umax",
+ * "hUmax", "hUUmax", "twet"), as nblkmem*324 values of the handle's T.  A
+ * remesh restarts them on the new mesh. */
+bgf_status bgf_set_stats(bgf_handle h, int mask, double wet_threshold);
+bgf_status bgf_reset_stats(bgf_handle h, int which);
+bgf_status bgf_get_stats(bgf_handle h, const char* name, void* out);
+
 /* A raster (values[i + j*nx] at xo + i*dx, yo + j*dx) sampled onto the
  * interior cells of the handle's mesh as interp2BUQ (src/GridManip.cu:333-560:
  * bilinear where the cell is no coarser than the raster, else the mean of the

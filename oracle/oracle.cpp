@@ -2213,6 +2213,57 @@ template <class T> void sample_raster_t(const orc_param* p, const orc_blocks* b,
     }
 }
 
+// Calcmeanmax (src/Meanmax.cu:15-112) after a step, on the model's evolving
+// state: with mask & 1 the sums of h, zs, u, v (addavg_var, :374-400), the
+// velocity magnitude U, h U and h U U assigned (not summed: addUandhU,
+// :495-525, assigns), all seven divided by nstep (the count including this
+// step) at an output step, nextoutputtime - totaltime <= dt * T(1e-5)
+// (divavg_var, :434-460); with mask & 2 the running maxima of h, zs, u, v, U,
+// h U and h (u^2 + v^2) (max_var, max_Norm, max_hU, max_hUU, :560-660); with
+// mask & 4 the wet time, + dt where h > wet_threshold (addwettime, :786-818).
+// a: mean h, zs, u, v, U, hU, hUU, max h, zs, u, v, U, hU, hUU, wet time.
+template <class T> void calcmeanmax_t(Mdl<T>& M, const orc_loop& L, int nstep, int mask, double thr, T* const* a) {
+    const bool divide = L.nextoutputtime - L.totaltime <= L.dt * (double)T(0.00001);
+    auto mx = [](T x, T y) { return x < y ? y : x; };
+    for (int ibl = 0; ibl < M.p.nblk; ibl++) {
+        const int ib = M.b.active[ibl];
+        for (int iy = 0; iy < BW; iy++)
+            for (int ix = 0; ix < BW; ix++) {
+                const int i = memloc(ix, iy, ib);
+                const T h = M.ev.h[i], zs = M.ev.zs[i], u = M.ev.u[i], v = M.ev.v[i];
+                if (mask & 1) {
+                    a[0][i] = a[0][i] + h;
+                    a[1][i] = a[1][i] + zs;
+                    a[2][i] = a[2][i] + u;
+                    a[3][i] = a[3][i] + v;
+                    const T U = std::sqrt((u * u) + (v * v));
+                    a[4][i] = U;
+                    a[5][i] = h * U;
+                    a[6][i] = h * U * U;
+                    if (divide)
+                        for (int k = 0; k < 7; k++) a[k][i] = a[k][i] / T(nstep);
+                }
+                if (mask & 2) {
+                    a[7][i] = mx(a[7][i], h);
+                    a[8][i] = mx(a[8][i], zs);
+                    a[9][i] = mx(a[9][i], u);
+                    a[10][i] = mx(a[10][i], v);
+                    a[11][i] = mx(a[11][i], T(std::sqrt((u * u) + (v * v))));
+                    a[12][i] = mx(a[12][i], T(h * std::sqrt((u * u) + (v * v))));
+                    a[13][i] = mx(a[13][i], T(h * (u * u + v * v)));
+                }
+                if ((mask & 4) && h > T(thr)) a[14][i] = a[14][i] + T(L.dt);
+            }
+    }
+}
+
+extern "C" int orc_calcmeanmax(orc_model m, const orc_loop* loop, int nstep, int mask, double wet_threshold,
+                               void* const* arrays) {
+    if (m->m.dbl) calcmeanmax_t<double>(*m->m.d, *loop, nstep, mask, wet_threshold, (double* const*)arrays);
+    else calcmeanmax_t<float>(*m->m.f, *loop, nstep, mask, wet_threshold, (float* const*)arrays);
+    return 0;
+}
+
 extern "C" int orc_sample_raster(const orc_param* p, const orc_blocks* b, int nx, int ny, double xo, double yo,
                                  double dx, const float* val, void* out) {
     BedSource src{nx, ny, xo, yo, dx, val};

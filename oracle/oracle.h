@@ -112,6 +112,12 @@ int orc_remesh(orc_model m, const orc_param* p, const orc_blocks* b, const int* 
  * values of T (double when p->doubleprecision). */
 int orc_sample_raster(const orc_param* p, const orc_blocks* b, int nx, int ny, double xo, double yo, double dx,
                       const float* val, void* out);
+/* Calcmeanmax (src/Meanmax.cu:15-112) on the model's state after a step:
+ * arrays are 15 nblkmem*324 arrays of T (mean h, zs, u, v, U, hU, hUU; max
+ * h, zs, u, v, U, hU, hUU; wet time); nstep counts the steps since the means
+ * were reset, this one included; mask: 1 means, 2 maxima, 4 wet time. */
+int orc_calcmeanmax(orc_model m, const orc_loop* loop, int nstep, int mask, double wet_threshold,
+                    void* const* arrays);
 /* Spherical metric terms (src/Spherical.cu:15-64). */
 double orc_calccm_d(double Radius, double delta, double yo, int iy);
 double orc_calcfm_d(double Radius, double delta, double yo, double iy);

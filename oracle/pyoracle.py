@@ -99,6 +99,9 @@ def lib():
         L.orc_remesh.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcParam), ctypes.POINTER(OrcBlocks), ip,
                                  ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_void_p]
+        L.orc_calcmeanmax.restype = ctypes.c_int
+        L.orc_calcmeanmax.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_void_p]
         L.orc_sample_raster.restype = ctypes.c_int
         L.orc_sample_raster.argtypes = [ctypes.POINTER(OrcParam), ctypes.POINTER(OrcBlocks), ctypes.c_int, ctypes.c_int,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
@@ -280,6 +283,40 @@ class Oracle:
         self.p = p
         self._keep = keep
         self.size = p.nblkmem * BS
+
+
+class Stats:
+    """The 15 output-statistics arrays of Calcmeanmax for an Oracle (orc_calcmeanmax), with
+    Initmeanmax / resetmeanmax resets (src/Meanmax.cu:129-199)."""
+    NAMES = ("hmean", "zsmean", "umean", "vmean", "Umean", "hUmean", "hUUmean", "hmax", "zsmax", "umax", "vmax",
+             "Umax", "hUmax", "hUUmax", "twet")
+
+    def __init__(self, oracle, mask, wet_threshold=0.1):
+        self.o = oracle
+        self.mask = mask
+        self.thr = wet_threshold
+        T = np.float64 if oracle.p.doubleprecision else np.float32
+        self.T = T
+        self.a = [np.zeros(oracle.size, T) for _ in range(15)]
+        self.n = 0
+        self.reset(7)
+
+    def reset(self, which):
+        for k in range(15):
+            bit = 1 if k < 7 else 2 if k < 14 else 4
+            if which & bit:
+                self.a[k][:] = -np.finfo(self.T).max if bit == 2 else 0
+        if which & 1:
+            self.n = 0
+
+    def update(self, loop):
+        """After a step and totaltime += dt."""
+        self.n += 1
+        ptrs = (ctypes.c_void_p * 15)(*[x.ctypes.data for x in self.a])
+        assert lib().orc_calcmeanmax(self.o.h, ctypes.byref(loop), self.n, self.mask, float(self.thr), ptrs) == 0
+
+    def get(self, name):
+        return self.a[self.NAMES.index(name)]
 
 
 def sample_raster(p, blocks, values, xo, yo, dx):
