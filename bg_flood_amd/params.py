@@ -13,11 +13,14 @@ parsed as std::stod / std::stoi do (the longest numeric prefix after leading whi
 is an error), booleans as readparambool (:2075-2091): 1/true/yes/on and -1/false/no/off, anything
 else keeps the current value.
 
+read_bg_param_all reads every key of the reference's parameter file (Param and Forcing keys, with
+their aliases), and read_timeseries / read_asc read the text series (rain, river, boundary files)
+and ESRI ASCII grids those keys name.
+
 read_bg_param returns the Param fields the step reads, plus the run control and the mesh levels,
 with the sanity rules of checkparamsanity (:1189-1440) that touch them: levels ordered and
 initlevel clamped, zsoffset NaN -> 0, engine 5 -> CFL >= 0.25, an unset endtime -> 1/tiny, and
-delta = dx (dx * Radius * pi / 180 on spherical grids, :1293-1305).  Forcing, boundary and output
-keys are outside this path and ignored.
+delta = dx (dx * Radius * pi / 180 on spherical grids, :1293-1305).
 
 Deliberate differences, each for a reason of this path:
   - engine: the reference maps b/butt/buttinger/1 to engine 1 and every other value to engine 5
@@ -215,3 +218,290 @@ def domain_param(bgp, nblk, nblkmem, precision=None, rain_mmh=0.0):
              spherical=int(bool(bgp["spherical"])), nblk=int(nblk), nblkmem=int(nblkmem), precision=precision,
              rain_on=int(rain_mmh != 0.0), rain=float(rain_mmh))
     return p
+
+
+# ---- the full key set ---------------------------------------------------------------------------
+# Every key of readparamstr (src/ReadInput.cu:162-1188) with its aliases and kind, beyond KEYS above:
+# f/i/b/t as above, s = the text itself, e = the Adaptation criterion list.  Defaults from
+# src/Param.h:10-239.
+MORE_KEYS = {
+    "test": (("test",), "i"),
+    "GPUDEVICE": (("GPUDEVICE", "gpu"), "i"),
+    "adaptmaxiteration": (("adaptmaxiteration", "maxiterationadapt"), "i"),
+    "conserveElevation": (("conserveElevation",), "b"),
+    "membuffer": (("membuffer",), "f"),
+    "VelThreshold": (("VelThreshold", "vthresh", "vmax", "velmax"), "f"),
+    "Cd": (("Cd",), "f"),
+    "Pa2m": (("Pa2m",), "f"),
+    "Paref": (("Paref",), "f"),
+    "dt": (("dt",), "f"),
+    "dtmin": (("dtmin",), "f"),
+    "bndtaper": (("bndtaper",), "f"),
+    "bndrelaxtime": (("bndrelaxtime",), "f"),
+    "bndfiltertime": (("bndfiltertime",), "f"),
+    "reftime": (("reftime", "referencetime", "timeref"), "s"),
+    "outfile": (("outfile",), "s"),
+    "resetmax": (("resetmax",), "resetmax"),
+    "outishift": (("outishift",), "i"),
+    "outjshift": (("outjshift",), "i"),
+    "nx": (("nx",), "i"),
+    "ny": (("ny",), "i"),
+    "grdalpha": (("grdalpha",), "f"),
+    "xmax": (("xmax",), "f"),
+    "ymax": (("ymax",), "f"),
+    "rho": (("rho",), "f"),
+    "smallnc": (("smallnc",), "i"),
+    "scalefactor": (("scalefactor",), "f"),
+    "addoffset": (("addoffset",), "f"),
+    "posdown": (("posdown",), "i"),
+    "use_catalyst": (("use_catalyst",), "i"),
+    "catalyst_python_pipeline": (("catalyst_python_pipeline",), "s"),
+    "vtk_output_frequency": (("vtk_output_frequency",), "i"),
+    "vtk_output_time_interval": (("vtk_output_time_interval",), "f"),
+    "vtk_outputfile_root": (("vtk_outputfile_root",), "s"),
+    "python_pipeline": (("python_pipeline",), "s"),
+    "rainbnd": (("rainbnd", "rainonbnd"), "b"),
+    "hotstartfile": (("hotstartfile",), "s"),
+    "hotstep": (("hotstep",), "i"),
+    "crs_ref": (("crs_ref", "crs", "spatialref", "spatial_ref", "wtk", "crsinfo", "crs_info"), "s"),
+    "Toutput": (("Toutput",), "s"),
+    "savebyblk": (("savebyblk", "writebyblk", "saveperblk", "writeperblk", "savebyblock", "writebyblock",
+                   "saveperblock", "writeperblock"), "b"),
+    "Adaptation": (("Adaptation",), "e"),
+    "il": (("il", "Rain_il", "initialloss"), "cf"),
+    "cl": (("cl", "Rain_cl", "continuousloss"), "cf"),
+}
+MORE_DEFAULTS = dict(test=-1, GPUDEVICE=0, adaptmaxiteration=20, conserveElevation=False, membuffer=1.05,
+                     VelThreshold=-1.0, Cd=0.002, Pa2m=0.00009916, Paref=101300.0, dt=0.0, dtmin=0.0005,
+                     bndtaper=0.0, bndrelaxtime=3600.0, bndfiltertime=60.0, reftime="", outfile="Output.nc",
+                     resetmax=False, outishift=0, outjshift=0, nx=0, ny=0, grdalpha=math.nan, xmax=math.nan,
+                     ymax=math.nan, rho=1025.0, smallnc=0, scalefactor=0.01, addoffset=0.0, posdown=0,
+                     use_catalyst=0, catalyst_python_pipeline="", vtk_output_frequency=0,
+                     vtk_output_time_interval=1.0, vtk_outputfile_root="bg_out", python_pipeline="",
+                     rainbnd=False, hotstartfile="", hotstep=0, crs_ref="no_crs", Toutput="", savebyblk=True,
+                     Adaptation=None, il=0.0, cl=0.0)
+
+# outvars accepted by the reference (src/ReadInput.cu:514)
+SUPPORTED_OUTVARS = ("zb", "zs", "u", "v", "h", "hmean", "zsmean", "umean", "vmean", "hUmean", "hUUmean", "Umean",
+                     "hmax", "zsmax", "umax", "vmax", "hUmax", "hUUmax", "Umax", "twet", "dhdx", "dhdy", "dzsdx",
+                     "dzsdy", "dzbdx", "dzbdy", "dudx", "dudy", "dvdx", "dvdy", "Fhu", "Fhv", "Fqux", "Fqvy", "Fquy",
+                     "Fqvx", "Su", "Sv", "dh", "dhu", "dhv", "cf", "Patm", "datmpdx", "datmpdy", "il", "cl", "hgw",
+                     "hu", "hv", "hfu", "hfv", "hau", "hav", "Fux", "Fvx", "Fuy", "Fvy")
+MEAN_VARS = ("hmean", "zsmean", "umean", "vmean", "Umean", "hUmean", "hUUmean")
+MAX_VARS = ("hmax", "zsmax", "umax", "vmax", "Umax", "hUmax", "hUUmax")
+
+# forcing keys (readparamstr for Forcing<T>, src/ReadInput.cu:883-1188)
+FORCING_KEYS = {
+    "bathy": ("Bathy", "bathyfile", "bathymetry", "depfile", "depthfile", "topofile", "topo", "DEM"),
+    "AOI": ("AOI", "aoipoly"),
+    "left": ("left", "leftbndfile", "leftbnd"),
+    "right": ("right", "rightbndfile", "rightbnd"),
+    "top": ("top", "topbndfile", "topbnd"),
+    "bot": ("bot", "botbndfile", "botbnd", "bottom"),
+    "bnd": ("bnd", "bndseg"),
+    "deform": ("deform",),
+    "cavity": ("cavity",),
+    "rivers": ("rivers", "river"),
+    "wind": ("Wind", "windfiles"),
+    "atmp": ("Atmp", "atmpfile"),
+    "rain": ("Rain", "rainfile"),
+}
+
+
+def file_spec(value):
+    """readfileinfo's split of "file?var" (src/ReadForcing.cu): the file, the variable (or ""), and
+    the extension."""
+    parts = value.split("?")
+    f = _trim(parts[0])
+    var = _trim(parts[1]) if len(parts) > 1 else ""
+    ext = f.rsplit(".", 1)[-1] if "." in f else ""
+    return dict(file=f, var=var, ext=ext)
+
+
+def _bnd_side(value, side):
+    """readbndlineside (src/ReadInput.cu:1952-1990): "type", "file,type" or "type,file"."""
+    items = _split(value, ",")
+    out = dict(side=side, type=None, file=None)
+    if len(items) == 1:
+        out["type"] = stoi(items[0])
+    elif len(items) >= 2:
+        if items[1][:1].isdigit():
+            out["type"], out["file"] = stoi(items[1]), _trim(items[0])
+        else:
+            out["type"], out["file"] = stoi(items[0]), _trim(items[1])
+    return out
+
+
+def read_bg_param_all(text=None, path=None):
+    """Every key of a BG_param.txt (readparamstr for Param and Forcing, src/ReadInput.cu:162-1188):
+    {"param": all Param fields (the path's after checkparamsanity's rules, as read_bg_param), "outvars",
+    "outmean", "outmax", "outtwet", "TSnodesout", "outzone", "forcing": {bathy, AOI, boundaries,
+    bnd, deform, cavity, rivers, wind, atmp, rain, cfmap, ilmap, clmap}, "unused": lines setting no key}."""
+    if text is None:
+        with open(path) as f:
+            text = f.read()
+    p = parse_lines(text)
+    p.update(MORE_DEFAULTS)
+    out = dict(outvars=[], outmean=False, outmax=False, outtwet=False, TSnodesout=[], outzone=[], unused=[])
+    frc = dict(bathy=[], AOI=None, boundaries=[], bnd=[], deform=[], cavity=[], rivers=[], wind=None, atmp=None,
+               rain=None, cfmap=None, ilmap=None, clmap=None)
+    for line in text.splitlines():
+        used = False
+        for name, (aliases, kind) in list(KEYS.items()):
+            if find_parameter(aliases, line):
+                used = True
+        for name, (aliases, kind) in MORE_KEYS.items():
+            v = find_parameter(aliases, line)
+            if not v:
+                continue
+            used = True
+            if kind == "f":
+                p[name] = stod(v)
+            elif kind == "i":
+                p[name] = stoi(v)
+            elif kind == "b":
+                p[name] = read_param_bool(v, p[name])
+            elif kind == "s":
+                p[name] = v
+            elif kind == "resetmax":
+                if stoi(v) == 1:
+                    p[name] = True
+            elif kind == "e":
+                p[name] = [_trim(x) for x in _split(v, ",")]
+            elif kind == "cf":
+                if any(c.isalpha() for c in v):
+                    frc[name + "map"] = file_spec(v)
+                else:
+                    p[name] = stod(v)
+        # the roughness map of the cf key (src/ReadInput.cu:1049-1066)
+        v = find_parameter(KEYS["cf"][0], line)
+        if v and any(c.isalpha() for c in v):
+            frc["cfmap"] = file_spec(v)
+        v = find_parameter(("outvars",), line)
+        if v:
+            used = True
+            for var in _split(v, ","):
+                var = _trim(var)
+                if var in SUPPORTED_OUTVARS:
+                    out["outvars"].append(var)
+                out["outmean"] = out["outmean"] or var in MEAN_VARS
+                out["outmax"] = out["outmax"] or var in MAX_VARS
+                out["outtwet"] = out["outtwet"] or var == "twet"
+        v = find_parameter(("TSnodesout", "TSOutput"), line)
+        if v:
+            used = True
+            items = _split(v, ",")
+            if len(items) < 3:
+                raise ValueError("TSOutput needs filename, x, y: %r" % v)
+            out["TSnodesout"].append(dict(name=_trim(items[0]), x=stod(items[1]), y=stod(items[2])))
+        v = find_parameter(("outzone",), line)
+        if v:
+            used = True
+            items = _split(v, ",")
+            if len(items) < 5:
+                raise ValueError("outzone needs filename, xstart, xend, ystart, yend: %r" % v)
+            out["outzone"].append(dict(name=_trim(items[0]), xstart=stod(items[1]), xend=stod(items[2]),
+                                       ystart=stod(items[3]), yend=stod(items[4]), Toutput=",".join(items[5:])))
+        for key, aliases in FORCING_KEYS.items():
+            v = find_parameter(aliases, line)
+            if not v:
+                continue
+            used = True
+            if key == "bathy":
+                frc["bathy"].append(file_spec(v))
+            elif key in ("left", "right", "top", "bot"):
+                frc["boundaries"].append(_bnd_side(v, key))
+            elif key == "bnd":
+                frc["bnd"].append(v)
+            elif key in ("deform", "cavity"):
+                items = _split(v, ",")
+                d = file_spec(items[0])
+                d.update(start=stod(items[1]) if len(items) > 1 else 0.0,
+                         duration=stod(items[2]) if len(items) > 2 else 0.0)
+                frc[key].append(d)
+            elif key == "rivers":
+                items = _split(v, ",")
+                if len(items) != 5:
+                    raise ValueError("river needs filename, xstart, xend, ystart, yend: %r" % v)
+                frc["rivers"].append(dict(file=_trim(items[0]), xstart=stod(items[1]), xend=stod(items[2]),
+                                          ystart=stod(items[3]), yend=stod(items[4])))
+            elif key == "wind":
+                frc["wind"] = [file_spec(x) for x in _split(v, ",")]
+            elif key == "AOI":
+                frc["AOI"] = v
+            else:
+                frc[key] = file_spec(v)
+        if not used and _split(line, "=") and len(_split(line, "=")) >= 2 and not line.lstrip().startswith("#"):
+            out["unused"].append(line)
+    out["param"] = sanity(p)
+    out["forcing"] = frc
+    return out
+
+
+def read_timeseries(path=None, text=None):
+    """A two-or-more-column text series (rain, river discharge and boundary water-level files:
+    time then values, whitespace, tab or comma separated; '#' lines and blank lines skipped), as
+    an (n, k) float64 array."""
+    if text is None:
+        with open(path) as f:
+            text = f.read()
+    rows = []
+    for line in text.splitlines():
+        s = line.strip()
+        if not s or s.startswith("#"):
+            continue
+        vals = [t for t in re.split(r"[\s,;]+", s) if t]
+        rows.append([float(t) for t in vals])
+    if not rows:
+        raise ValueError("no values in the series")
+    width = min(len(r) for r in rows)
+    return __import__("numpy").array([r[:width] for r in rows], dtype=float)
+
+
+def read_asc(path=None, text=None):
+    """An ESRI ASCII grid as the reference reads it (readbathyASCHead / readbathyASCzb,
+    src/ReadForcing.cu:2067-2210): six header lines (ncols, nrows, xll..., yll..., cellsize,
+    nodata), then rows from the top.  Returns (x, y, z[ny, nx] float32) with z[0] the southern
+    row.  The reference's origin rule is kept: a yllcenter or xllcorner key marks the grid as
+    cell-corner registered and shifts the origin by half a cell, so a file with xllcenter and
+    yllcenter has its first node at xllcenter + dx/2."""
+    import numpy as np
+    if text is None:
+        with open(path) as f:
+            text = f.read()
+    lines = text.splitlines()
+    head = {}
+    k = 0
+    nh = 0
+    while nh < 6 and k < len(lines):
+        line = lines[k]
+        k += 1
+        if not line:
+            continue
+        el = _split(line, " ")
+        if len(el) < 2:
+            el = _split(line, "\t")
+        head[_trim(el[0])] = el[1]
+        nh += 1
+    nx, ny = stoi(head["ncols"]), stoi(head["nrows"])
+    dx = stod(head["cellsize"])
+    xo = yo = 0.0
+    pixelreg = True
+    if "xllcenter" in head:
+        xo = stod(head["xllcenter"])
+    if "yllcenter" in head:
+        pixelreg = False
+        yo = stod(head["yllcenter"])
+    if "xllcorner" in head:
+        pixelreg = False
+        xo = stod(head["xllcorner"])
+    if "yllcorner" in head:
+        yo = stod(head["yllcorner"])
+    if not pixelreg:
+        xo += 0.5 * dx
+        yo += 0.5 * dx
+    vals = np.array(" ".join(lines[k:]).split(), dtype=np.float64)[:nx * ny]
+    if vals.size < nx * ny:
+        raise ValueError("ASCII grid has %d values, expected %d" % (vals.size, nx * ny))
+    z = vals.reshape(ny, nx)[::-1].astype(np.float32)
+    return xo + dx * np.arange(nx), yo + dx * np.arange(ny), z

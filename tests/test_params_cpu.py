@@ -3,6 +3,7 @@ sanity rules for the step's parameters (:1189-1440), on the text of the referenc
 parameter files (Examples/*/BG_param.txt, quoted in part) and on edge cases whose outcome follows
 from those rules."""
 import math
+import os
 
 import pytest
 
@@ -153,3 +154,90 @@ def test_domain_param_builds_the_c_struct(tmp_path):
     assert P.spherical == 1 and P.Radius == 6371220.0 and P.nblk == 4 and P.nblkmem == 5
     with pytest.raises(ValueError):
         params.domain_param(params.read_bg_param(CONICAL), 1, 1)
+
+
+def test_full_key_set_examples():
+    r = params.read_bg_param_all(CONICAL)
+    assert r["forcing"]["bathy"] == [dict(file="Bathy_CI.asc", var="", ext="asc")]
+    assert r["forcing"]["boundaries"] == [dict(side="left", type=2, file=None)]
+    # outvars keeps the names the reference supports: hh, uu, vv are dropped (src/ReadInput.cu:514-522)
+    assert r["outvars"] == ["zs", "zb"] and not r["outmax"]
+    assert r["param"]["GPUDEVICE"] == -1 and r["param"]["posdown"] == -1
+    s = params.read_bg_param_all(SIMPLERAIN)
+    assert s["forcing"]["bathy"] == [dict(file="PortCharles_DEM.nc", var="z", ext="nc")]
+    assert s["forcing"]["rain"] == dict(file="rain.txt", var="", ext="txt")
+    assert s["param"]["cf"] == 0.04 and s["param"]["engine"] == 5
+    t = params.read_bg_param_all(TPAC)
+    assert t["param"]["spherical"] and t["param"]["yo"] == -78.0 and t["param"]["ymax"] == 14.32
+
+
+def test_full_key_set_rules():
+    text = """outvars = zs, hmax, twet, Umean, bogus;
+TSOutput = gauge1.txt, 10.5, 20.25
+TSnodesout = g2.txt,1,2
+outzone = zone.nc, 0, 100, 0, 50
+river = flow.txt, 1.0, 2.0, 3.0, 4.0
+right = 3, bnd_right.txt
+top = bnd_top.nc,2
+cf = roughness.nc?n
+il = 5.0
+hotstartfile = hot.nc
+hotstep = 3
+resetmax = 1
+smallnc = 1; scalefactor = 0.001
+Adaptation = Threshold, zb, 10
+savebyblk = no
+crs = EPSG:2193
+somethingelse = 1
+"""
+    r = params.read_bg_param_all(text)
+    assert r["outvars"] == ["zs", "hmax", "twet", "Umean"]
+    assert r["outmean"] and r["outmax"] and r["outtwet"]
+    assert r["TSnodesout"] == [dict(name="gauge1.txt", x=10.5, y=20.25), dict(name="g2.txt", x=1.0, y=2.0)]
+    assert r["outzone"][0]["name"] == "zone.nc" and r["outzone"][0]["xend"] == 100.0
+    assert r["forcing"]["rivers"] == [dict(file="flow.txt", xstart=1.0, xend=2.0, ystart=3.0, yend=4.0)]
+    assert dict(side="right", type=3, file="bnd_right.txt") in r["forcing"]["boundaries"]
+    assert dict(side="top", type=2, file="bnd_top.nc") in r["forcing"]["boundaries"]
+    assert r["forcing"]["cfmap"] == dict(file="roughness.nc", var="n", ext="nc")
+    p = r["param"]
+    assert p["il"] == 5.0 and p["hotstartfile"] == "hot.nc" and p["hotstep"] == 3 and p["resetmax"]
+    assert p["smallnc"] == 1  # the ';' cut keeps the first piece only
+    assert p["scalefactor"] == 0.01
+    assert p["Adaptation"] == ["Threshold", "zb", "10"]
+    assert p["savebyblk"] is False and p["crs_ref"] == "EPSG:2193"
+    assert r["unused"] == ["somethingelse = 1"]
+    # "file, type" with a space before the type: the reference's isdigit test fails and stoi of the
+    # file name throws (src/ReadInput.cu:1960-1972)
+    with pytest.raises(ValueError):
+        params.read_bg_param_all("top = bnd_top.nc, 2")
+
+
+def test_timeseries_reader():
+    a = params.read_timeseries(text="# rain\n0.0\t317.0\n45.0 317.0\n\n45.1,0.0\n150.0 0.0")
+    assert a.shape == (4, 2) and a[2, 0] == 45.1 and a[0, 1] == 317.0
+
+
+def test_asc_reader_rules():
+    body = "1 2 3\n4 5 6\n"
+    x, y, z = params.read_asc(text="ncols 3\nnrows 2\nxllcorner 10\nyllcorner 20\ncellsize 2\nnodata_value -9999\n" + body)
+    # corner-registered: first node half a cell in; rows stored from the south
+    assert list(x) == [11.0, 13.0, 15.0] and list(y) == [21.0, 23.0]
+    assert z.tolist() == [[4, 5, 6], [1, 2, 3]]
+    x, y, z = params.read_asc(text="ncols 3\nnrows 2\nxllcenter 10\nyllcenter 20\ncellsize 2\nnodata_value -9999\n" + body)
+    # the reference's rule: yllcenter also marks the grid corner-registered
+    assert list(x) == [11.0, 13.0, 15.0] and list(y) == [21.0, 23.0]
+
+
+RAIN_CEA = "/root/reference/Examples/Rain_Cea2008"
+
+
+@pytest.mark.skipif(not os.path.exists(RAIN_CEA), reason="the reference's example data is not present")
+def test_reference_rain_example_files():
+    r = params.read_bg_param_all(path=os.path.join(RAIN_CEA, "BG_param.txt"))
+    assert r["outvars"] == ["zs", "zb", "zsmax"] and r["outmax"]
+    assert r["forcing"]["boundaries"] == [dict(side="bot", type=1, file=None)]
+    assert r["param"]["doubleprecision"] == 1 and r["param"]["theta"] == 1.14 and r["param"]["frictionmodel"] == 1
+    s = params.read_timeseries(os.path.join(RAIN_CEA, r["forcing"]["rain"]["file"]))
+    assert s.tolist() == [[0.0, 317.0], [45.0, 317.0], [45.1, 0.0], [150.0, 0.0]]
+    x, y, z = params.read_asc(os.path.join(RAIN_CEA, "topo.asc"))
+    assert z.shape == (256, 208) and abs(x[1] - x[0] - 0.01) < 1e-12
