@@ -212,6 +212,9 @@ extern "C" {
 
 bgf_status bgf_batch_create(const bgf_param* common, int max_blocks, int max_members, bgf_batch* out) {
     if (!out || max_blocks <= 0 || max_members <= 0) return fail(BGF_ERR_INVALID_ARG, "bad batch arguments");
+    if (max_blocks > kMaxBlocks)
+        return fail(BGF_ERR_UNSUPPORTED, "max_blocks above " + std::to_string(kMaxBlocks) +
+                                             ": cell indices would not fit in 32 bits");
     *out = nullptr;
     bgf_param p = *common;
     p.nblk = 1;

@@ -988,7 +988,7 @@ bgf_status bgf_remesh(bgf_handle h, const bgf_param* p, const bgf_blocks* b, con
     if (nbm > h->cap) {
         // grow: a handle with room for half as many blocks again, the state moved into it
         bgf_handle nh = nullptr;
-        s = create_handle(&q, b, nbm + nbm / 2, &nh);
+        s = create_handle(&q, b, (int)std::min<long long>((long long)nbm + nbm / 2, kMaxBlocks), &nh);
         if (s != BGF_OK) return s;
         bool ok = hipMalloc((void**)&nh->d_remap, sizeof(int) * 5 * (size_t)nh->cap) == hipSuccess &&
                   hipMalloc((void**)&nh->d_bxo, sizeof(double) * nh->cap) == hipSuccess &&

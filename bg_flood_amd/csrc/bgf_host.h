@@ -93,11 +93,18 @@ inline std::vector<T> metric_table(const bgf_param* p, const bgf_blocks* b, int 
     return M;
 }
 
+// Cell indices are 32-bit ints, as memloc's (src/MemManagement.cu:37-40):
+// at most this many allocated blocks (1.7e9 cells).
+constexpr int kMaxBlocks = 2147483647 / BS;
+
 inline bgf_status check_param(const bgf_param* p) {
     if (!p) return fail(BGF_ERR_INVALID_ARG, "null param");
     if (p->engine != 2) return fail(BGF_ERR_UNSUPPORTED, "only engine 2 (Kurganov) is provided");
     if (p->spherical && !(p->Radius > 0.0)) return fail(BGF_ERR_INVALID_ARG, "spherical grids need Radius > 0");
     if (p->nblk <= 0 || p->nblkmem < p->nblk) return fail(BGF_ERR_INVALID_ARG, "need 0 < nblk <= nblkmem");
+    if (p->nblkmem > kMaxBlocks)
+        return fail(BGF_ERR_UNSUPPORTED, "nblkmem above " + std::to_string(kMaxBlocks) +
+                                             " blocks: cell indices would not fit in 32 bits");
     if (p->precision < 0 || p->precision > 3) return fail(BGF_ERR_INVALID_ARG, "bad precision");
     if (p->aoibnd != 0 && p->aoibnd != 1)
         return fail(BGF_ERR_UNSUPPORTED, "remainder boundary type must be 0 (wall) or 1 (Neumann)");

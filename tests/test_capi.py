@@ -62,3 +62,15 @@ def test_domain_without_device_fails_loudly():
     sc = scenarios.gaussian_hump(n=32)
     with pytest.raises(bgf.BgfError):
         bgf.Domain(sc["param"], sc["blocks"])
+
+
+def test_index_range_guard():
+    """Cell indices are 32-bit as memloc's: more than 2147483647 // 324 allocated blocks is refused."""
+    import ctypes
+    p = bgf.default_param(dx=1.0, nblk=1, nblkmem=2147483647 // 324 + 1, precision="fp32")
+    P = bgf._param_struct(p)
+    keep = {}
+    B = bgf._blocks_struct(bgf.mesh_uniform(1, 1, 1.0), 1, keep)
+    h = ctypes.c_void_p()
+    rc = bgf.lib().bgf_create(ctypes.byref(P), ctypes.byref(B), ctypes.byref(h))
+    assert rc == 5 and b"32 bits" in bgf.lib().bgf_last_error()
