@@ -447,6 +447,13 @@ class Domain:
         ny, nx = a.shape
         _check(lib().bgf_set_bed_source(self.h, nx, ny, float(xo), float(yo), float(dx), a.ctypes.data))
 
+    def device_array(self, name):
+        """Device address of one of the handle's arrays ("h", "u", "v", "zb", "cf", "dh", "dhu",
+        "dhv"), for callers that keep their own views (bgf_device_array)."""
+        p = ctypes.c_void_p()
+        _check(lib().bgf_device_array(self.h, name.encode(), ctypes.byref(p)))
+        return p.value
+
     STATS = ("hmean", "zsmean", "umean", "vmean", "Umean", "hUmean", "hUUmean", "hmax", "zsmax", "umax", "vmax",
              "Umax", "hUmax", "hUUmax", "twet")
 

@@ -31,6 +31,7 @@ struct bgf_handle_s {
     int *d_active = nullptr, *d_order = nullptr, *d_level = nullptr;
     int* d_queue = nullptr;  // work-queue counters of the persistent kernels
     int* d_rec = nullptr;    // per-position block records of the processing order
+    bool cf_shared = false;  // the caller holds the cf device array
     int *d_nb[8] = {nullptr};  // LeftBot LeftTop RightBot RightTop TopLeft TopRight BotLeft BotRight
     void *d_h[2] = {nullptr, nullptr}, *d_u[2] = {nullptr, nullptr}, *d_v[2] = {nullptr, nullptr};
     void *d_zb = nullptr, *d_cf = nullptr, *d_dh = nullptr, *d_dhu = nullptr, *d_dhv = nullptr;
@@ -579,6 +580,31 @@ bgf_status bgf_set_state(bgf_handle h, const void* zb, const void* hh, const voi
     HIPCHK(hipMemcpyAsync(h->d_v[0], v, bytes, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(h->d_cf, cf, bytes, hipMemcpyHostToDevice, h->stream));
     if (h->d_ch) HIPCHK(hipMemsetAsync(h->d_ch, 0, h->es * h->S, h->stream));
+    // A roughness that is one value over the active cells (a constant Param.cf, the usual case) is
+    // passed to the one-level corrector as that value instead of being read per cell; not once the
+    // caller holds the device array (bgf_device_array), which it may change.
+    {
+        int uni = !h->cf_shared;
+        double val = 0.0;
+        for (int ibl = 0; uni && ibl < h->nblk; ibl++) {
+            const int ib = h->active[ibl];
+            for (int j = 0; uni && j < BW; j++)
+                for (int i = 0; i < BW; i++) {
+                    const size_t n = (size_t)memloc(i, j, ib);
+                    const double c = h->prec == 0 ? ((const double*)cf)[n] : (double)((const float*)cf)[n];
+                    if (ibl == 0 && i == 0 && j == 0) val = c;
+                    else if (!(c == val) || std::signbit(c) != std::signbit(val)) { uni = 0; break; }
+                }
+        }
+        if (uni != h->kp.cf_uniform || (uni && val != h->kp.cf_value)) {
+            h->kp.cf_uniform = uni;
+            h->kp.cf_value = uni ? val : 0.0;
+            if (h->graph) {  // the captured launches carry the old parameters
+                (void)hipGraphExecDestroy(h->graph);
+                h->graph = nullptr;
+            }
+        }
+    }
     if (h->amr) {
         // halo of the bed, as InitzbHalo (src/InitialConditions.cu:135-147)
         bgf::launch_amr_fill(h->prec == 0 ? 0 : 1, h->kp, h->d_zb, h->stream);
@@ -1181,7 +1207,17 @@ bgf_status bgf_device_array(bgf_handle h, const char* name, void** devptr) {
     else if (n == "uo") p = h->d_u[1];
     else if (n == "vo") p = h->d_v[1];
     else if (n == "zb") p = h->d_zb;
-    else if (n == "cf") p = h->d_cf;
+    else if (n == "cf") {
+        p = h->d_cf;
+        h->cf_shared = true;  // the caller may write it: read it per cell from now on
+        if (h->kp.cf_uniform) {
+            h->kp.cf_uniform = 0;
+            if (h->graph) {
+                (void)hipGraphExecDestroy(h->graph);
+                h->graph = nullptr;
+            }
+        }
+    }
     else if (n == "dh") p = h->d_dh;
     else if (n == "dhu") p = h->d_dhu;
     else if (n == "dhv") p = h->d_dhv;

@@ -56,6 +56,8 @@ struct KParams {
     int frictionmodel, aoibnd, ForceMassConserve, rain_on, nblk, wetdryfix;
     int grid_pred, grid_corr;  // persistent workgroup counts (multiples of 8)
     int* queue;                // the persistent kernels' 16 work-queue counters, QSTRIDE apart (zeroed at allocation)
+    int cf_uniform;            // every active cell has roughness cf_value: the one-level corrector skips the cf array
+    double cf_value;
     const int* active;  // active blocks in the reference order
     const int* order;   // the same blocks in the persistent kernels' processing order
     const int* rec;     // per position of the order: block, LeftBot, RightBot, BotLeft, TopLeft, member, 0, 0

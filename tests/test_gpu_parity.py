@@ -79,6 +79,37 @@ def test_exact_wet_dry_rain_quadratic_friction(prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_exact_varying_roughness(prec):
+    """A roughness map (cf varying per cell) is read per cell; the uniform-roughness shortcut of the
+    one-level corrector must not apply."""
+    sc = scenarios.rain_terrain(n=96, dx=5.0, amp=4.0, rain=3600.0, manning=0.01, seed=5, precision=prec, aoibnd=1)
+    sc["param"]["frictionmodel"] = 0
+    cf = np.asarray(sc["cf"]).copy()
+    cf *= (1.0 + 0.5 * np.sin(np.arange(cf.size) * 0.37)).astype(cf.dtype)
+    sc["cf"] = cf
+    side_by_side(sc, prec, 30)
+
+
+@pytest.mark.parametrize("prec", ["fp32_fast", "fp32"])
+def test_uniform_roughness_shortcut_equals_array(prec):
+    """A uniform roughness passed as one value gives the same state, bit for bit, as the same
+    roughness read per cell (forced by handing out the cf device array)."""
+    sc = scenarios.rain_terrain(n=96, dx=5.0, amp=4.0, rain=200.0, manning=0.03, seed=8, precision="fp32")
+    out = []
+    for share in (False, True):
+        d = domain_for(sc, precision=prec)
+        if share:
+            d.device_array("cf")
+            d.set_state(*[np.asarray(sc[k]).astype(d.dtype) for k in ("zb", "h", "u", "v", "cf")])
+        lp = d.init_dt(bgf.Loop(nextoutputtime=1e9, dt=0, dtmax=0, totaltime=0))
+        d.run(lp, 30)
+        out.append((d.get_state()["h"], lp.totaltime))
+        d.close()
+    assert out[0][1] == out[1][1]
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_exact_walls_wet_dry(prec):
     sc = scenarios.lake_at_rest(n=48, dx=2.0, level=0.3, amp=1.0, seed=7, precision=prec, aoibnd=0)
     # disturb the lake: a raised patch of water
