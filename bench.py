@@ -373,7 +373,10 @@ def main():
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=tr, kernel=dom,
                 kernel_ms=round(kms, 4), alg_bytes_per_launch=alg_bytes,
-                step_frac=round(r["gcups"] * 1e9 * 18 * es / (HBM_PEAK_GBS * 1e9), 4))
+                step_frac=round(r["gcups"] * 1e9 * 18 * es / (HBM_PEAK_GBS * 1e9), 4),
+                note="achieved = the 11-word algorithmic bytes of the corrector (h, u, v, dh, dhu, dhv, zb, cf "
+                     "read; h, u, v written) over its duration; the roughness here is uniform and is passed "
+                     "as a value, so the kernel reads 7 of those 8 input arrays (traffic is what it moves)")
 
     variants = {}
     if not args.no_variants:
