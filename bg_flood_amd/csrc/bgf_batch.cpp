@@ -405,6 +405,19 @@ bgf_status bgf_batch_add(bgf_batch b, const bgf_param* p, const bgf_blocks* blk,
     std::vector<int> nog(NGAUGE, -1);
     HIPCHK(hipMemcpy(b->d_gcell + NGAUGE * m, nog.data(), sizeof(int) * NGAUGE, hipMemcpyHostToDevice));
     L.done = L.totaltime >= L.endtime ? 1 : 0;
+    // a roughness that is one value over the member's active cells reaches the corrector as that value
+    L.cf_uniform = 1;
+    for (int ibl = 0; L.cf_uniform && ibl < p->nblk; ibl++) {
+        const int ib = blk->active[ibl];
+        for (int j = 0; L.cf_uniform && j < BW; j++)
+            for (int i = 0; i < BW; i++) {
+                const size_t n = (size_t)memloc(i, j, ib);
+                const double c = b->prec == 0 ? ((const double*)cf)[n] : (double)((const float*)cf)[n];
+                if (ibl == 0 && i == 0 && j == 0) L.cf_value = c;
+                else if (!(c == L.cf_value) || std::signbit(c) != std::signbit(L.cf_value)) { L.cf_uniform = 0; break; }
+            }
+    }
+    if (!L.cf_uniform) L.cf_value = 0.0;
     b->loops[m] = L;
     *member = m;
     return BGF_OK;

@@ -34,6 +34,8 @@ struct DevLoop {
     int rain_n;
     double tcorr;    // time of the corrector's boundary values: the step's start + dt (set by the dt reduction)
     int stats_n;     // steps since the output statistics' means were reset (Loop.nstep of Calcmeanmax)
+    int cf_uniform;  // batch member: every active cell has roughness cf_value (the corrector skips the cf array)
+    double cf_value;
     unsigned int slot32[NSLOT];
     unsigned long long slot64[NSLOT];
 };

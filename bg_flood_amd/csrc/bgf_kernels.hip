@@ -742,7 +742,7 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
         ph = ldc(B.h, ib, kb); pu = ldc(B.u, ib, kb); pv = ldc(B.v, ib, kb);
         pdh = ldc(B.dh, ib, kb); pdhu = ldc(B.dhu, ib, kb); pdhv = ldc(B.dhv, ib, kb);
         pz = ldc(B.zb, ib, kb);
-        if (!P.cf_uniform) pcf = ldn(B.cf, ib, kb);
+        if (!(BATCH ? ldu(&L[n.sid].cf_uniform, 0) : P.cf_uniform)) pcf = ldn(B.cf, ib, kb);
         if (t < 128) {
             const int src = strip_src(t, n.ib, n.nL, n.nR, n.nB, n.nT);
             sh = B.h[src]; su = B.u[src]; sv = B.v[src]; sdh = B.dh[src]; sdhu = B.dhu[src]; sdhv = B.dhv[src];
@@ -774,7 +774,8 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
         const T dt = T(Lm->dt);
         const T hdt = dt * T(0.5);
         // this block's U^n (kept for the full-step advance) and its U* tile
-        const T h0 = ph, u0 = pu, v0 = pv, zb0 = pz, cf0 = P.cf_uniform ? T(P.cf_value) : pcf;
+        const bool cfu = BATCH ? Lm->cf_uniform : P.cf_uniform;
+        const T h0 = ph, u0 = pu, v0 = pv, zb0 = pz, cf0 = cfu ? T(BATCH ? Lm->cf_value : P.cf_value) : pcf;
         // Consume the prefetched values here, where the loads are waited for
         // anyway, so that no wait inside the block's arithmetic can stall on
         // the next block's prefetch.
