@@ -81,8 +81,8 @@ class Server:
                 d.set_state(r.sc["zb"], r.sc["h"], r.sc["u"], r.sc["v"], r.sc["cf"])
                 L = Loop(nextoutputtime=r.horizon, dt=0, dtmax=0, totaltime=0)
                 d.init_dt(L)
-                st = d.get_state()
-                self.domains.append([r, d, L, st["h"].astype(np.float32), ([], []), need])
+                d.set_stats(max=True)  # the running maximum of h, every step, on the GPU
+                self.domains.append([r, d, L, None, ([], []), need])
                 self.used += need
             else:
                 try:
@@ -125,7 +125,9 @@ class Server:
             if L.totaltime >= r.horizon:
                 p = r.sc["param"]
                 mask = interior_mask(p["nblkmem"], r.sc["blocks"]["active"][:p["nblk"]])
-                r.result = dict(maxdepth=np.where(mask, rec[3], 0), gauges=np.array(rec[4][0]).T if rec[4][0] else None,
+                h0 = np.asarray(r.sc["h"], np.float32)
+                hmax = np.maximum(d.stat("hmax").astype(np.float32), h0)  # the initial state counts too
+                r.result = dict(maxdepth=np.where(mask, hmax, 0), gauges=np.array(rec[4][0]).T if rec[4][0] else None,
                                 times=np.array(rec[4][1]), totaltime=L.totaltime)
                 r.finished = now
                 d.close()
@@ -136,9 +138,8 @@ class Server:
     def _sync_domain(self, rec):
         r, d, L = rec[0], rec[1], rec[2]
         d.sync(L)
-        st = d.get_state()
-        np.maximum(rec[3], st["h"].astype(np.float32), out=rec[3])
         if r.gauges:
+            st = d.get_state()
             rec[4][0].append(st["h"][r.gauges].astype(np.float32))
             rec[4][1].append(L.totaltime)
         n = d.last_run_steps()
