@@ -250,21 +250,31 @@ struct Reader {
     std::unique_ptr<FILE, int (*)(FILE*)> fp{nullptr, fclose};
     int version = 0;
 
+    // Reads the header: the first 64 KiB of the file, more (x8) while the header runs past it.
     bool open(const char* path) {
         fp.reset(fopen(path, "rb"));
         if (!fp) return fail_("cannot open " + std::string(path));
-        std::vector<uint8_t> buf;
-        {
-            fseeko(fp.get(), 0, SEEK_END);
-            const off_t n = ftello(fp.get());
-            // the header is at most a few MB for the files handled here
-            const size_t want = (size_t)std::min<off_t>(n, (off_t)64 << 20);
-            buf.resize(want);
+        fseeko(fp.get(), 0, SEEK_END);
+        const uint64_t n = (uint64_t)ftello(fp.get());
+        uint64_t want = std::min<uint64_t>(n, 64 << 10);
+        for (;;) {
+            std::vector<uint8_t> buf(want);
             fseeko(fp.get(), 0, SEEK_SET);
-            if (fread(buf.data(), 1, want, fp.get()) != want) return fail_("short read");
+            if (want && fread(buf.data(), 1, want, fp.get()) != want) return fail_("short read");
+            S = Schema();
+            bool truncated = false;
+            if (parse(buf, truncated)) return true;
+            if (!truncated || want >= n) return false;
+            want = std::min<uint64_t>(n, want * 8);
         }
+    }
+    bool parse(const std::vector<uint8_t>& buf, bool& truncated) {
         size_t p = 0;
-        auto need = [&](size_t k) { return p + k <= buf.size(); };
+        auto need = [&](size_t k) {
+            const bool ok = p + k <= buf.size();
+            if (!ok) truncated = true;
+            return ok;
+        };
         if (!need(8) || buf[0] != 'C' || buf[1] != 'D' || buf[2] != 'F')
             return fail_("not a NetCDF classic file (HDF5-based NetCDF-4 files are not read)");
         version = buf[3];

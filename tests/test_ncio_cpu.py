@@ -214,6 +214,25 @@ def test_reader_on_scipy_files(tmp_path, version):
     assert ncio.check_var_name(path, "nothere") == ""
 
 
+def test_reader_long_header(tmp_path):
+    """A header far longer than the reader's first 64 KiB read (a 600 kB attribute) is read whole;
+    a file cut inside its header is an error, not a crash."""
+    path = str(tmp_path / "long.nc")
+    z = np.arange(12.0).reshape(3, 4)
+    with netcdf_file(path, "w", version=2) as nc:
+        nc.createDimension("y", 3)
+        nc.createDimension("x", 4)
+        nc.history = "x" * 600_000
+        vz = nc.createVariable("z", "d", ("y", "x"))
+        vz[:] = z
+        vz.units = "m"
+    np.testing.assert_array_equal(ncio.read_var(path, "z"), z)
+    cut = tmp_path / "cut.nc"
+    cut.write_bytes(open(path, "rb").read()[:300_000])
+    with pytest.raises(bgf.BgfError):
+        ncio.read_var(cut, "z")
+
+
 def test_errors(tmp_path):
     p, blocks = _uniform(nbx=1, nby=1)
     with pytest.raises(bgf.BgfError):
