@@ -73,3 +73,37 @@ def test_fp32_forms_against_fp64_oracle(case, prec):
         assert dv <= 1e-4
     else:
         assert dv <= (1e-6 if prec == "fp32_comp" else 1e-4)
+
+
+def _hump(theta, precision):
+    sc = scenarios.gaussian_hump(n=64, dx=10.0, radius=60.0, precision=precision)
+    sc["param"] = dict(sc["param"], theta=theta)
+    return sc
+
+
+@pytest.mark.parametrize("theta", [0.0, 1.0, 2.0])
+def test_fast_slope_limiter_theta(theta):
+    """The FAST kernels' limiter carries theta as a factor (minmod(d1, (d1+d2)/(2 theta), d2)
+    times theta/2, with theta <= 0 giving zero slopes): against the FP64 oracle at the same theta
+    within 2e-3 m (at theta = 1 the exact FP32 step itself is 1.2e-3 m off the FP64 one on this
+    symmetric hump: the limiter switches branches on rounding), and five times closer to it than
+    to the oracle at theta = 1.3, which lies 0.008-0.17 m away (theta takes effect)."""
+    T = 40.0
+    other = 1.3 if theta != 1.3 else 2.0
+    h32, _ = gpu_at(_hump(theta, "fp32"), "fp32_fast", T)
+    m = interior(_hump(theta, "fp32"))
+    refs = {}
+    for th in (theta, other):
+        o = oracle_for(_hump(th, "fp64"), precision="fp64")
+        L = po.OrcLoop(nextoutputtime=T, dt=0, dtmax=0, totaltime=0)
+        o.init_loop(L)
+        while L.totaltime < T:
+            o.step(L)
+            L.totaltime += L.dt
+        refs[th] = o.get("h")
+    err = np.abs(h32[m] - refs[theta][m]).max()
+    err_other = np.abs(h32[m] - refs[other][m]).max()
+    print("theta", theta, "max|dh|", err, "vs theta", other, err_other, flush=True)
+    assert np.isfinite(h32[m]).all()
+    assert err < 2e-3
+    assert err < 0.2 * err_other
