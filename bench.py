@@ -70,6 +70,32 @@ def measure(sc, precision, warmup, steps, timing_steps):
                          gcups=cells * steps / secs / 1e9, ms_per_step=secs / steps * 1e3)
 
 
+def transfer_rate(d, sc, r, precision):
+    """The host-buffer boundary's cost: one bgf_set_state (zb, h, u, v, cf from pageable host
+    arrays, as a caller hands them over) and one bgf_get_state (zs, h, u, v back), timed around
+    device synchronisations, and the rate of the timed steps with that upload and download
+    included.  The warmed state is read back first and set again, so the domain is unchanged."""
+    import numpy as np
+    T = np.float64 if precision == "fp64" else np.float32
+    st = d.get_state()
+    gpu_sync()
+    t0 = time.perf_counter()
+    st = d.get_state()
+    t1 = time.perf_counter()
+    arrs = [np.asarray(sc["zb"], T), st["h"], st["u"], st["v"], np.asarray(sc["cf"], T)]
+    gpu_sync()
+    t2 = time.perf_counter()
+    d.set_state(*arrs)
+    gpu_sync()
+    t3 = time.perf_counter()
+    up, down = t3 - t2, t1 - t0
+    return dict(upload_s=round(up, 4), download_s=round(down, 4),
+                bytes_up=int(sum(a.nbytes for a in arrs)), bytes_down=int(sum(a.nbytes for a in st.values())),
+                gcups_incl_transfer=round(r["cells"] * r["steps"] / (r["secs"] + up + down) / 1e9, 4),
+                steps=r["steps"],
+                note="value with one state upload before and one download after the timed steps")
+
+
 def cpu_baseline(d, sc, window=2048, nsteps=24):
     """Oracle FP64 on a window x window corner of the warmed GPU state (Neumann edges)."""
     import numpy as np
@@ -397,6 +423,13 @@ def main():
         except Exception as e:  # report, never hide
             cpu = dict(value=None, unit="Gcell-updates/s", cores=4, kind="port", sample="failed: %r" % (e,))
 
+    xfer = None
+    try:
+        xfer = transfer_rate(d, sc, r, args.precision)
+        log("transfer: %s" % (xfer,))
+    except Exception as e:  # report, never hide
+        xfer = dict(error=repr(e))
+
     d.close()
     nblk_main = sc["param"]["nblk"]
     del sc
@@ -439,6 +472,7 @@ def main():
                    "kernels_ms": {k: round(v, 4) for k, v in r["kt"].items()}},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "transfer": xfer,
         "variants": variants,
         "ensemble": ens,
         "serving": srv,

@@ -323,18 +323,20 @@ __device__ __forceinline__ Q4<T> slopes(const Cst<T>& C, const Q4<T>& m, const Q
 // Slopes of the tile: every thread its own cell along x and y; the halo ring
 // on wave 2, lanes 0-31 the x slopes at x = -1, 16 and lanes 32-63 the y
 // slopes at y = -1, 16.
-template <class T, bool FAST, bool DEV>
-__device__ __forceinline__ void tile_gradients(Tile<T>& s, const Cst<T>& C, int t) {
+template <class T, bool FAST, bool DEV, bool OWN>
+__device__ __forceinline__ void tile_gradients(Tile<T>& s, const Cst<T>& C, int t, const Q4<T>& q, Q4<T>& gxo,
+                                               Q4<T>& gyo) {
     {
         const int x = t & 15, y = t >> 4;
         const int c = tix(x, y);
-        const Q4<T> q0 = s.q[c];
-        s.gx[gxi(x, y)] = slopes<T, FAST, DEV>(C, s.q[c - 1], q0, s.q[c + 1]);
+        const Q4<T> q0 = OWN ? q : s.q[c];
+        gxo = slopes<T, FAST, DEV>(C, s.q[c - 1], q0, s.q[c + 1]);
+        s.gx[gxi(x, y)] = gxo;
         Q4<T> m = s.q[c - TP], p = s.q[c + TP];
         Q4<T> g = slopes<T, FAST, DEV>(C, m, q0, p);
-        Q4<T> gs;  // y slopes stored as (h, zs, v, u)
-        gs.a = g.a; gs.b = g.b; gs.c = g.d; gs.d = g.c;
-        s.gy[gyi(x, y)] = gs;
+        // y slopes stored as (h, zs, v, u)
+        gyo.a = g.a; gyo.b = g.b; gyo.c = g.d; gyo.d = g.c;
+        s.gy[gyi(x, y)] = gyo;
     }
     if ((t >> 6) == 2) {
         const int r = t & 63, j = r & 15;
@@ -349,25 +351,30 @@ __device__ __forceinline__ void tile_gradients(Tile<T>& s, const Cst<T>& C, int 
     }
 }
 
-template <class T, bool FAST, bool WANT_DT, bool DEV>
-__device__ __forceinline__ bool xface(Tile<T>& s, const Cst<T>& C, int xf, int y, T& dt) {
-    const int ci = tix(xf, y);
-    const Q4<T> qi = s.q[ci], qn = s.q[ci - 1];
-    const int gi = gxi(xf, y);
-    Q4<T> F;
-    bool wet = kurg_face<T, FAST, WANT_DT, DEV>(C, qi.a, qn.a, qi.b, qn.b, qi.c, qn.c, qi.d, qn.d, s.gx[gi], s.gx[gi - 1],
-                                           F, dt);
+// The left (bottom) face of the thread's own cell: the cell's state qi and
+// slopes si come in registers, the neighbour's from the tile; the face goes
+// to the tile for the neighbour and back to the caller in F.
+template <class T, bool FAST, bool WANT_DT, bool DEV, bool OWN>
+__device__ __forceinline__ bool xface(Tile<T>& s, const Cst<T>& C, int xf, int y, const Q4<T>& qo, const Q4<T>& so,
+                                      Q4<T>& F, T& dt) {
+    const int ci = tix(xf, y), gi = gxi(xf, y);
+    const Q4<T> qn = s.q[ci - 1];
+    const Q4<T> qi = OWN ? qo : s.q[ci];
+    const Q4<T> si = OWN ? so : s.gx[gi];
+    bool wet = kurg_face<T, FAST, WANT_DT, DEV>(C, qi.a, qn.a, qi.b, qn.b, qi.c, qn.c, qi.d, qn.d, si, s.gx[gi - 1], F,
+                                           dt);
     s.fx[fxi(xf, y)] = F;
     return wet;
 }
 
-template <class T, bool FAST, bool WANT_DT, bool DEV>
-__device__ __forceinline__ bool yface(Tile<T>& s, const Cst<T>& C, int x, int yf, T& dt) {
-    const int ci = tix(x, yf);
-    const Q4<T> qi = s.q[ci], qn = s.q[ci - TP];
-    const int gi = gyi(x, yf);
-    Q4<T> F;
-    bool wet = kurg_face<T, FAST, WANT_DT, DEV>(C, qi.a, qn.a, qi.b, qn.b, qi.d, qn.d, qi.c, qn.c, s.gy[gi], s.gy[gi - GYP],
+template <class T, bool FAST, bool WANT_DT, bool DEV, bool OWN>
+__device__ __forceinline__ bool yface(Tile<T>& s, const Cst<T>& C, int x, int yf, const Q4<T>& qo, const Q4<T>& so,
+                                      Q4<T>& F, T& dt) {
+    const int ci = tix(x, yf), gi = gyi(x, yf);
+    const Q4<T> qn = s.q[ci - TP];
+    const Q4<T> qi = OWN ? qo : s.q[ci];
+    const Q4<T> si = OWN ? so : s.gy[gi];
+    bool wet = kurg_face<T, FAST, WANT_DT, DEV>(C, qi.a, qn.a, qi.b, qn.b, qi.d, qn.d, qi.c, qn.c, si, s.gy[gi - GYP],
                                            F, dt);
     s.fy[fyi(x, yf)] = F;
     return wet;
@@ -380,16 +387,21 @@ __device__ __forceinline__ bool yface(Tile<T>& s, const Cst<T>& C, int x, int yf
 // at a right (top) wall the boundary face has no mass flux and the last
 // cell's left-face Fq is replaced by its right-face S.  Returns the cell's
 // dtmax: its X face then its Y face, a dry face storing 1/epsi outright
-// (src/Kurganov.cu:102-105, 159, 1488-1491, 1537).
-template <class T, bool FAST, bool WANT_DT, bool DEV>
-__device__ __forceinline__ T tile_fluxes(Tile<T>& s, const Cst<T>& C, int t, bool wl, bool wr, bool wb, bool wt,
-                                         T& dh, T& dhu, T& dhv) {
-    tile_gradients<T, FAST, DEV>(s, C, t);
+// (src/Kurganov.cu:102-105, 159, 1488-1491, 1537).  q is the state the thread
+// stored for its cell; that state, the cell's slopes and its left and bottom
+// faces stay in registers instead of being read back from the tile (LDS
+// bandwidth bounds these kernels about as much as VALU issue does).
+template <class T, bool FAST, bool WANT_DT, bool DEV, bool OWN = true>
+__device__ __forceinline__ T tile_fluxes(Tile<T>& s, const Cst<T>& C, int t, const Q4<T>& qo, bool wl, bool wr,
+                                         bool wb, bool wt, T& dh, T& dhu, T& dhv) {
+    Q4<T> gxo, gyo;
+    tile_gradients<T, FAST, DEV, OWN>(s, C, t, qo, gxo, gyo);
     __syncthreads();
     const int x = t & 15, y = t >> 4;
     T dtx = T(0.0), dty = T(0.0), dtd;
-    xface<T, FAST, WANT_DT, DEV>(s, C, x, y, dtx);
-    bool wy = yface<T, FAST, WANT_DT, DEV>(s, C, x, y, dty);
+    Q4<T> Fl, Fb;
+    xface<T, FAST, WANT_DT, DEV, OWN>(s, C, x, y, qo, gxo, Fl, dtx);
+    bool wy = yface<T, FAST, WANT_DT, DEV, OWN>(s, C, x, y, qo, gyo, Fb, dty);
     if (t >= 192 && t < 224) {
         // the seventeenth faces on wave 3: lanes 0-15 the X faces at xf = 16,
         // lanes 16-31 the Y faces at yf = 16
@@ -407,9 +419,12 @@ __device__ __forceinline__ T tile_fluxes(Tile<T>& s, const Cst<T>& C, int t, boo
     T dtc = T(0.0);
     if (WANT_DT) dtc = wy ? (dty < dtx ? dty : dtx) : C.inv_epsi;
     __syncthreads();
-    const Q4<T> Fl = s.fx[fxi(x, y)], Fr = s.fx[fxi(x + 1, y)];
-    const Q4<T> Fb = s.fy[fyi(x, y)], Ft = s.fy[fyi(x, y + 1)];
-    const Q4<T> q = s.q[tix(x, y)];
+    const Q4<T> Fr = s.fx[fxi(x + 1, y)], Ft = s.fy[fyi(x, y + 1)];
+    if (!OWN) {
+        Fl = s.fx[fxi(x, y)];
+        Fb = s.fy[fyi(x, y)];
+    }
+    const Q4<T> q = OWN ? qo : s.q[tix(x, y)];
     T Fhl = Fl.a, Fhr = Fr.a, Fql = Fl.b, Sr = Fr.c;
     T Fhb = Fb.a, Fht = Ft.a, Fqb = Fb.b, St = Ft.c;
     if (wl && x == 0) { Fhl = T(0.0); Sr = Fql; }
@@ -706,10 +721,10 @@ __global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuff
         fetch(cur);
     }
     for (int npos; pos < R.end; pos = npos) {
+        Q4<T> qv;
+        qv.a = ph; qv.b = DEV ? pz : pz + ph; qv.c = pu; qv.d = pv;
+        s.q[tix(x, y)] = qv;
         {
-            Q4<T> qv;
-            qv.a = ph; qv.b = DEV ? pz : pz + ph; qv.c = pu; qv.d = pv;
-            s.q[tix(x, y)] = qv;
             if (t < 128) {
                 Q4<T> r;
                 r.a = sh; r.b = DEV ? sz : sz + sh; r.c = su; r.d = sv;
@@ -733,8 +748,10 @@ __global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuff
             if (ldu(&L[n.sid].done, 0)) continue;
         }
         T dh, dhu, dhv;
-        T dtc = tile_fluxes<T, FAST, true, DEV>(s, C, t, P.btype[0] == 0 && n.nL == n.ib, P.btype[1] == 0 && n.nR == n.ib,
-                                           P.btype[2] == 0 && n.nB == n.ib, P.btype[3] == 0 && n.nT == n.ib, dh, dhu, dhv);
+        // own values in registers: measured faster for the exact kernels (-6 %), slower for FAST (+1.5 %)
+        T dtc = tile_fluxes<T, FAST, true, DEV, !FAST>(s, C, t, qv, P.btype[0] == 0 && n.nL == n.ib,
+                                                P.btype[1] == 0 && n.nR == n.ib, P.btype[2] == 0 && n.nB == n.ib,
+                                                P.btype[3] == 0 && n.nT == n.ib, dh, dhu, dhv);
         if (EXT && P.bopen && (n.nL == n.ib || n.nR == n.ib || n.nB == n.ib || n.nT == n.ib))
             // boundary values at the predictor's time, totaltime + dt/2 with the previous dt (src/FlowCPU.cu:94)
             open_sides<T>(s, C, P, t, n.ib, n.nL, n.nR, n.nB, n.nT, L->totaltime + L->dt * 0.5, B.h, B.u, B.v, B.zb, dh,
@@ -764,8 +781,8 @@ template <class T> __device__ __forceinline__ T kahan_add(T h, T d, T& c) {
 // Persistent corrector with the same schedule and prefetch.  COMP: the depth
 // update accumulates with a compensated (Kahan) sum, so the small increments
 // of rain and of slow flows are not lost to FP32 rounding of h.
-template <class T, bool FAST, bool BATCH, bool COMP, bool EXT = false>
-__global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuffers<T> B, DevLoop* L) {
+template <class T, bool FAST, bool BATCH, bool COMP, bool EXT>
+__device__ __forceinline__ void corrector_body(const KParams& P, const Cst<T>& C, const StepBuffers<T>& B, DevLoop* L) {
     if (blockIdx.x == 0 && threadIdx.x < NXCD) P.queue[threadIdx.x * QSTRIDE] = 0;  // the predictor's
     if (!BATCH && !L->go) return;
     __shared__ Tile<T> s;
@@ -822,11 +839,11 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
         // anyway, so that no wait inside the block's arithmetic can stall on
         // the next block's prefetch.
         asm volatile("" ::"v"(h0), "v"(u0), "v"(v0), "v"(zb0), "v"(cf0));
+        Q4<T> qs;
+        advance<T, FAST>(C, fmc, hdt, h0, u0, v0, pdh, pdhu, pdhv, qs.a, qs.c, qs.d);
+        qs.b = COMP ? zb0 : zb0 + qs.a;
+        s.q[tix(x, y)] = qs;
         {
-            Q4<T> q;
-            advance<T, FAST>(C, fmc, hdt, h0, u0, v0, pdh, pdhu, pdhv, q.a, q.c, q.d);
-            q.b = COMP ? zb0 : zb0 + q.a;
-            s.q[tix(x, y)] = q;
             if (t < 128) {
                 Q4<T> r;
                 advance<T, FAST>(C, fmc, hdt, sh, su, sv, sdh, sdhu, sdhv, r.a, r.c, r.d);
@@ -844,7 +861,7 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
         }
         const int ib = n.ib, nL = n.nL, nR = n.nR, nB = n.nB, nT = n.nT;
         T dh, dhu, dhv;
-        (void)tile_fluxes<T, FAST, false, COMP>(s, C, t, P.btype[0] == 0 && nL == ib, P.btype[1] == 0 && nR == ib,
+        (void)tile_fluxes<T, FAST, false, COMP>(s, C, t, qs, P.btype[0] == 0 && nL == ib, P.btype[1] == 0 && nR == ib,
                                                 P.btype[2] == 0 && nB == ib, P.btype[3] == 0 && nT == ib, dh, dhu, dhv);
         if (EXT && P.bopen && (nL == ib || nR == ib || nB == ib || nT == ib))
             // boundary values at totaltime + dt, from the step-start state (src/FlowCPU.cu:187)
@@ -918,6 +935,20 @@ __global__ __launch_bounds__(256) void k_corrector(KParams P, Cst<T> C, StepBuff
         }
     }
     queue_end(t, pend);
+}
+
+// Six resident workgroups per CU is the tile's LDS limit; asking for six waves
+// per SIMD holds the plain corrector to the 80 VGPRs that allow it (it needs a
+// few more unbounded, which would cost a sixth of its waves).  The forcing
+// kernels (open boundaries, rivers, rain maps) need more registers than that
+// and keep their own unbounded entry point.
+template <class T, bool FAST, bool BATCH, bool COMP>
+__global__ __launch_bounds__(256, 6) void k_corrector(KParams P, Cst<T> C, StepBuffers<T> B, DevLoop* L) {
+    corrector_body<T, FAST, BATCH, COMP, false>(P, C, B, L);
+}
+template <class T, bool FAST, bool BATCH, bool COMP>
+__global__ __launch_bounds__(256) void k_corrector_ext(KParams P, Cst<T> C, StepBuffers<T> B, DevLoop* L) {
+    corrector_body<T, FAST, BATCH, COMP, true>(P, C, B, L);
 }
 
 // One wave per loop state: min of the 64 block-minimum slots and 1/epsi
@@ -1210,8 +1241,14 @@ void launch_step_t(const KParams& P, const StepBuffers<T>& B, DevLoop* L, int ad
     if (ev) (void)hipEventRecord(ev[1], st);
     hipLaunchKernelGGL((k_reduce_dt<T>), dim3(BATCH ? P.nmember : 1), dim3(64), 0, st, L, advance_time);
     if (ev) (void)hipEventRecord(ev[2], st);
-    if (ext) hipLaunchKernelGGL((k_corrector<T, FAST, BATCH, COMP, EXT_OK>), dim3(P.grid_corr), dim3(256), 0, st, P, C, B, L);
-    else hipLaunchKernelGGL((k_corrector<T, FAST, BATCH, COMP>), dim3(P.grid_corr), dim3(256), 0, st, P, C, B, L);
+    bool done = false;
+    if constexpr (EXT_OK) {
+        if (ext) {
+            hipLaunchKernelGGL((k_corrector_ext<T, FAST, BATCH, COMP>), dim3(P.grid_corr), dim3(256), 0, st, P, C, B, L);
+            done = true;
+        }
+    }
+    if (!done) hipLaunchKernelGGL((k_corrector<T, FAST, BATCH, COMP>), dim3(P.grid_corr), dim3(256), 0, st, P, C, B, L);
     if (ev) (void)hipEventRecord(ev[3], st);
 }
 
