@@ -367,8 +367,9 @@ def main():
         copy_probe()
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU sharding is described in DESIGN.md "
-              "and not launched (requested --gpus %d)" % args.gpus, file=sys.stderr)
+        print("bench.py: bg_flood_amd runs one domain (or one batch) per GPU; sharding a domain over several "
+              "GPUs (contiguous ranges of the processing order, halo strips exchanged per stage) is not built, "
+              "so only --gpus 1 runs (requested --gpus %d)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     if args.ensemble_only:
