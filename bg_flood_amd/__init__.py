@@ -449,7 +449,9 @@ class Domain:
 
     def device_array(self, name):
         """Device address of one of the handle's arrays ("h", "u", "v", "zb", "cf", "dh", "dhu",
-        "dhv"), for callers that keep their own views (bgf_device_array)."""
+        "dhv"), for callers that keep their own views (bgf_device_array).  State arrays are in the
+        block layout (324 values per block); on one-level meshes the tendencies hold each block's
+        16x16 interior (256 values per block) and are overwritten each step."""
         p = ctypes.c_void_p()
         _check(lib().bgf_device_array(self.h, name.encode(), ctypes.byref(p)))
         return p.value

@@ -10,6 +10,7 @@ namespace bgf {
 constexpr int BW = 16;      // cells per block side (src/Param.h:52)
 constexpr int MW = 18;      // stored tile side: one halo cell each way
 constexpr int BS = MW * MW; // values per block
+constexpr int BSI = BW * BW; // values per block of the interior-only tendency arrays
 
 // src/MemManagement.cu:37-40
 __device__ __forceinline__ int memloc(int i, int j, int ib) { return (i + 1) + (j + 1) * MW + ib * BS; }

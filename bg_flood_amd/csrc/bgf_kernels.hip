@@ -265,17 +265,32 @@ __device__ __forceinline__ int strip_cell(int k) {
     const int j = k & 15, o = strip_off(k);
     return k < 64 ? tix(o, j) : tix(j, o);
 }
-__device__ __forceinline__ int strip_src(int k, int ib, int nL, int nR, int nB, int nT) {
+struct StripCell {
+    int x, y, nb;
+};
+__device__ __forceinline__ StripCell strip_cell_src(int k, int ib, int nL, int nR, int nB, int nT) {
     const int j = k & 15, o = strip_off(k);
     const bool lo = o < 0;
+    StripCell c;
     if (k < 64) {
-        const int nb = lo ? nL : nR;
-        const int sx = nb == ib ? (lo ? 0 : BW - 1) : (lo ? BW + o : o - BW);
-        return memloc(sx, j, nb);
+        c.nb = lo ? nL : nR;
+        c.x = c.nb == ib ? (lo ? 0 : BW - 1) : (lo ? BW + o : o - BW);
+        c.y = j;
+    } else {
+        c.nb = lo ? nB : nT;
+        c.x = j;
+        c.y = c.nb == ib ? (lo ? 0 : BW - 1) : (lo ? BW + o : o - BW);
     }
-    const int nb = lo ? nB : nT;
-    const int sy = nb == ib ? (lo ? 0 : BW - 1) : (lo ? BW + o : o - BW);
-    return memloc(j, sy, nb);
+    return c;
+}
+__device__ __forceinline__ int strip_src(int k, int ib, int nL, int nR, int nB, int nT) {
+    const StripCell c = strip_cell_src(k, ib, nL, nR, nB, nT);
+    return memloc(c.x, c.y, c.nb);
+}
+// the same source cell in the interior-only layout of the tendency arrays
+__device__ __forceinline__ int strip_src_i(int k, int ib, int nL, int nR, int nB, int nT) {
+    const StripCell c = strip_cell_src(k, ib, nL, nR, nB, nT);
+    return c.nb * BSI + c.y * BW + c.x;
 }
 
 // FAST minmod of three: minmod(d1, d2, d3) = med3(d1, med3(d2, d3, 0), 0)
@@ -566,6 +581,22 @@ template <class T> __device__ __forceinline__ T ldn(const T* a, int ib, unsigned
     return __builtin_nontemporal_load((const __attribute__((address_space(1))) T*)(blk(a, ib) + vof(kb)));
 }
 
+// Tendency arrays (dh, dhu, dhv) hold each block's 16x16 interior only, 256
+// values per block in row order, written by the predictor and read by the
+// corrector: a block's rows are contiguous aligned 64-byte segments, where
+// the 18-value pitch of the state arrays splits each row over two.
+template <class T> __device__ __forceinline__ gchar* blk_i(const T* a, int ib) {
+    const T* p = a + (size_t)(unsigned)ib * BSI;
+    asm("" : "+s"(p));
+    return (gchar*)p;
+}
+template <class T> __device__ __forceinline__ T ldc_i(const T* a, int ib, unsigned kb) {
+    return *(const __attribute__((address_space(1))) T*)(blk_i(a, ib) + vof(kb));
+}
+template <class T> __device__ __forceinline__ void stn_i(T* a, int ib, unsigned kb, T v) {
+    __builtin_nontemporal_store(v, (__attribute__((address_space(1))) T*)(blk_i(a, ib) + vof(kb)));
+}
+
 struct Nb {
     int ib, nL, nR, nB, nT, sid;
 };
@@ -703,6 +734,7 @@ __global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuff
     int csid = -1;
     T ph = T(0), pz = T(0), pu = T(0), pv = T(0), sh = T(0), sz = T(0), su = T(0), sv = T(0);
     const unsigned kb = (unsigned)memloc(x, y, 0) * sizeof(T);
+    const unsigned ki = (unsigned)(y * BW + x) * sizeof(T);  // in the tendency arrays
     auto fetch = [&](const Nb& n) {
         ph = ldc(B.h, n.ib, kb); pz = ldc(B.zb, n.ib, kb); pu = ldc(B.u, n.ib, kb); pv = ldc(B.v, n.ib, kb);
         if (t < 128) {
@@ -757,9 +789,9 @@ __global__ __launch_bounds__(256) void k_predictor(KParams P, Cst<T> C, StepBuff
             open_sides<T>(s, C, P, t, n.ib, n.nL, n.nR, n.nB, n.nT, L->totaltime + L->dt * 0.5, B.h, B.u, B.v, B.zb, dh,
                           dhu, dhv);
         if (EXT && P.nriver) dh = river_inflow<T>(P, n.ib, x, y, dh);
-        stn(B.dh, n.ib, kb, dh);
-        stn(B.dhu, n.ib, kb, dhu);
-        stn(B.dhv, n.ib, kb, dhv);
+        stn_i(B.dh, n.ib, ki, dh);
+        stn_i(B.dhu, n.ib, ki, dhu);
+        stn_i(B.dhv, n.ib, ki, dhv);
         bmin = dtc < bmin ? dtc : bmin;
     }
     if (BATCH) {
@@ -796,15 +828,17 @@ __device__ __forceinline__ void corrector_body(const KParams& P, const Cst<T>& C
     T ph = T(0), pu = T(0), pv = T(0), pdh = T(0), pdhu = T(0), pdhv = T(0), pz = T(0), pcf = T(0);
     T sh = T(0), su = T(0), sv = T(0), sdh = T(0), sdhu = T(0), sdhv = T(0), sz = T(0);
     const unsigned kb = (unsigned)memloc(x, y, 0) * sizeof(T);
+    const unsigned ki = (unsigned)(y * BW + x) * sizeof(T);  // in the tendency arrays
     auto fetch = [&](const Nb& n) {
         const int ib = n.ib;
         ph = ldc(B.h, ib, kb); pu = ldc(B.u, ib, kb); pv = ldc(B.v, ib, kb);
-        pdh = ldc(B.dh, ib, kb); pdhu = ldc(B.dhu, ib, kb); pdhv = ldc(B.dhv, ib, kb);
+        pdh = ldc_i(B.dh, ib, ki); pdhu = ldc_i(B.dhu, ib, ki); pdhv = ldc_i(B.dhv, ib, ki);
         pz = ldc(B.zb, ib, kb);
         if (!(BATCH ? ldu(&L[n.sid].cf_uniform, 0) : P.cf_uniform)) pcf = ldn(B.cf, ib, kb);
         if (t < 128) {
             const int src = strip_src(t, n.ib, n.nL, n.nR, n.nB, n.nT);
-            sh = B.h[src]; su = B.u[src]; sv = B.v[src]; sdh = B.dh[src]; sdhu = B.dhu[src]; sdhv = B.dhv[src];
+            const int sri = strip_src_i(t, n.ib, n.nL, n.nR, n.nB, n.nT);
+            sh = B.h[src]; su = B.u[src]; sv = B.v[src]; sdh = B.dh[sri]; sdhu = B.dhu[sri]; sdhv = B.dhv[sri];
             sz = B.zb[src];
         }
     };

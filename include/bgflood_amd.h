@@ -144,7 +144,10 @@ bgf_status bgf_last_run_steps(bgf_handle h, int* nsteps);
 bgf_status bgf_volume(bgf_handle h, double* volume);
 
 /* Device pointers of the handle's Model arrays by name ("h", "u", "v", "zb",
- * "cf", "dh", "dhu", "dhv"), for callers that keep Model<T> views. */
+ * "cf", "dh", "dhu", "dhv"), for callers that keep Model<T> views.  The state
+ * arrays use the block layout (324 values per block, halo included); on
+ * one-level meshes the tendencies dh, dhu, dhv hold each block's 16x16
+ * interior only (256 values per block, row y at 16*y) and live within a step. */
 bgf_status bgf_device_array(bgf_handle h, const char* name, void** devptr);
 
 /* Launch schedule of the persistent one-level kernels (no counterpart in the
