@@ -380,7 +380,8 @@ bgf_status bgf_batch_add(bgf_batch b, const bgf_param* p, const bgf_blocks* blk,
     if (hipMemset((char*)b->d_hmax + sizeof(float) * (size_t)off * BS, 0, sizeof(float) * (size_t)nbm * BS) !=
         hipSuccess)
         return undo(fail(BGF_ERR_HIP, "max-depth reset failed"));
-    if (b->d_ch && hipMemset((char*)b->d_ch + o, 0, bytes) != hipSuccess)
+    // the compensation array holds each block's interior only (BSI values per block)
+    if (b->d_ch && hipMemset((char*)b->d_ch + b->es * (size_t)off * BSI, 0, b->es * (size_t)nbm * BSI) != hipSuccess)
         return undo(fail(BGF_ERR_HIP, "compensation reset failed"));
     s = rebuild(b);
     if (s != BGF_OK) return undo(s);

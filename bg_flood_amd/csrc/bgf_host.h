@@ -18,6 +18,7 @@ namespace bgf_host {
 constexpr int BW = 16;
 constexpr int MW = 18;
 constexpr int BS = MW * MW;
+constexpr int BSI = BW * BW;  // values per block of the interior-only arrays (tendencies, compensation)
 
 inline thread_local std::string g_err;
 

@@ -581,9 +581,10 @@ template <class T> __device__ __forceinline__ T ldn(const T* a, int ib, unsigned
     return __builtin_nontemporal_load((const __attribute__((address_space(1))) T*)(blk(a, ib) + vof(kb)));
 }
 
-// Tendency arrays (dh, dhu, dhv) hold each block's 16x16 interior only, 256
-// values per block in row order, written by the predictor and read by the
-// corrector: a block's rows are contiguous aligned 64-byte segments, where
+// Tendency arrays (dh, dhu, dhv) and the compensation of h (COMP) hold each
+// block's 16x16 interior only, 256 values per block in row order (the
+// tendencies written by the predictor and read by the corrector, the
+// compensation read and written by the corrector): a block's rows are contiguous aligned 64-byte segments, where
 // the 18-value pitch of the state arrays splits each row over two.
 template <class T> __device__ __forceinline__ gchar* blk_i(const T* a, int ib) {
     const T* p = a + (size_t)(unsigned)ib * BSI;
@@ -592,6 +593,9 @@ template <class T> __device__ __forceinline__ gchar* blk_i(const T* a, int ib) {
 }
 template <class T> __device__ __forceinline__ T ldc_i(const T* a, int ib, unsigned kb) {
     return *(const __attribute__((address_space(1))) T*)(blk_i(a, ib) + vof(kb));
+}
+template <class T> __device__ __forceinline__ void stc_i(T* a, int ib, unsigned kb, T v) {
+    *(__attribute__((address_space(1))) T*)(blk_i(a, ib) + vof(kb)) = v;
 }
 template <class T> __device__ __forceinline__ void stn_i(T* a, int ib, unsigned kb, T v) {
     __builtin_nontemporal_store(v, (__attribute__((address_space(1))) T*)(blk_i(a, ib) + vof(kb)));
@@ -902,7 +906,7 @@ __device__ __forceinline__ void corrector_body(const KParams& P, const Cst<T>& C
             open_sides<T>(s, C, P, t, ib, nL, nR, nB, nT, L->tcorr, B.h, B.u, B.v, B.zb, dh, dhu, dhv);
         if (EXT && P.nriver) dh = river_inflow<T>(P, ib, x, y, dh);
         T h1, u1, v1;
-        T comp = COMP ? ldc(B.ch, ib, kb) : T(0);
+        T comp = COMP ? ldc_i(B.ch, ib, ki) : T(0);
         if (COMP) {
             // advance() with the depth increment added through the compensated sum
             T edt = dt;
@@ -961,7 +965,7 @@ __device__ __forceinline__ void corrector_body(const KParams& P, const Cst<T>& C
         stn(B.ho, ib, kb, h1);
         stn(B.uo, ib, kb, u1);
         stn(B.vo, ib, kb, v1);
-        if (COMP) stc(B.ch, ib, kb, comp);
+        if (COMP) stc_i(B.ch, ib, ki, comp);
         if (P.hmax) {
             const float hv = (float)h1;
             const unsigned kf = (unsigned)memloc(x, y, 0) * 4u;
